@@ -1,0 +1,1455 @@
+/* Oracle hot path: one NS_Time_advance_LU_SGS iteration in reference operation order.
+ * TEST INFRASTRUCTURE ONLY (see orc_internal.h). */
+#include "orc_internal.h"
+#include "ocfd_oracle.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SQ(a) ((a) * (a))
+
+/* ------------------------------------------------------------------------------------------
+ * reconstruction schemes: scheme_fP / scheme_fm (sub_scheme.f90:6-96), u[] holds u(-3..4) at u[k+3]
+ */
+static double minmod(double a, double b) {          /* sub_scheme.f90:281-292 */
+    if (a * b <= 0.0) return 0.0;
+    if (fabs(a) <= fabs(b)) return a;
+    return b;
+}
+
+static double weno5_L(const double *u) {            /* sub_scheme.f90:101-119 */
+    const double ep = 1.0e-6, C03 = 3.0 / 10.0, C13 = 3.0 / 5.0, C23 = 1.0 / 10.0;
+    const double c1312 = 13.0 / 12.0, c14 = 1.0 / 4.0;
+    double S0 = c1312 * SQ((u[3] - 2.0 * u[4]) + u[5]) + c14 * SQ((3.0 * u[3] - 4.0 * u[4]) + u[5]);
+    double S1 = c1312 * SQ((u[2] - 2.0 * u[3]) + u[4]) + c14 * SQ(u[2] - u[4]);
+    double S2 = c1312 * SQ((u[1] - 2.0 * u[2]) + u[3]) + c14 * SQ((u[1] - 4.0 * u[2]) + 3.0 * u[3]);
+    double a0 = C03 / SQ(ep + S0), a1 = C13 / SQ(ep + S1), a2 = C23 / SQ(ep + S2);
+    double am = (a0 + a1) + a2;
+    double q03 = ((1.0 / 3.0) * u[3] + (5.0 / 6.0) * u[4]) - (1.0 / 6.0) * u[5];
+    double q13 = (-(1.0 / 6.0) * u[2] + (5.0 / 6.0) * u[3]) + (1.0 / 3.0) * u[4];
+    double q23 = ((1.0 / 3.0) * u[1] - (7.0 / 6.0) * u[2]) + (11.0 / 6.0) * u[3];
+    return ((a0 * q03 + a1 * q13) + a2 * q23) / am;
+}
+
+static double weno5_R(const double *u) {            /* sub_scheme.f90:122-140 */
+    const double ep = 1.0e-6, C03 = 3.0 / 10.0, C13 = 3.0 / 5.0, C23 = 1.0 / 10.0;
+    const double c1312 = 13.0 / 12.0, c14 = 1.0 / 4.0;
+    double S0 = c1312 * SQ((u[4] - 2.0 * u[3]) + u[2]) + c14 * SQ((3.0 * u[4] - 4.0 * u[3]) + u[2]);
+    double S1 = c1312 * SQ((u[5] - 2.0 * u[4]) + u[3]) + c14 * SQ(u[5] - u[3]);
+    double S2 = c1312 * SQ((u[6] - 2.0 * u[5]) + u[4]) + c14 * SQ((u[6] - 4.0 * u[5]) + 3.0 * u[4]);
+    double a0 = C03 / SQ(ep + S0), a1 = C13 / SQ(ep + S1), a2 = C23 / SQ(ep + S2);
+    double am = (a0 + a1) + a2;
+    double q03 = ((1.0 / 3.0) * u[4] + (5.0 / 6.0) * u[3]) - (1.0 / 6.0) * u[2];
+    double q13 = (-(1.0 / 6.0) * u[5] + (5.0 / 6.0) * u[4]) + (1.0 / 3.0) * u[3];
+    double q23 = ((1.0 / 3.0) * u[6] - (7.0 / 6.0) * u[5]) + (11.0 / 6.0) * u[4];
+    return ((a0 * q03 + a1 * q13) + a2 * q23) / am;
+}
+
+static double scheme_fP(const double *u, int scheme) {
+    const double k2c = 1.0, k2u = -1.0, b2 = 2.0, k3 = 1.0 / 3.0, ep = 1.0e-6;
+    const double eta = (double)0.05f;
+    const double *U = u + 3;                           /* U[k] = u(k) */
+    switch (scheme) {
+    case S_CD2: return 0.5 * (U[0] + U[1]);
+    case S_UD1: return U[0];
+    case S_UD3: return ((-U[-1] + 5.0 * U[0]) + 2.0 * U[1]) / 6.0;
+    case S_NND2: return U[0] + 0.5 * minmod(U[0] - U[-1], U[1] - U[0]);
+    case S_MUSCL2U:
+        return U[0] + 0.25 * ((1.0 - k2u) * minmod(U[0] - U[-1], b2 * (U[1] - U[0])) + (1.0 + k2u) * minmod(U[1] - U[0], b2 * (U[0] - U[-1])));
+    case S_MUSCL2C:
+        return U[0] + 0.25 * ((1.0 - k2c) * minmod(U[0] - U[-1], b2 * (U[1] - U[0])) + (1.0 + k2c) * minmod(U[1] - U[0], b2 * (U[0] - U[-1])));
+    case S_MUSCL3: {
+        double up = U[1] - U[0], um = U[0] - U[-1];
+        double s = (2.0 * up * um + ep) / ((up * up + um * um) + ep);
+        return U[0] + (0.25 * s) * ((1.0 - k3 * s) * um + (1.0 + k3 * s) * up);
+    }
+    case S_OMUSCL2: {
+        double r1 = ((U[0] - U[-1]) + ep) / ((U[1] - U[0]) + ep), r2 = ((U[1] - U[0]) + ep) / ((U[2] - U[1]) + ep);
+        double f1 = ((1.0 - eta) + ((eta - 0.55) * 0.5) / r2) + ((eta + 0.55) * 0.5) * r1;
+        double mn = 2.0; if (f1 < mn) mn = f1; if (2.0 * r1 < mn) mn = 2.0 * r1;
+        double f = mn > 0.0 ? mn : 0.0;
+        return U[0] + (0.5 * f) * (U[1] - U[0]);
+    }
+    case S_WENO5: return weno5_L(u);
+    case S_UD5: return ((((2.0 * U[-2] - 13.0 * U[-1]) + 47.0 * U[0]) + 27.0 * U[1]) - 3.0 * U[2]) / 60.0;
+    default: fprintf(stderr, "oracle: scheme %d not supported\n", scheme); abort();
+    }
+}
+
+static double scheme_fm(const double *u, int scheme) {
+    const double k2c = 1.0, k2u = -1.0, b2 = 2.0, k3 = 1.0 / 3.0, ep = 1.0e-6;
+    const double eta = (double)0.05f;
+    const double *U = u + 3;
+    switch (scheme) {
+    case S_CD2: return 0.5 * (U[0] + U[1]);
+    case S_UD1: return U[1];
+    case S_UD3: return ((2.0 * U[0] + 5.0 * U[1]) - U[2]) / 6.0;
+    case S_NND2: return U[1] - 0.5 * minmod(U[1] - U[0], U[2] - U[1]);
+    case S_MUSCL2U:
+        return U[1] - 0.25 * ((1.0 - k2u) * minmod(U[2] - U[1], b2 * (U[1] - U[0])) + (1.0 + k2u) * minmod(U[1] - U[0], b2 * (U[2] - U[1])));
+    case S_MUSCL2C:
+        return U[1] - 0.25 * ((1.0 - k2c) * minmod(U[2] - U[1], b2 * (U[1] - U[0])) + (1.0 + k2c) * minmod(U[1] - U[0], b2 * (U[2] - U[1])));
+    case S_MUSCL3: {
+        double up = U[2] - U[1], um = U[1] - U[0];
+        double s = (2.0 * up * um + ep) / ((up * up + um * um) + ep);
+        return U[1] - (0.25 * s) * ((1.0 - k3 * s) * up + (1.0 + k3 * s) * um);
+    }
+    case S_OMUSCL2: {
+        double r1 = ((U[2] - U[1]) + ep) / ((U[1] - U[0]) + ep), r2 = ((U[1] - U[0]) + ep) / ((U[0] - U[-1]) + ep);
+        double f1 = ((1.0 - eta) + ((eta - 0.55) * 0.5) / r2) + ((eta + 0.55) * 0.5) * r1;
+        double mn = 2.0; if (f1 < mn) mn = f1; if (2.0 * r1 < mn) mn = 2.0 * r1;
+        double f = mn > 0.0 ? mn : 0.0;
+        return U[1] - (0.5 * f) * (U[1] - U[0]);
+    }
+    case S_WENO5: return weno5_R(u);
+    case S_UD5: return ((((-3.0 * U[-1] + 27.0 * U[0]) + 47.0 * U[1]) - 13.0 * U[2]) + 2.0 * U[3]) / 60.0;
+    default: fprintf(stderr, "oracle: scheme %d not supported\n", scheme); abort();
+    }
+}
+
+/* Reconstuction_original / _conservative / _Characteristic (sub_Residual.f90:1258-1347).
+ * U0[m][k+3], k=-3..4 */
+static void reconstruct(double U0[5][8], double *UL, double *UR, double gamma, int scheme, int recon) {
+    if (recon == 0) {
+        for (int m = 0; m < 5; m++) { UL[m] = scheme_fP(U0[m], scheme); UR[m] = scheme_fm(U0[m], scheme); }
+        return;
+    }
+    double QL[5], QR[5];
+    if (recon == 1) {
+        for (int m = 0; m < 5; m++) { QL[m] = scheme_fP(U0[m], scheme); QR[m] = scheme_fm(U0[m], scheme); }
+    } else {
+        double Uh[5];
+        for (int m = 0; m < 5; m++) Uh[m] = 0.5 * (U0[m][3] + U0[m][4]);
+        double d1 = Uh[0], u1 = Uh[1] / d1, v1 = Uh[2] / d1, w1 = Uh[3] / d1;
+        double p1 = (Uh[4] - ((Uh[1] * u1 + Uh[2] * v1) + Uh[3] * w1) * 0.5) * (gamma - 1.0);
+        double c1 = sqrt(gamma * p1 / d1);
+        double V2 = ((u1 * u1 + v1 * v1) + w1 * w1) * 0.5;
+        double tmp1 = (gamma - 1.0) / c1, tmp3 = (gamma - 1.0) / (c1 * c1), tmp5 = 1.0 / (2.0 * c1), tmp0 = 1.0 / tmp3;
+        double S[5][5] = {
+            {V2 - tmp0, -u1, -v1, -w1, 1.0},
+            {-v1, 0.0, 1.0, 0.0, 0.0},
+            {-w1, 0.0, 0.0, 1.0, 0.0},
+            {-u1 - V2 * tmp1, 1.0 + tmp1 * u1, tmp1 * v1, tmp1 * w1, -tmp1},
+            {-u1 + V2 * tmp1, 1.0 - tmp1 * u1, -tmp1 * v1, -tmp1 * w1, tmp1}};
+        double S1[5][5] = {
+            {-tmp3, 0.0, 0.0, -tmp5, tmp5},
+            {-tmp3 * u1, 0.0, 0.0, 0.5 - u1 * tmp5, 0.5 + u1 * tmp5},
+            {-tmp3 * v1, 1.0, 0.0, -v1 * tmp5, v1 * tmp5},
+            {-tmp3 * w1, 0.0, 1.0, -w1 * tmp5, w1 * tmp5},
+            {-tmp3 * V2, v1, w1, ((c1 * u1 - V2) - tmp0) * tmp5, ((c1 * u1 + V2) + tmp0) * tmp5}};
+        double V0[5][8], VL[5], VR[5];
+        for (int k = 0; k < 8; k++)
+            for (int m = 0; m < 5; m++) {
+                double a = 0.0;
+                for (int j = 0; j < 5; j++) a = a + S[m][j] * U0[j][k];
+                V0[m][k] = a;
+            }
+        for (int m = 0; m < 5; m++) { VL[m] = scheme_fP(V0[m], scheme); VR[m] = scheme_fm(V0[m], scheme); }
+        for (int m = 0; m < 5; m++) {
+            double a = 0.0, bq = 0.0;
+            for (int j = 0; j < 5; j++) { a = a + S1[m][j] * VL[j]; bq = bq + S1[m][j] * VR[j]; }
+            QL[m] = a; QR[m] = bq;
+        }
+    }
+    UL[0] = QL[0]; UL[1] = QL[1] / UL[0]; UL[2] = QL[2] / UL[0]; UL[3] = QL[3] / UL[0];
+    UL[4] = (QL[4] - ((UL[1] * QL[1] + UL[2] * QL[2]) + UL[3] * QL[3]) * 0.5) * (gamma - 1.0);
+    UR[0] = QR[0]; UR[1] = QR[1] / UR[0]; UR[2] = QR[2] / UR[0]; UR[3] = QR[3] / UR[0];
+    UR[4] = (QR[4] - ((UR[1] * QR[1] + UR[2] * QR[2]) + UR[3] * QR[3]) * 0.5) * (gamma - 1.0);
+}
+
+/* ------------------------------------------------------------------------------------------
+ * 1-D fluxes in the face frame (sub_flux_split.f90)
+ */
+static void flux_hll_hllc(const double *QL, const double *QR, double *F, double gamma, int iflux) {   /* :13-95 */
+    double dl = QL[0], uul = QL[1], vvl = QL[2], wwl = QL[3], pl = QL[4];
+    double dr = QR[0], uur = QR[1], vvr = QR[2], wwr = QR[3], pr = QR[4];
+    double UL[5], UR[5], Fl[5], Fr[5];
+    UL[0] = dl; UL[1] = dl * uul; UL[2] = dl * vvl; UL[3] = dl * wwl;
+    UL[4] = pl / (gamma - 1.0) + (dl * ((uul * uul + vvl * vvl) + wwl * wwl)) * 0.5;
+    UR[0] = dr; UR[1] = dr * uur; UR[2] = dr * vvr; UR[3] = dr * wwr;
+    UR[4] = pr / (gamma - 1.0) + (dr * ((uur * uur + vvr * vvr) + wwr * wwr)) * 0.5;
+    double al = sqrt(gamma * pl / dl), ar = sqrt(gamma * pr / dr);
+    double p_pvrs = 0.5 * (pl + pr) - (((uur - uul) * (dl + dr)) * (al + ar)) * 0.125;
+    double p_star = p_pvrs > 0.0 ? p_pvrs : 0.0;
+    double qql, qqr;
+    if (p_star <= pl) qql = 1.0; else qql = sqrt(1.0 + ((gamma + 1.0) / (2.0 * gamma)) * (p_star / pl - 1.0));
+    if (p_star <= pr) qqr = 1.0; else qqr = sqrt(1.0 + ((gamma + 1.0) / (2.0 * gamma)) * (p_star / pr - 1.0));
+    double Sl = uul - al * qql, Sr = uur + ar * qqr;
+    Fl[0] = UL[1]; Fl[1] = UL[1] * uul + pl; Fl[2] = UL[2] * uul; Fl[3] = UL[3] * uul; Fl[4] = uul * (UL[4] + pl);
+    Fr[0] = UR[1]; Fr[1] = UR[1] * uur + pr; Fr[2] = UR[2] * uur; Fr[3] = UR[3] * uur; Fr[4] = uur * (UR[4] + pr);
+    if (iflux == F_HLL) {
+        if (Sl >= 0) { for (int m = 0; m < 5; m++) F[m] = Fl[m]; }
+        else if (Sr <= 0) { for (int m = 0; m < 5; m++) F[m] = Fr[m]; }
+        else { for (int m = 0; m < 5; m++) F[m] = ((Sr * Fl[m] - Sl * Fr[m]) + (Sl * Sr) * (UR[m] - UL[m])) / (Sr - Sl); }
+        return;
+    }
+    double S_star = ((pr - pl + (dl * uul) * (Sl - uul)) - (dr * uur) * (Sr - uur)) / (dl * (Sl - uul) - dr * (Sr - uur));
+    double tmpl = dl * (Sl - uul) / (Sl - S_star), tmpr = dr * (Sr - uur) / (Sr - S_star);
+    double Fsl[5], Fsr[5], tmp;
+    Fsl[0] = Fl[0] + Sl * (tmpl - UL[0]);
+    Fsl[1] = Fl[1] + Sl * (tmpl * S_star - UL[1]);
+    Fsl[2] = Fl[2] + Sl * (tmpl * vvl - UL[2]);
+    Fsl[3] = Fl[3] + Sl * (tmpl * wwl - UL[3]);
+    tmp = UL[4] / dl + (S_star - uul) * (S_star + pl / (dl * (Sl - uul)));
+    Fsl[4] = Fl[4] + Sl * (tmpl * tmp - UL[4]);
+    Fsr[0] = Fr[0] + Sr * (tmpr - UR[0]);
+    Fsr[1] = Fr[1] + Sr * (tmpr * S_star - UR[1]);
+    Fsr[2] = Fr[2] + Sr * (tmpr * vvr - UR[2]);
+    Fsr[3] = Fr[3] + Sr * (tmpr * vvr - UR[3]);          /* vvr, as in sub_flux_split.f90:67 */
+    tmp = UR[4] / dr + (S_star - uur) * (S_star + pr / (dr * (Sr - uur)));
+    Fsr[4] = Fr[4] + Sr * (tmpr * tmp - UR[4]);
+    (void)wwr;
+    const double *src = (Sl >= 0) ? Fl : (Sr <= 0) ? Fr : (S_star >= 0) ? Fsl : Fsr;
+    for (int m = 0; m < 5; m++) F[m] = src[m];
+}
+
+static void flux_sw(const double *QL, const double *QR, double *F, double gamma) {   /* :102-141 */
+    double dl = QL[0], uul = QL[1], vvl = QL[2], wwl = QL[3], pl = QL[4];
+    double dr = QR[0], uur = QR[1], vvr = QR[2], wwr = QR[3], pr = QR[4];
+    double al = sqrt(gamma * pl / dl), ar = sqrt(gamma * pr / dr);
+    double tmp1 = 2.0 * (gamma - 1.0), tmp3 = (3.0 - gamma) / (2.0 * (gamma - 1.0));
+    double E1P = (uul + fabs(uul)) * 0.5, E2P = ((uul - al) + fabs(uul - al)) * 0.5, E3P = ((uul + al) + fabs(uul + al)) * 0.5;
+    double tmp0 = dl / (2.0 * gamma), fp[5], fm[5];
+    fp[0] = tmp0 * ((tmp1 * E1P + E2P) + E3P);
+    fp[1] = tmp0 * (((tmp1 * E1P) * uul + E2P * (uul - al)) + E3P * (uul + al));
+    fp[2] = tmp0 * (((tmp1 * E1P) * vvl + E2P * vvl) + E3P * vvl);
+    fp[3] = tmp0 * (((tmp1 * E1P) * wwl + E2P * wwl) + E3P * wwl);
+    fp[4] = tmp0 * ((((E1P * (gamma - 1.0)) * ((uul * uul + vvl * vvl) + wwl * wwl) + (E2P * ((SQ(uul - al) + vvl * vvl) + wwl * wwl)) * 0.5)
+                     + (E3P * ((SQ(uul + al) + vvl * vvl) + wwl * wwl)) * 0.5) + ((tmp3 * al) * al) * (E2P + E3P));
+    double E1M = (uur - fabs(uur)) * 0.5, E2M = ((uur - ar) - fabs(uur - ar)) * 0.5, E3M = ((uur + ar) - fabs(uur + ar)) * 0.5;
+    tmp0 = dr / (2.0 * gamma);
+    fm[0] = tmp0 * ((tmp1 * E1M + E2M) + E3M);
+    fm[1] = tmp0 * (((tmp1 * E1M) * uur + E2M * (uur - ar)) + E3M * (uur + ar));
+    fm[2] = tmp0 * (((tmp1 * E1M) * vvr + E2M * vvr) + E3M * vvr);
+    fm[3] = tmp0 * (((tmp1 * E1M) * wwr + E2M * wwr) + E3M * wwr);
+    fm[4] = tmp0 * ((((E1M * (gamma - 1.0)) * ((uur * uur + vvr * vvr) + wwr * wwr) + (E2M * ((SQ(uur - ar) + vvr * vvr) + wwr * wwr)) * 0.5)
+                     + (E3M * ((SQ(uur + ar) + vvr * vvr) + wwr * wwr)) * 0.5) + ((tmp3 * ar) * ar) * (E2M + E3M));
+    for (int m = 0; m < 5; m++) F[m] = fp[m] + fm[m];
+}
+
+static void flux_roe(const double *QL, const double *QR, double *F, double gamma) {   /* :148-210 */
+    const double delt = 0.1;
+    const double eps8 = (double)1.e-8f;               /* single-precision literal 1.e-8 (:195,197) */
+    double dl = QL[0], uul = QL[1], vvl = QL[2], wwl = QL[3], pl = QL[4];
+    double dr = QR[0], uur = QR[1], vvr = QR[2], wwr = QR[3], pr = QR[4];
+    double hl = gamma * pl / ((gamma - 1.0) * dl) + 0.5 * ((uul * uul + vvl * vvl) + wwl * wwl);
+    double hr = gamma * pr / ((gamma - 1.0) * dr) + 0.5 * ((uur * uur + vvr * vvr) + wwr * wwr);
+    double UL[5], UR[5], Fl[5], Fr[5], arr[5], lam[3];
+    UL[0] = dl; UL[1] = dl * uul; UL[2] = dl * vvl; UL[3] = dl * wwl;
+    UL[4] = pl / (gamma - 1.0) + (dl * ((uul * uul + vvl * vvl) + wwl * wwl)) * 0.5;
+    UR[0] = dr; UR[1] = dr * uur; UR[2] = dr * vvr; UR[3] = dr * wwr;
+    UR[4] = pr / (gamma - 1.0) + (dr * ((uur * uur + vvr * vvr) + wwr * wwr)) * 0.5;
+    Fl[0] = UL[1]; Fl[1] = UL[1] * uul + pl; Fl[2] = UL[2] * uul; Fl[3] = UL[3] * uul; Fl[4] = uul * (UL[4] + pl);
+    Fr[0] = UR[1]; Fr[1] = UR[1] * uur + pr; Fr[2] = UR[2] * uur; Fr[3] = UR[3] * uur; Fr[4] = uur * (UR[4] + pr);
+    double tmpp = sqrt(dr / dl);
+    double avd = ((0.25 * dl) * (1.0 + tmpp)) * (1.0 + tmpp); (void)avd;
+    double avu = (uul + tmpp * uur) / (1.0 + tmpp);
+    double avv = (vvl + tmpp * vvr) / (1.0 + tmpp);
+    double avw = (wwl + tmpp * wwr) / (1.0 + tmpp);
+    double avh = (hl + tmpp * hr) / (1.0 + tmpp);
+    double ava = sqrt((gamma - 1.0) * (avh - 0.5 * ((avu * avu + avv * avv) + avw * avw)));
+    lam[0] = fabs(avu - ava); lam[1] = fabs(avu); lam[2] = fabs(avu + ava);
+    for (int q = 0; q < 3; q++) if (lam[q] < delt) lam[q] = (SQ(lam[q]) + delt * delt) / (2.0 * delt);
+    arr[1] = (UR[2] - UL[2]) / (avv + eps8) - (dr - dl);
+    arr[2] = (UR[3] - UL[3]) / (avw + eps8) - (dr - dl);
+    arr[3] = (gamma - 1.0) * ((((((dr - dl) * (((avh - avu * avu) - avv * avv) - avw * avw) + avu * (UR[1] - UL[1])) + avv * (UR[2] - UL[2]))
+                               + avw * (UR[3] - UL[3])) - (UR[4] - UL[4]))) / (ava * ava);
+    arr[0] = ((((avu + ava) * (dr - dl) - UR[1]) + UL[1]) - ava * arr[3]) / (2.0 * ava);
+    arr[4] = (dr - dl) - (arr[0] + arr[3]);
+    F[0] = 0.5 * (Fl[0] + Fr[0]) - 0.5 * ((lam[0] * arr[0] + lam[1] * arr[3]) + lam[2] * arr[4]);
+    F[1] = 0.5 * (Fl[1] + Fr[1]) - 0.5 * ((((lam[0] * arr[0]) * (avu - ava) + (lam[1] * arr[3]) * avu) + (lam[2] * arr[4]) * (avu + ava)));
+    F[2] = 0.5 * (Fl[2] + Fr[2]) - 0.5 * (((((lam[0] * arr[0]) * avv + (lam[1] * arr[1]) * avv) + (lam[1] * arr[3]) * avv) + (lam[2] * arr[4]) * avv));
+    F[3] = 0.5 * (Fl[3] + Fr[3]) - 0.5 * (((((lam[0] * arr[0]) * avw + (lam[1] * arr[2]) * avw) + (lam[1] * arr[3]) * avw) + (lam[2] * arr[4]) * avw));
+    F[4] = 0.5 * (Fl[4] + Fr[4]) - 0.5 * ((((((lam[0] * arr[0]) * (avh - avu * ava) + ((lam[1] * arr[1]) * avv) * avv) + ((lam[1] * arr[2]) * avw) * avw)
+                                             + ((lam[1] * arr[3]) * 0.5) * ((avu * avu + avv * avv) + avw * avw)) + (lam[2] * arr[4]) * (avh + avu * ava)));
+}
+
+static void flux_vl(const double *QL, const double *QR, double *F, double gamma) {   /* :217-275 */
+    double dl = QL[0], uul = QL[1], vvl = QL[2], wwl = QL[3], pl = QL[4];
+    double dr = QR[0], uur = QR[1], vvr = QR[2], wwr = QR[3], pr = QR[4];
+    double al = sqrt(gamma * pl / dl), ar = sqrt(gamma * pr / dr);
+    double Ml = uul / al, Mr = uur / ar, fp[5] = {0, 0, 0, 0, 0}, fm[5] = {0, 0, 0, 0, 0};
+    if (Ml >= 1.0) {
+        fp[0] = dl * uul; fp[1] = (dl * uul) * uul + pl; fp[2] = (dl * uul) * vvl; fp[3] = (dl * uul) * wwl;
+        fp[4] = uul * (gamma * pl / (gamma - 1.0) + (0.5 * dl) * ((uul * uul + vvl * vvl) + wwl * wwl));
+    } else if (fabs(Ml) < 1.0) {
+        double Mp = (0.25 * (1.0 + Ml)) * (1.0 + Ml), tmp0 = (dl * al) * Mp;
+        fp[0] = tmp0;
+        fp[1] = tmp0 * ((gamma - 1.0) * uul + 2.0 * al) / gamma;
+        fp[2] = tmp0 * vvl; fp[3] = tmp0 * wwl;
+        fp[4] = tmp0 * ((((gamma - 1.0) * uul + 2.0 * al) * ((gamma - 1.0) * uul + 2.0 * al)) * 0.5 / (gamma * gamma - 1.0) + 0.5 * (vvl * vvl + wwl * wwl));
+    }
+    if (Mr >= 1.0) {
+    } else if (fabs(Mr) < 1.0) {
+        double Mm = (-0.25 * (Mr - 1.0)) * (Mr - 1.0), tmp0 = (dr * ar) * Mm;
+        fm[0] = tmp0;
+        fm[1] = tmp0 * ((gamma - 1.0) * uur - 2.0 * ar) / gamma;
+        fm[2] = tmp0 * vvr; fm[3] = tmp0 * wwr;
+        fm[4] = tmp0 * ((((gamma - 1.0) * uur - 2.0 * ar) * ((gamma - 1.0) * uur - 2.0 * ar)) * 0.5 / (gamma * gamma - 1.0) + 0.5 * (vvr * vvr + wwr * wwr));
+    } else if (Mr <= -1.0) {
+        fm[0] = dr * uur; fm[1] = (dr * uur) * uur + pr; fm[2] = (dr * uur) * vvr; fm[3] = (dr * uur) * wwr;
+        fm[4] = uur * (gamma * pr / (gamma - 1.0) + (0.5 * dr) * ((uur * uur + vvr * vvr) + wwr * wwr));
+    }
+    for (int m = 0; m < 5; m++) F[m] = fp[m] + fm[m];
+}
+
+static double fmin2(double a, double b) { return a < b ? a : b; }
+
+static void flux_ausmpw(const double *QL, const double *QR, double *F, double gamma) {   /* :356-442 */
+    double dl = QL[0], uul = QL[1], vvl = QL[2], wwl = QL[3], pl = QL[4];
+    double dr = QR[0], uur = QR[1], vvr = QR[2], wwr = QR[3], pr = QR[4];
+    double hl = gamma * pl / ((gamma - 1.0) * dl) + 0.5 * ((uul * uul + vvl * vvl) + wwl * wwl);
+    double hr = gamma * pr / ((gamma - 1.0) * dr) + 0.5 * ((uur * uur + vvr * vvr) + wwr * wwr);
+    double al = sqrt(gamma * pl / dl), ar = sqrt(gamma * pr / dr);
+    double a = 0.5 * (al + ar), Ml = uul / a, Mr = uur / a;
+    double mlr = fmin2(pl / pr, pr / pl);
+    double wpl = 1.0 - (mlr * mlr) * mlr;      /* flang evaluates x**3.d0 as (x*x)*x */
+    double plpl = (0.75 <= mlr && 1.0 > mlr) ? 4.0 * mlr - 3.0 : 0.0;
+    double mrl = fmin2(pr / pl, pl / pr);
+    double plpr = (0.75 <= mrl && 1.0 > mrl) ? 4.0 * mrl - 3.0 : 0.0;
+    double Pp5, Pm5, Mp4, Mm4, fL, fR;
+    if (fabs(Ml) >= 1.0) Pp5 = 0.5 * (Ml + fabs(Ml)) / Ml;
+    else Pp5 = ((0.25 * (Ml + 1.0)) * (Ml + 1.0)) * (2.0 - Ml) + (((3.0 * Ml) * (Ml * Ml - 1.0)) * (Ml * Ml - 1.0)) / 16.0;
+    if (fabs(Mr) >= 1.0) Pm5 = 0.5 * (Mr - fabs(Mr)) / Mr;
+    else Pm5 = ((0.25 * (Mr - 1.0)) * (Mr - 1.0)) * (2.0 + Mr) - (((3.0 * Mr) * (Mr * Mr - 1.0)) * (Mr * Mr - 1.0)) / 16.0;
+    double p = pl * Pp5 + pr * Pm5;
+    if (fabs(Ml) >= 1.0) { Mp4 = 0.5 * (Ml + fabs(Ml)); fL = 0.0; }
+    else {
+        Mp4 = (0.25 * (Ml + 1.0)) * (Ml + 1.0) + (0.125 * (Ml * Ml - 1.0)) * (Ml * Ml - 1.0);
+        fL = (((pl / p - 1.0) * plpl) * fabs((0.25 * (Ml + 1.0)) * (Ml + 1.0))) * fmin2(1.0, pow(sqrt((uul * uul + vvl * vvl) + wwl * wwl) / a, 0.25));
+    }
+    if (fabs(Mr) >= 1.0) { Mm4 = 0.5 * (Mr - fabs(Mr)); fR = 0.0; }
+    else {
+        Mm4 = (-0.25 * (Mr - 1.0)) * (Mr - 1.0) - (0.125 * (Mr * Mr - 1.0)) * (Mr * Mr - 1.0);
+        fR = (((pr / p - 1.0) * plpr) * fabs((-0.25 * (Mr - 1.0)) * (Mr - 1.0))) * fmin2(1.0, pow(sqrt((uur * uur + vvr * vvr) + wwr * wwr) / a, 0.25));
+    }
+    double M = Mp4 + Mm4, MLp, MRm;
+    if (M >= 0.0) MLp = ((Mp4 + Mm4) - (Mm4 * wpl) * (1.0 + fR)) + (fL * Mp4 + fR * Mm4);
+    else MLp = (Mp4 * wpl) * (1.0 + fL);
+    if (M < 0.0) MRm = ((Mp4 + Mm4) - (Mp4 * wpl) * (1.0 + fL)) + (fL * Mp4 + fR * Mm4);
+    else MRm = (Mm4 * wpl) * (1.0 + fR);
+    double fp[5] = {dl, dl * uul, dl * vvl, dl * wwl, dl * hl}, fm[5] = {dr, dr * uur, dr * vvr, dr * wwr, dr * hr};
+    for (int m = 0; m < 5; m++) F[m] = (MLp * a) * fp[m] + (MRm * a) * fm[m];
+    F[1] = F[1] + p;
+}
+
+static void riemann(const double *QL, const double *QR, double *F, double gamma, int iflux) {
+    if (iflux == F_SW) flux_sw(QL, QR, F, gamma);
+    else if (iflux == F_ROE) flux_roe(QL, QR, F, gamma);
+    else if (iflux == F_VL) flux_vl(QL, QR, F, gamma);
+    else if (iflux == F_AUSM) flux_ausmpw(QL, QR, F, gamma);     /* id 6 -> AUSMPW (sub_Residual.f90:327) */
+    else flux_hll_hllc(QL, QR, F, gamma, iflux);
+}
+
+/* ------------------------------------------------------------------------------------------
+ * comput_duvtpc (sub_Residual.f90:1417-1456)
+ */
+static void primitives(orc_ctx *c, orc_block *b) {
+    double p00 = 1.0 / (c->gamma * c->Ma * c->Ma);
+    for (size_t s = 0; s < b->n; s++) {
+        double d = b->U[0][s];
+        b->d[s] = d;
+        b->uu[s] = b->U[1][s] / d; b->v[s] = b->U[2][s] / d; b->w[s] = b->U[3][s] / d;
+        b->T[s] = (b->U[4][s] - (0.5 * d) * ((b->uu[s] * b->uu[s] + b->v[s] * b->v[s]) + b->w[s] * b->w[s])) / (c->Cv * d);
+        b->p[s] = (p00 * d) * b->T[s];
+        b->cc[s] = sqrt(b->T[s]) / c->Ma;
+    }
+}
+
+/* get_viscous (sub_Residual.f90:1355-1383) */
+static void get_viscous(orc_ctx *c, orc_block *b) {
+    int nx = b->nx, ny = b->ny, nz = b->nz;
+    double Tsb = 110.4 / c->T_inf;
+    for (int k = 1; k <= nz - 1; k++)
+        for (int j = 1; j <= ny - 1; j++)
+            for (int i = 1; i <= nx - 1; i++) {
+                double T = b->T[IX(b, i, j, k)];
+                b->mu[IX(b, i, j, k)] = ((1.0 / c->Re) * (1.0 + Tsb)) * sqrt((T * T) * T) / (Tsb + T);
+            }
+    for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) {
+        b->mu[IX(b, 0, j, k)] = b->mu[IX(b, 1, j, k)]; b->mu[IX(b, nx, j, k)] = b->mu[IX(b, nx - 1, j, k)];
+    }
+    for (int k = 1; k <= nz - 1; k++) for (int i = 1; i <= nx - 1; i++) {
+        b->mu[IX(b, i, 0, k)] = b->mu[IX(b, i, 1, k)]; b->mu[IX(b, i, ny, k)] = b->mu[IX(b, i, ny - 1, k)];
+    }
+    for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+        b->mu[IX(b, i, j, 0)] = b->mu[IX(b, i, j, 1)]; b->mu[IX(b, i, j, nz)] = b->mu[IX(b, i, j, nz - 1)];
+    }
+}
+
+/* Amut_boundary (sub_turbulence_SA.f90:282-368) */
+static void amut_boundary(orc_block *b) {
+    for (int q = 0; q < b->nsub; q++) {
+        orc_bcmsg *B = &b->bc[q];
+        if (B->bc != BCW) continue;
+        int ib = B->ib, ie = B->ie, jb = B->jb, je = B->je, kb = B->kb, ke = B->ke;
+        switch (B->face) {
+        case 1: for (int k = kb; k <= ke - 1; k++) for (int j = jb; j <= je - 1; j++) b->mut[IX(b, 0, j, k)] = -b->mut[IX(b, 1, j, k)]; break;
+        case 4: for (int k = kb; k <= ke - 1; k++) for (int j = jb; j <= je - 1; j++) b->mut[IX(b, ie, j, k)] = -b->mut[IX(b, ie - 1, j, k)]; break;
+        case 2: for (int k = kb; k <= ke - 1; k++) for (int i = ib; i <= ie - 1; i++) b->mut[IX(b, i, 0, k)] = -b->mut[IX(b, i, 1, k)]; break;
+        case 5: for (int k = kb; k <= ke - 1; k++) for (int i = ib; i <= ie - 1; i++) b->mut[IX(b, i, je, k)] = -b->mut[IX(b, i, je - 1, k)]; break;
+        case 3: for (int j = jb; j <= je - 1; j++) for (int i = ib; i <= ie - 1; i++) b->mut[IX(b, i, j, 0)] = -b->mut[IX(b, i, j, 1)]; break;
+        case 6: for (int j = jb; j <= je - 1; j++) for (int i = ib; i <= ie - 1; i++) b->mut[IX(b, i, j, ke)] = -b->mut[IX(b, i, j, ke - 1)]; break;
+        }
+    }
+}
+
+/* limit_mut (sub_Residual.f90:1387-1408) */
+static void limit_mut(orc_ctx *c, orc_block *b) {
+    for (int k = 0; k <= b->nz; k++)
+        for (int j = 0; j <= b->ny; j++)
+            for (int i = 0; i <= b->nx; i++) {
+                size_t s = IX(b, i, j, k);
+                if (b->mut[s] < 0.0) b->mut[s] = 0.0;
+                if (c->MUT_MAX >= 0.0 && b->mut[s] > c->MUT_MAX * b->mu[s]) b->mut[s] = c->MUT_MAX * b->mu[s];
+            }
+    amut_boundary(b);
+}
+
+static double powi3(double x) { return (x * x) * x; }
+/* flang expands integer powers by sequential multiplication: x**6 = ((((x*x)*x)*x)*x)*x */
+static double powi6(double x) { return ((((x * x) * x) * x) * x) * x; }
+static double powi4(double x) { return ((x * x) * x) * x; }
+
+/* Turbulence_model_SA (sub_turbulence_SA.f90:8-273) */
+static void turbulence_sa(orc_ctx *c, orc_block *b) {
+    const double SA_sigma = 2.0 / 3.0, Cv1 = 7.1, Cb1 = 0.1355, Cb2 = 0.622, SA_k = 0.41;
+    const double Cw1 = Cb1 / (SA_k * SA_k) + (1.0 + Cb2) / SA_sigma, Cw2 = 0.3, Cw3 = 2.0, Ct3 = 1.2, Ct4 = 0.5;
+    const double Cv1_3 = powi3(Cv1), Cw3_6 = powi6(Cw3);
+    const double c_v0 = (0.5 * (1.0 + Cb2)) / SA_sigma;
+    int nx = b->nx, ny = b->ny, nz = b->nz;
+    double Re = c->Re;
+    double *vt = b->U[5];
+    double *Fv = (double *)calloc(b->n, sizeof(double)), *Fv2 = (double *)calloc(b->n, sizeof(double));
+    double *vtc = (double *)calloc(b->n, sizeof(double));
+    for (int k = 0; k <= nz; k++) for (int j = 0; j <= ny; j++) for (int i = 0; i <= nx; i++) {
+        size_t s = IX(b, i, j, k);
+        b->mu[s] = b->mu[s] * Re;
+    }
+    for (int k = 0; k <= nz; k++) for (int j = 0; j <= ny; j++) for (int i = 0; i <= nx; i++) {
+        size_t s = IX(b, i, j, k);
+        vtc[s] = vt[s];
+        double X = b->d[s] * vtc[s] / b->mu[s];
+        double fv1 = powi3(X) / (powi3(X) + Cv1_3);
+        b->mut[s] = (fv1 * b->d[s]) * vtc[s];
+    }
+    limit_mut(c, b);
+#define VT(i, j, k) vtc[IX(b, i, j, k)]
+#define MU(i, j, k) b->mu[IX(b, i, j, k)]
+#define D(i, j, k) b->d[IX(b, i, j, k)]
+#define UU(i, j, k) b->uu[IX(b, i, j, k)]
+#define VV(i, j, k) b->v[IX(b, i, j, k)]
+#define WW(i, j, k) b->w[IX(b, i, j, k)]
+    /* i faces */
+    for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx; i++) {
+        size_t s = IX(b, i, j, k);
+        double s1x = b->nv[0][s], s1y = b->nv[1][s], s1z = b->nv[2][s];
+        double vti = VT(i, j, k) - VT(i - 1, j, k);
+        double vtj = 0.25 * (((VT(i, j + 1, k) - VT(i, j - 1, k)) + VT(i - 1, j + 1, k)) - VT(i - 1, j - 1, k));
+        double vtk = 0.25 * (((VT(i, j, k + 1) - VT(i, j, k - 1)) + VT(i - 1, j, k + 1)) - VT(i - 1, j, k - 1));
+        double *const *mt = b->met[1];
+        double vtx = (vti * mt[0][s] + vtj * mt[3][s]) + vtk * mt[6][s];
+        double vty = (vti * mt[1][s] + vtj * mt[4][s]) + vtk * mt[7][s];
+        double vtz = (vti * mt[2][s] + vtj * mt[5][s]) + vtk * mt[8][s];
+        double v0 = c_v0 * (((VT(i, j, k) + MU(i, j, k) / D(i, j, k)) + VT(i - 1, j, k)) + MU(i - 1, j, k) / D(i - 1, j, k));
+        double vn1 = (UU(i - 1, j, k) * s1x + VV(i - 1, j, k) * s1y) + WW(i - 1, j, k) * s1z;
+        double vn2 = (UU(i, j, k) * s1x + VV(i, j, k) * s1y) + WW(i, j, k) * s1z;
+        double vfi = 0.5 * ((vn1 + fabs(vn1)) * VT(i - 1, j, k) + (vn2 - fabs(vn2)) * VT(i, j, k));
+        double gn = (vtx * s1x + vty * s1y) + vtz * s1z;
+        Fv[s] = (-vfi + (v0 / Re) * gn) * b->Si[s];
+        Fv2[s] = gn * b->Si[s];
+    }
+    for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+        double v0 = ((VT(i, j, k) + MU(i, j, k) / D(i, j, k)) * Cb2) / SA_sigma;
+        b->Res[5][IX(b, i, j, k)] = (Fv[IX(b, i + 1, j, k)] - Fv[IX(b, i, j, k)]) - (v0 / Re) * (Fv2[IX(b, i + 1, j, k)] - Fv2[IX(b, i, j, k)]);
+    }
+    /* j faces */
+    for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny; j++) for (int i = 1; i <= nx - 1; i++) {
+        size_t s = IX(b, i, j, k);
+        double s1x = b->nv[3][s], s1y = b->nv[4][s], s1z = b->nv[5][s];
+        double vti = 0.25 * (((VT(i + 1, j, k) - VT(i - 1, j, k)) + VT(i + 1, j - 1, k)) - VT(i - 1, j - 1, k));
+        double vtj = VT(i, j, k) - VT(i, j - 1, k);
+        double vtk = 0.25 * (((VT(i, j, k + 1) - VT(i, j, k - 1)) + VT(i, j - 1, k + 1)) - VT(i, j - 1, k - 1));
+        double *const *mt = b->met[2];
+        double vtx = (vti * mt[0][s] + vtj * mt[3][s]) + vtk * mt[6][s];
+        double vty = (vti * mt[1][s] + vtj * mt[4][s]) + vtk * mt[7][s];
+        double vtz = (vti * mt[2][s] + vtj * mt[5][s]) + vtk * mt[8][s];
+        double v0 = c_v0 * (((VT(i, j, k) + MU(i, j, k) / D(i, j, k)) + VT(i, j - 1, k)) + MU(i, j - 1, k) / D(i, j - 1, k));
+        double vn1 = (UU(i, j - 1, k) * s1x + VV(i, j - 1, k) * s1y) + WW(i, j - 1, k) * s1z;
+        double vn2 = (UU(i, j, k) * s1x + VV(i, j, k) * s1y) + WW(i, j, k) * s1z;
+        double vfi = 0.5 * ((vn1 + fabs(vn1)) * VT(i, j - 1, k) + (vn2 - fabs(vn2)) * VT(i, j, k));
+        double gn = (vtx * s1x + vty * s1y) + vtz * s1z;
+        Fv[s] = (-vfi + (v0 / Re) * gn) * b->Sj[s];
+        Fv2[s] = gn * b->Sj[s];
+    }
+    for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+        double v0 = ((VT(i, j, k) + MU(i, j, k) / D(i, j, k)) * Cb2) / SA_sigma;
+        size_t s = IX(b, i, j, k);
+        b->Res[5][s] = ((b->Res[5][s] + Fv[IX(b, i, j + 1, k)]) - Fv[s]) - (v0 / Re) * (Fv2[IX(b, i, j + 1, k)] - Fv2[s]);
+    }
+    /* k faces */
+    for (int k = 1; k <= nz; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+        size_t s = IX(b, i, j, k);
+        double s1x = b->nv[6][s], s1y = b->nv[7][s], s1z = b->nv[8][s];
+        double vti = 0.25 * (((VT(i + 1, j, k) - VT(i - 1, j, k)) + VT(i + 1, j, k - 1)) - VT(i - 1, j, k - 1));
+        double vtj = 0.25 * (((VT(i, j + 1, k) - VT(i, j - 1, k)) + VT(i, j + 1, k - 1)) - VT(i, j - 1, k - 1));
+        double vtk = VT(i, j, k) - VT(i, j, k - 1);
+        double *const *mt = b->met[3];
+        double vtx = (vti * mt[0][s] + vtj * mt[3][s]) + vtk * mt[6][s];
+        double vty = (vti * mt[1][s] + vtj * mt[4][s]) + vtk * mt[7][s];
+        double vtz = (vti * mt[2][s] + vtj * mt[5][s]) + vtk * mt[8][s];
+        double v0 = c_v0 * (((VT(i, j, k) + MU(i, j, k) / D(i, j, k)) + VT(i, j, k - 1)) + MU(i, j, k - 1) / D(i, j, k - 1));
+        double vn1 = (UU(i, j, k - 1) * s1x + VV(i, j, k - 1) * s1y) + WW(i, j, k - 1) * s1z;
+        double vn2 = (UU(i, j, k) * s1x + VV(i, j, k) * s1y) + WW(i, j, k) * s1z;
+        double vfi = 0.5 * ((vn1 + fabs(vn1)) * VT(i, j, k - 1) + (vn2 - fabs(vn2)) * VT(i, j, k));
+        double gn = (vtx * s1x + vty * s1y) + vtz * s1z;
+        Fv[s] = (-vfi + (v0 / Re) * gn) * b->Sk[s];
+        Fv2[s] = gn * b->Sk[s];
+    }
+    for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+        double v0 = ((VT(i, j, k) + MU(i, j, k) / D(i, j, k)) * Cb2) / SA_sigma;
+        size_t s = IX(b, i, j, k);
+        b->Res[5][s] = ((b->Res[5][s] + Fv[IX(b, i, j, k + 1)]) - Fv[s]) - (v0 / Re) * (Fv2[IX(b, i, j, k + 1)] - Fv2[s]);
+    }
+    /* source (:181-249) */
+    for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+        size_t s = IX(b, i, j, k);
+        double ui = UU(i + 1, j, k) - UU(i - 1, j, k), vi = VV(i + 1, j, k) - VV(i - 1, j, k), wi = WW(i + 1, j, k) - WW(i - 1, j, k);
+        double uj = UU(i, j + 1, k) - UU(i, j - 1, k), vj = VV(i, j + 1, k) - VV(i, j - 1, k), wj = WW(i, j + 1, k) - WW(i, j - 1, k);
+        double uk = UU(i, j, k + 1) - UU(i, j, k - 1), vk = VV(i, j, k + 1) - VV(i, j, k - 1), wk = WW(i, j, k + 1) - WW(i, j, k - 1);
+        double *const *mt = b->met[0];
+        double ix = mt[0][s], iy = mt[1][s], iz = mt[2][s], jx = mt[3][s], jy = mt[4][s], jz = mt[5][s], kx = mt[6][s], ky = mt[7][s], kz = mt[8][s];
+        double ux = (ui * ix + uj * jx) + uk * kx, vx = (vi * ix + vj * jx) + vk * kx, wx = (wi * ix + wj * jx) + wk * kx;
+        double uy = (ui * iy + uj * jy) + uk * ky, vy = (vi * iy + vj * jy) + vk * ky, wy = (wi * iy + wj * jy) + wk * ky;
+        double uz = (ui * iz + uj * jz) + uk * kz, vz = (vi * iz + vj * jz) + vk * kz, wz = (wi * iz + wj * jz) + wk * kz;
+        (void)vx; (void)uy; (void)wz; (void)ux; (void)vy;
+        double S = sqrt((SQ(wy - vz) + SQ(uz - wx)) + SQ(vx - uy));
+        double vts = VT(i, j, k), dw = b->dw[s];
+        double X = D(i, j, k) * vts / MU(i, j, k);
+        double fv1 = powi3(X) / (powi3(X) + Cv1_3);
+        double fv2 = 1.0 - X / (1.0 + X * fv1);
+        double ft2 = Ct3 * exp(-((Ct4 * X) * X));
+        double S1 = S + (fv2 * vts) / (Re * SQ(SA_k * dw));
+        if (!(S1 >= 0.0)) S1 = S1 > 0.0 ? S1 : 0.0;
+        double r = vts / (((((Re * S1) * SA_k) * SA_k) * dw) * dw);
+        if (r > 10.0) r = 10.0;
+        double g = r + Cw2 * (powi6(r) - r);
+        double fw = g * pow((1.0 + Cw3_6) / (powi6(g) + Cw3_6), 1.0 / 6.0);
+        double Q3 = (Cb1 * ((1.0 - ft2) * fv2 + ft2)) / (SA_k * SA_k) - Cw1 * fw;
+        double Q_SA = ((Cb1 * (1.0 - ft2)) * S1) * vts + (Q3 * SQ(vts / dw)) / Re;
+        b->Res[5][s] = b->Res[5][s] + Q_SA * b->vol[s];
+    }
+    for (int k = 0; k <= nz; k++) for (int j = 0; j <= ny; j++) for (int i = 0; i <= nx; i++) {
+        size_t s = IX(b, i, j, k);
+        b->mu[s] = b->mu[s] / Re;
+        b->mut[s] = b->mut[s] / Re;
+    }
+#undef VT
+#undef MU
+#undef D
+#undef UU
+#undef VV
+#undef WW
+    free(Fv); free(Fv2); free(vtc);
+}
+
+/* Turbulence_model_SST (sub_turbulence_SST.f90:7-348).  f1 on the last face of each
+ * direction is read from cell (nx,.,.) / (.,ny,.) / (.,.,nz), which the reference never
+ * writes; here that storage holds zero. */
+static void turbulence_sst(orc_ctx *c, orc_block *b) {
+    const double sigma_k1 = 0.85, sigma_w1 = 0.5, beta1 = 0.075, Cw1s = 0.533;
+    const double sigma_k2 = 1.0, sigma_w2 = 0.856, beta2 = 0.0828, Cw2s = 0.440;
+    const double a1 = 0.31, betas = 0.09;
+    const double c009 = (double)0.09f;                 /* single-precision literal (:108,120) */
+    int nx = b->nx, ny = b->ny, nz = b->nz;
+    double Re = c->Re;
+    double *Kt = (double *)calloc(b->n, sizeof(double)), *Wt = (double *)calloc(b->n, sizeof(double));
+    double *Fk = (double *)calloc(b->n, sizeof(double)), *Fw = (double *)calloc(b->n, sizeof(double));
+    double *f1 = (double *)calloc(b->n, sizeof(double));
+    for (int k = 0; k <= nz; k++) for (int j = 0; j <= ny; j++) for (int i = 0; i <= nx; i++) {
+        size_t s = IX(b, i, j, k); b->mu[s] = b->mu[s] * Re;
+    }
+    for (int k = 0; k <= nz; k++) for (int j = 0; j <= ny; j++) for (int i = 0; i <= nx; i++) {
+        size_t s = IX(b, i, j, k);
+        Kt[s] = b->U[5][s] / b->U[0][s]; Wt[s] = b->U[6][s] / b->U[0][s];
+    }
+#define KT(i, j, k) Kt[IX(b, i, j, k)]
+#define WT(i, j, k) Wt[IX(b, i, j, k)]
+#define UU(i, j, k) b->uu[IX(b, i, j, k)]
+#define VV(i, j, k) b->v[IX(b, i, j, k)]
+#define WW(i, j, k) b->w[IX(b, i, j, k)]
+    for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+        size_t s = IX(b, i, j, k);
+        double ui = UU(i + 1, j, k) - UU(i - 1, j, k), vi = VV(i + 1, j, k) - VV(i - 1, j, k), wi = WW(i + 1, j, k) - WW(i - 1, j, k);
+        double kti = KT(i + 1, j, k) - KT(i - 1, j, k), wti = WT(i + 1, j, k) - WT(i - 1, j, k);
+        double uj = UU(i, j + 1, k) - UU(i, j - 1, k), vj = VV(i, j + 1, k) - VV(i, j - 1, k), wj = WW(i, j + 1, k) - WW(i, j - 1, k);
+        double ktj = KT(i, j + 1, k) - KT(i, j - 1, k), wtj = WT(i, j + 1, k) - WT(i, j - 1, k);
+        double uk = UU(i, j, k + 1) - UU(i, j, k - 1), vk = VV(i, j, k + 1) - VV(i, j, k - 1), wk = WW(i, j, k + 1) - WW(i, j, k - 1);
+        double ktk = KT(i, j, k + 1) - KT(i, j, k - 1), wtk = WT(i, j, k + 1) - WT(i, j, k - 1);
+        double *const *mt = b->met[0];
+        double ix = mt[0][s], iy = mt[1][s], iz = mt[2][s], jx = mt[3][s], jy = mt[4][s], jz = mt[5][s], kx = mt[6][s], ky = mt[7][s], kz = mt[8][s];
+        double ux = (ui * ix + uj * jx) + uk * kx, vx = (vi * ix + vj * jx) + vk * kx, wx = (wi * ix + wj * jx) + wk * kx;
+        double ktx = (kti * ix + ktj * jx) + ktk * kx, wtx = (wti * ix + wtj * jx) + wtk * kx;
+        double uy = (ui * iy + uj * jy) + uk * ky, vy = (vi * iy + vj * jy) + vk * ky, wy = (wi * iy + wj * jy) + wk * ky;
+        double kty = (kti * iy + ktj * jy) + ktk * ky, wty = (wti * iy + wtj * jy) + wtk * ky;
+        double uz = (ui * iz + uj * jz) + uk * kz, vz = (vi * iz + vj * jz) + vk * kz, wz = (wi * iz + wj * jz) + wk * kz;
+        double ktz = (kti * iz + ktj * jz) + ktk * kz, wtz = (wti * iz + wtj * jz) + wtk * kz;
+        (void)ux; (void)vy; (void)wz;
+        double omega = sqrt((SQ(wy - vz) + SQ(uz - wx)) + SQ(vx - uy));
+        double d = b->d[s], mu = b->mu[s], dw = b->dw[s], kt = KT(i, j, k), wt = WT(i, j, k);
+        double a2a = 2.0 * sqrt(fabs(kt)) / ((((c009 * wt) * dw) * Re));
+        double a2b = (500.0 * mu) / ((((d * wt) * SQ(dw)) * Re) * Re);
+        double arg2 = a2a > a2b ? a2a : a2b;
+        double f2 = tanh(SQ(arg2));
+        double den = a1 * wt, alt = (f2 * fabs(omega)) / Re;
+        b->mut[s] = ((a1 * d) * kt) / (den > alt ? den : alt);
+        double Kws = ((((2.0 * ((ktx * wtx + kty * wty) + ktz * wtz)) * d) * sigma_w2) / (wt + 1.0e-20));
+        double CD_kw = Kws > 1.0e-20 ? Kws : 1.0e-20;
+        double a3a = sqrt(fabs(kt)) / (((c009 * wt) * dw) * Re);
+        double arg3 = a3a > a2b ? a3a : a2b;
+        double a1b = (((4.0 * d) * sigma_w2) * kt) / (CD_kw * SQ(dw));
+        double arg1 = arg3 < a1b ? arg3 : a1b;
+        f1[s] = tanh(powi4(arg1));
+        double Pk = (b->mut[s] * omega) * omega;
+        double Qk = Pk / Re - (((Re * betas) * d) * wt) * kt;
+        double Cw = f1[s] * Cw1s + (1.0 - f1[s]) * Cw2s;
+        double beta = f1[s] * beta1 + (1.0 - f1[s]) * beta2;
+        double Qw = ((((Cw * d) * omega) * omega) / Re - ((Re * beta) * d) * SQ(wt)) + ((1.0 - f1[s]) * Kws) / Re;
+        b->Res[5][s] = Qk * b->vol[s];
+        b->Res[6][s] = Qw * b->vol[s];
+    }
+    /* mu_t ghost copies over the whole -1:n+1 extent, then Amut_boundary (:170-177) */
+    for (int k = -1; k <= nz + 1; k++) for (int j = -1; j <= ny + 1; j++) b->mut[IX(b, 0, j, k)] = b->mut[IX(b, 1, j, k)];
+    for (int k = -1; k <= nz + 1; k++) for (int j = -1; j <= ny + 1; j++) b->mut[IX(b, nx, j, k)] = b->mut[IX(b, nx - 1, j, k)];
+    for (int k = -1; k <= nz + 1; k++) for (int i = -1; i <= nx + 1; i++) b->mut[IX(b, i, 0, k)] = b->mut[IX(b, i, 1, k)];
+    for (int k = -1; k <= nz + 1; k++) for (int i = -1; i <= nx + 1; i++) b->mut[IX(b, i, ny, k)] = b->mut[IX(b, i, ny - 1, k)];
+    for (int j = -1; j <= ny + 1; j++) for (int i = -1; i <= nx + 1; i++) b->mut[IX(b, i, j, 0)] = b->mut[IX(b, i, j, 1)];
+    for (int j = -1; j <= ny + 1; j++) for (int i = -1; i <= nx + 1; i++) b->mut[IX(b, i, j, nz)] = b->mut[IX(b, i, j, nz - 1)];
+    amut_boundary(b);
+#define MU(i, j, k) b->mu[IX(b, i, j, k)]
+#define MT(i, j, k) b->mut[IX(b, i, j, k)]
+    for (int dir = 0; dir < 3; dir++) {
+        int ni = nx - 1 + (dir == 0), nj = ny - 1 + (dir == 1), nk = nz - 1 + (dir == 2);
+        for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
+            size_t s = IX(b, i, j, k);
+            int im = i - (dir == 0), jm = j - (dir == 1), km = k - (dir == 2);
+            double sk = f1[s] * sigma_k1 + (1.0 - f1[s]) * sigma_k2;
+            double sw = f1[s] * sigma_w1 + (1.0 - f1[s]) * sigma_w2;
+            double muk = (((MU(im, jm, km) + MU(i, j, k)) + sk * (MT(im, jm, km) + MT(i, j, k))) * 0.5) / Re;
+            double muw = (((MU(im, jm, km) + MU(i, j, k)) + sw * (MT(im, jm, km) + MT(i, j, k))) * 0.5) / Re;
+            double s1x = b->nv[3 * dir][s], s1y = b->nv[3 * dir + 1][s], s1z = b->nv[3 * dir + 2][s];
+            double kti, wti, ktj, wtj, ktk, wtk;
+            if (dir == 0) {
+                kti = KT(i, j, k) - KT(i - 1, j, k); wti = WT(i, j, k) - WT(i - 1, j, k);
+                ktj = 0.25 * (((KT(i, j + 1, k) - KT(i, j - 1, k)) + KT(i - 1, j + 1, k)) - KT(i - 1, j - 1, k));
+                wtj = 0.25 * (((WT(i, j + 1, k) - WT(i, j - 1, k)) + WT(i - 1, j + 1, k)) - WT(i - 1, j - 1, k));
+                ktk = 0.25 * (((KT(i, j, k + 1) - KT(i, j, k - 1)) + KT(i - 1, j, k + 1)) - KT(i - 1, j, k - 1));
+                wtk = 0.25 * (((WT(i, j, k + 1) - WT(i, j, k - 1)) + WT(i - 1, j, k + 1)) - WT(i - 1, j, k - 1));
+            } else if (dir == 1) {
+                kti = 0.25 * (((KT(i + 1, j, k) - KT(i - 1, j, k)) + KT(i + 1, j - 1, k)) - KT(i - 1, j - 1, k));
+                ktj = KT(i, j, k) - KT(i, j - 1, k);
+                ktk = 0.25 * (((KT(i, j, k + 1) - KT(i, j, k - 1)) + KT(i, j - 1, k + 1)) - KT(i, j - 1, k - 1));
+                wti = 0.25 * (((WT(i + 1, j, k) - WT(i - 1, j, k)) + WT(i + 1, j - 1, k)) - WT(i - 1, j - 1, k));
+                wtj = WT(i, j, k) - WT(i, j - 1, k);
+                wtk = 0.25 * (((WT(i, j, k + 1) - WT(i, j, k - 1)) + WT(i, j - 1, k + 1)) - WT(i, j - 1, k - 1));
+            } else {
+                kti = 0.25 * (((KT(i + 1, j, k) - KT(i - 1, j, k)) + KT(i + 1, j, k - 1)) - KT(i - 1, j, k - 1));
+                ktj = 0.25 * (((KT(i, j + 1, k) - KT(i, j - 1, k)) + KT(i, j + 1, k - 1)) - KT(i, j - 1, k - 1));
+                ktk = KT(i, j, k) - KT(i, j, k - 1);
+                wti = 0.25 * (((WT(i + 1, j, k) - WT(i - 1, j, k)) + WT(i + 1, j, k - 1)) - WT(i - 1, j, k - 1));
+                wtj = 0.25 * (((WT(i, j + 1, k) - WT(i, j - 1, k)) + WT(i, j + 1, k - 1)) - WT(i, j - 1, k - 1));
+                wtk = WT(i, j, k) - WT(i, j, k - 1);
+            }
+            double *const *mt = b->met[dir + 1];
+            double ix = mt[0][s], iy = mt[1][s], iz = mt[2][s], jx = mt[3][s], jy = mt[4][s], jz = mt[5][s], kx = mt[6][s], ky = mt[7][s], kz = mt[8][s];
+            double ktx = (kti * ix + ktj * jx) + ktk * kx, wtx = (wti * ix + wtj * jx) + wtk * kx;
+            double kty = (kti * iy + ktj * jy) + ktk * ky, wty = (wti * iy + wtj * jy) + wtk * ky;
+            double ktz = (kti * iz + ktj * jz) + ktk * kz, wtz = (wti * iz + wtj * jz) + wtk * kz;
+            double vn1 = (UU(im, jm, km) * s1x + VV(im, jm, km) * s1y) + WW(im, jm, km) * s1z;
+            double vn2 = (UU(i, j, k) * s1x + VV(i, j, k) * s1y) + WW(i, j, k) * s1z;
+            double kfi = 0.5 * ((vn1 + fabs(vn1)) * KT(im, jm, km) + (vn2 - fabs(vn2)) * KT(i, j, k));
+            double wfi = 0.5 * ((vn1 + fabs(vn1)) * WT(im, jm, km) + (vn2 - fabs(vn2)) * WT(i, j, k));
+            double S = dir == 0 ? b->Si[s] : dir == 1 ? b->Sj[s] : b->Sk[s];
+            Fk[s] = (-kfi + muk * ((ktx * s1x + kty * s1y) + ktz * s1z)) * S;
+            Fw[s] = (-wfi + muw * ((wtx * s1x + wty * s1y) + wtz * s1z)) * S;
+        }
+        for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+            size_t s = IX(b, i, j, k), sp = IX(b, i + (dir == 0), j + (dir == 1), k + (dir == 2));
+            b->Res[5][s] = (b->Res[5][s] + Fk[sp]) - Fk[s];
+            b->Res[6][s] = (b->Res[6][s] + Fw[sp]) - Fw[s];
+        }
+    }
+    for (int k = 0; k <= nz; k++) for (int j = 0; j <= ny; j++) for (int i = 0; i <= nx; i++) {
+        size_t s = IX(b, i, j, k); b->mu[s] = b->mu[s] / Re; b->mut[s] = b->mut[s] / Re;
+    }
+#undef KT
+#undef WT
+#undef UU
+#undef VV
+#undef WW
+#undef MU
+#undef MT
+    free(Kt); free(Wt); free(Fk); free(Fw); free(f1);
+}
+
+/* ------------------------------------------------------------------------------------------
+ * flux_inviscous_i/j/k (sub_Residual.f90:182-705)
+ */
+static void flux_inviscid(orc_ctx *c, orc_block *b, int dir) {
+    int nx = b->nx, ny = b->ny, nz = b->nz;
+    int ni = nx - 1 + (dir == 0), nj = ny - 1 + (dir == 1), nk = nz - 1 + (dir == 2);
+    int nd = dir == 0 ? nx : dir == 1 ? ny : nz;
+    int di = dir == 0, dj = dir == 1, dk = dir == 2;
+    double gamma = c->gamma;
+    for (int m = 0; m < c->NVAR; m++) memset(b->Flux[m], 0, b->n * sizeof(double));
+    const double *S = dir == 0 ? b->Si : dir == 1 ? b->Sj : b->Sk;
+    for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
+        size_t s = IX(b, i, j, k);
+        int pos = dir == 0 ? i : dir == 1 ? j : k;
+        int scheme = c->Scheme;
+        int bc_lo, bc_hi;
+        if (dir == 0) { bc_lo = b->BcI[0][JK(b, j, k)]; bc_hi = b->BcI[1][JK(b, j, k)]; }
+        else if (dir == 1) { bc_lo = b->BcJ[0][IK(b, i, k)]; bc_hi = b->BcJ[1][IK(b, i, k)]; }
+        else { bc_lo = b->BcK[0][IJ(b, i, j)]; bc_hi = b->BcK[1][IJ(b, i, j)]; }
+        if (bc_lo == 1) { if (pos == 1) scheme = S_CD2; else if (pos < LAP) scheme = c->Bound_Scheme; }
+        if (bc_hi == 1) { if (pos == nd) scheme = S_CD2; else if (pos > nd - LAP + 1) scheme = c->Bound_Scheme; }
+        if (b->IF_OverLimit == 1) scheme = S_UD1;
+        double s1x = b->nv[3 * dir][s], s1y = b->nv[3 * dir + 1][s], s1z = b->nv[3 * dir + 2][s];
+        double t1x, t1y, t1z;
+        if (dir == 0) { t1x = b->x[IX(b, i, j + 1, k)] - b->x[IX(b, i, j, k + 1)]; t1y = b->y[IX(b, i, j + 1, k)] - b->y[IX(b, i, j, k + 1)]; t1z = b->z[IX(b, i, j + 1, k)] - b->z[IX(b, i, j, k + 1)]; }
+        else if (dir == 1) { t1x = b->x[IX(b, i + 1, j, k + 1)] - b->x[s]; t1y = b->y[IX(b, i + 1, j, k + 1)] - b->y[s]; t1z = b->z[IX(b, i + 1, j, k + 1)] - b->z[s]; }
+        else { t1x = b->x[IX(b, i + 1, j + 1, k)] - b->x[s]; t1y = b->y[IX(b, i + 1, j + 1, k)] - b->y[s]; t1z = b->z[IX(b, i + 1, j + 1, k)] - b->z[s]; }
+        double t10 = 1.0 / sqrt((t1x * t1x + t1y * t1y) + t1z * t1z);
+        double s2x = t1x * t10, s2y = t1y * t10, s2z = t1z * t10;
+        double s3x = s1y * s2z - s1z * s2y, s3y = s1z * s2x - s1x * s2z, s3z = s1x * s2y - s1y * s2x;
+        double U0[5][8], UL[5], UR[5];
+        for (int q = 1 - LAP; q <= LAP; q++) {
+            /* U0(q) holds the cell at face_index - 1 + q: q=0 -> cell i-1, q=1 -> cell i */
+            size_t t = IX(b, i + (q - 1) * di, j + (q - 1) * dj, k + (q - 1) * dk);
+            if (c->Recon == 0) {
+                U0[0][q + 3] = b->d[t]; U0[1][q + 3] = b->uu[t]; U0[2][q + 3] = b->v[t]; U0[3][q + 3] = b->w[t]; U0[4][q + 3] = b->p[t];
+            } else {
+                for (int m = 0; m < 5; m++) U0[m][q + 3] = b->U[m][t];
+            }
+        }
+        reconstruct(U0, UL, UR, gamma, scheme, c->Recon);
+        if (c->IF_Scheme_Positivity == 1) {
+            if (UL[0] <= LIM_ZERO || UL[4] <= LIM_ZERO) {
+                size_t t = IX(b, i - di, j - dj, k - dk);
+                UL[0] = b->d[t]; UL[1] = b->uu[t]; UL[2] = b->v[t]; UL[3] = b->w[t]; UL[4] = b->p[t];
+            }
+            if (UR[0] <= LIM_ZERO || UR[4] <= LIM_ZERO) {
+                UR[0] = b->d[s]; UR[1] = b->uu[s]; UR[2] = b->v[s]; UR[3] = b->w[s]; UR[4] = b->p[s];
+            }
+        }
+        double QL[5], QR[5], F0[5];
+        QL[0] = UL[0]; QL[1] = (UL[1] * s1x + UL[2] * s1y) + UL[3] * s1z; QL[2] = (UL[1] * s2x + UL[2] * s2y) + UL[3] * s2z;
+        QL[3] = (UL[1] * s3x + UL[2] * s3y) + UL[3] * s3z; QL[4] = UL[4];
+        QR[0] = UR[0]; QR[1] = (UR[1] * s1x + UR[2] * s1y) + UR[3] * s1z; QR[2] = (UR[1] * s2x + UR[2] * s2y) + UR[3] * s2z;
+        QR[3] = (UR[1] * s3x + UR[2] * s3y) + UR[3] * s3z; QR[4] = UR[4];
+        riemann(QL, QR, F0, gamma, c->Flux);
+        b->Flux[0][s] = -F0[0] * S[s];
+        b->Flux[1][s] = -((F0[1] * s1x + F0[2] * s2x) + F0[3] * s3x) * S[s];
+        b->Flux[2][s] = -((F0[1] * s1y + F0[2] * s2y) + F0[3] * s3y) * S[s];
+        b->Flux[3][s] = -((F0[1] * s1z + F0[2] * s2z) + F0[3] * s3z) * S[s];
+        b->Flux[4][s] = -F0[4] * S[s];
+    }
+    if (dir == 2) {
+        /* explicit average-state flux on physical k faces (sub_Residual.f90:667-688) */
+        for (int q = 0; q < b->nsub; q++) {
+            orc_bcmsg *B = &b->bc[q];
+            if (!(B->bc > 0 && (B->face == 3 || B->face == 6))) continue;
+            int k = B->kb;
+            for (int j = B->jb; j <= B->je - 1; j++) for (int i = B->ib; i <= B->ie - 1; i++) {
+                size_t s = IX(b, i, j, k), t = IX(b, i, j, k - 1);
+                double d0 = (b->d[s] + b->d[t]) * 0.5, uu0 = (b->uu[s] + b->uu[t]) * 0.5, v0 = (b->v[s] + b->v[t]) * 0.5;
+                double w0 = (b->w[s] + b->w[t]) * 0.5, p0 = (b->p[s] + b->p[t]) * 0.5;
+                double E0 = p0 / (gamma - 1.0) + (d0 * ((uu0 * uu0 + v0 * v0) + w0 * w0)) * 0.5;
+                double n1 = b->nv[6][s], n2 = b->nv[7][s], n3 = b->nv[8][s];
+                double un = (uu0 * n1 + v0 * n2) + w0 * n3;
+                b->Flux[0][s] = -(d0 * un) * b->Sk[s];
+                b->Flux[1][s] = -((d0 * un) * uu0 + p0 * n1) * b->Sk[s];
+                b->Flux[2][s] = -((d0 * un) * v0 + p0 * n2) * b->Sk[s];
+                b->Flux[3][s] = -((d0 * un) * w0 + p0 * n3) * b->Sk[s];
+                b->Flux[4][s] = -((E0 + p0) * un) * b->Sk[s];
+            }
+        }
+    }
+    for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+        size_t s = IX(b, i, j, k), sp = IX(b, i + di, j + dj, k + dk);
+        for (int m = 0; m < 5; m++) {
+            if (dir == 0) b->Res[m][s] = b->Flux[m][sp] - b->Flux[m][s];
+            else b->Res[m][s] = (b->Res[m][s] + b->Flux[m][sp]) - b->Flux[m][s];
+        }
+    }
+}
+
+/* flux_viscous_i/j/k (sub_Residual.f90:712-1247) */
+static void flux_viscous(orc_ctx *c, orc_block *b, int dir) {
+    int nx = b->nx, ny = b->ny, nz = b->nz;
+    const double tmp1 = 4.0 / 3.0, tmp2 = 2.0 / 3.0;
+    if (b->IF_OverLimit == 1) {
+        int fa = dir, fb = dir + 3;
+        size_t n2 = (size_t)(b->NI > b->NJ ? b->NI : b->NJ) * (size_t)(b->NJ > b->NK ? b->NJ : b->NK) * 2;
+        for (int q = 0; q < 3; q++) { memset(b->Surf[fa][q], 0, n2 * sizeof(double)); memset(b->Surf[fb][q], 0, n2 * sizeof(double)); }
+        return;
+    }
+    int ni = nx - 1 + (dir == 0), nj = ny - 1 + (dir == 1), nk = nz - 1 + (dir == 2);
+    int di = dir == 0, dj = dir == 1, dk = dir == 2;
+    const double *S = dir == 0 ? b->Si : dir == 1 ? b->Sj : b->Sk;
+    double *uu = b->uu, *vv = b->v, *ww = b->w, *T = b->T;
+#define A(arr, i, j, k) arr[IX(b, i, j, k)]
+    for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
+        size_t s = IX(b, i, j, k);
+        int im = i - di, jm = j - dj, km = k - dk;
+        double s1x = b->nv[3 * dir][s], s1y = b->nv[3 * dir + 1][s], s1z = b->nv[3 * dir + 2][s];
+        /* derivative along the face normal index direction and averaged tangential ones */
+        double dn[4], ta1[4], ta2[4], tb1[4], tb2[4];
+        double *Q[4] = {uu, vv, ww, T};
+        for (int q = 0; q < 4; q++) dn[q] = A(Q[q], i, j, k) - A(Q[q], im, jm, km);
+        /* tangential directions: for dir i -> (j,k); dir j -> (i,k); dir k -> (i,j) */
+        int tdA = dir == 0 ? 1 : 0, tdB = dir == 2 ? 1 : 2;
+        int pos = dir == 0 ? i : dir == 1 ? j : k;
+        int posA = tdA == 0 ? i : j, posB = tdB == 1 ? j : k;
+        int ndim = dir == 0 ? nx : dir == 1 ? ny : nz;
+        int nA = tdA == 0 ? nx : ny, nB = tdB == 1 ? ny : nz;
+        int zeroA = (pos == 1 || pos == ndim) && (posA == 1 || posA == nA - 1);
+        int zeroB = (pos == 1 || pos == ndim) && (posB == 1 || posB == nB - 1);
+        int ai = tdA == 0, aj = tdA == 1, bj = tdB == 1, bk = tdB == 2;
+        for (int q = 0; q < 4; q++) {
+            if (zeroA) { ta1[q] = 0.0; ta2[q] = 0.0; }
+            else {
+                ta1[q] = 0.5 * (A(Q[q], i - ai, j - aj, k) + A(Q[q], im - ai, jm - aj, km));
+                ta2[q] = 0.5 * (A(Q[q], i + ai, j + aj, k) + A(Q[q], im + ai, jm + aj, km));
+            }
+            if (zeroB) { tb1[q] = 0.0; tb2[q] = 0.0; }
+            else {
+                tb1[q] = 0.5 * (A(Q[q], i, j - bj, k - bk) + A(Q[q], im, jm - bj, km - bk));
+                tb2[q] = 0.5 * (A(Q[q], i, j + bj, k + bk) + A(Q[q], im, jm + bj, km + bk));
+            }
+        }
+        double gi[4], gj[4], gk[4];
+        for (int q = 0; q < 4; q++) {
+            double ga = 0.5 * (ta2[q] - ta1[q]), gb = 0.5 * (tb2[q] - tb1[q]);
+            if (dir == 0) { gi[q] = dn[q]; gj[q] = ga; gk[q] = gb; }
+            else if (dir == 1) { gi[q] = ga; gj[q] = dn[q]; gk[q] = gb; }
+            else { gi[q] = ga; gj[q] = gb; gk[q] = dn[q]; }
+        }
+        double *const *mt = b->met[dir + 1];
+        double ix = mt[0][s], iy = mt[1][s], iz = mt[2][s], jx = mt[3][s], jy = mt[4][s], jz = mt[5][s], kx = mt[6][s], ky = mt[7][s], kz = mt[8][s];
+        double ux = (gi[0] * ix + gj[0] * jx) + gk[0] * kx, vx = (gi[1] * ix + gj[1] * jx) + gk[1] * kx;
+        double wx = (gi[2] * ix + gj[2] * jx) + gk[2] * kx, Tx = (gi[3] * ix + gj[3] * jx) + gk[3] * kx;
+        double uy = (gi[0] * iy + gj[0] * jy) + gk[0] * ky, vy = (gi[1] * iy + gj[1] * jy) + gk[1] * ky;
+        double wy = (gi[2] * iy + gj[2] * jy) + gk[2] * ky, Ty = (gi[3] * iy + gj[3] * jy) + gk[3] * ky;
+        double uz = (gi[0] * iz + gj[0] * jz) + gk[0] * kz, vz = (gi[1] * iz + gj[1] * jz) + gk[1] * kz;
+        double wz = (gi[2] * iz + gj[2] * jz) + gk[2] * kz, Tz = (gi[3] * iz + gj[3] * jz) + gk[3] * kz;
+        double mu0 = 0.5 * (((A(b->mu, i, j, k) + A(b->mut, i, j, k)) + A(b->mu, im, jm, km)) + A(b->mut, im, jm, km));
+        double k0 = (0.5 * c->Cp) * (((A(b->mu, i, j, k) / c->PrL + A(b->mut, i, j, k) / c->PrT) + A(b->mu, im, jm, km) / c->PrL) + A(b->mut, im, jm, km) / c->PrT);
+        double s11 = mu0 * (tmp1 * ux - tmp2 * (vy + wz)), s12 = mu0 * (uy + vx), s13 = mu0 * (uz + wx);
+        double s22 = mu0 * (tmp1 * vy - tmp2 * (ux + wz)), s23 = mu0 * (vz + wy), s33 = mu0 * (tmp1 * wz - tmp2 * (ux + vy));
+        double u1 = (A(uu, i, j, k) + A(uu, im, jm, km)) * 0.5, v1 = (A(vv, i, j, k) + A(vv, im, jm, km)) * 0.5, w1 = (A(ww, i, j, k) + A(ww, im, jm, km)) * 0.5;
+        double E1 = ((u1 * s11 + v1 * s12) + w1 * s13) + k0 * Tx;
+        double E2 = ((u1 * s12 + v1 * s22) + w1 * s23) + k0 * Ty;
+        double E3 = ((u1 * s13 + v1 * s23) + w1 * s33) + k0 * Tz;
+        b->Flux[1][s] = ((s11 * s1x + s12 * s1y) + s13 * s1z) * S[s];
+        b->Flux[2][s] = ((s12 * s1x + s22 * s1y) + s23 * s1z) * S[s];
+        b->Flux[3][s] = ((s13 * s1x + s23 * s1y) + s33 * s1z) * S[s];
+        b->Flux[4][s] = ((E1 * s1x + E2 * s1y) + E3 * s1z) * S[s];
+    }
+#undef A
+    for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+        size_t s = IX(b, i, j, k), sp = IX(b, i + di, j + dj, k + dk);
+        for (int m = 1; m < 5; m++) b->Res[m][s] = (b->Res[m][s] + b->Flux[m][sp]) - b->Flux[m][s];
+    }
+    /* wall-face viscous forces Surf1..6 */
+    if (dir == 0) {
+        for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int q = 0; q < 3; q++) {
+            b->Surf[0][q][JK(b, j, k)] = b->Flux[1 + q][IX(b, 1, j, k)];
+            b->Surf[3][q][JK(b, j, k)] = b->Flux[1 + q][IX(b, nx, j, k)];
+        }
+    } else if (dir == 1) {
+        for (int k = 1; k <= nz - 1; k++) for (int i = 1; i <= nx - 1; i++) for (int q = 0; q < 3; q++) {
+            b->Surf[1][q][IK(b, i, k)] = b->Flux[1 + q][IX(b, i, 1, k)];
+            b->Surf[4][q][IK(b, i, k)] = b->Flux[1 + q][IX(b, i, ny, k)];
+        }
+    } else {
+        for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) for (int q = 0; q < 3; q++) {
+            b->Surf[2][q][IJ(b, i, j)] = b->Flux[1 + q][IX(b, i, j, 1)];
+            b->Surf[5][q][IJ(b, i, j)] = b->Flux[1 + q][IX(b, i, j, nz)];
+        }
+    }
+}
+
+/* comput_Lijk (sub_time_acceleraction.f90:7-56) */
+static void comput_lijk(orc_ctx *c, orc_block *b) {
+    const double g43 = c->gamma > (double)(4.0f / 3.0f) ? c->gamma : (double)(4.0f / 3.0f);
+    for (int k = 1; k <= b->nz - 1; k++) for (int j = 1; j <= b->ny - 1; j++) for (int i = 1; i <= b->nx - 1; i++) {
+        size_t s = IX(b, i, j, k);
+        double D0 = c->If_viscous == 1 ? (g43 / b->d[s]) * (b->mu[s] / c->PrL + b->mut[s] / c->PrT) : 0.0;
+        double vol1 = 1.0 / b->vol[s];
+        double *Lc[3] = {b->Lci, b->Lcj, b->Lck}, *Lv[3] = {b->Lvi, b->Lvj, b->Lvk};
+        const double *Sd[3] = {b->Si, b->Sj, b->Sk};
+        for (int dir = 0; dir < 3; dir++) {
+            size_t sp = IX(b, i + (dir == 0), j + (dir == 1), k + (dir == 2));
+            double un = 0.5 * ((b->uu[s] * (b->nv[3 * dir][s] + b->nv[3 * dir][sp]) + b->v[s] * (b->nv[3 * dir + 1][s] + b->nv[3 * dir + 1][sp]))
+                               + b->w[s] * (b->nv[3 * dir + 2][s] + b->nv[3 * dir + 2][sp]));
+            double S0 = (Sd[dir][s] + Sd[dir][sp]) * 0.5;
+            Lc[dir][s] = (fabs(un) + b->cc[s]) * S0;
+            Lv[dir][s] = ((D0 * S0) * S0) * vol1;
+        }
+    }
+}
+
+/* comput_dt (sub_time_acceleraction.f90:61-112) */
+static void comput_dt(orc_ctx *c, orc_block *b) {
+    double dt_fac = b->IF_OverLimit == 0 ? 1.0 : 0.1;
+    for (int k = 1; k <= b->nz - 1; k++) for (int j = 1; j <= b->ny - 1; j++) for (int i = 1; i <= b->nx - 1; i++) {
+        size_t s = IX(b, i, j, k);
+        if (c->Iflag_local_dt == 1) {
+            double dt = ((dt_fac * c->CFL) * b->vol[s]) / (((b->Lci[s] + b->Lcj[s]) + b->Lck[s]) + 1.0 * ((b->Lvi[s] + b->Lvj[s]) + b->Lvk[s]));
+            if (c->If_dtime_mesh == 1) dt = dt * b->dtime_mesh[s];
+            if (dt > c->dtmax) dt = c->dtmax;
+            if (dt < c->dtmin) dt = c->dtmin;
+            b->dt[s] = dt;
+        } else {
+            b->dt[s] = c->dt_global;
+        }
+    }
+}
+
+/* comput_DFn (sub_LU_SGS.f90:135-158); U is the neighbour state (after limit_vt), n the normal */
+static void comput_dfn(int NV, double *DF, const double *U, const double *DU, double n1, double n2, double n3, double gamma) {
+    double U2[ORC_MAXV];
+    for (int m = 0; m < NV; m++) U2[m] = U[m] + DU[m];
+    double un1 = ((U[1] * n1 + U[2] * n2) + U[3] * n3) / U[0];
+    double p1 = (gamma - 1.0) * (U[4] - (0.5 * ((U[1] * U[1] + U[2] * U[2]) + U[3] * U[3])) / U[0]);
+    double un2 = ((U2[1] * n1 + U2[2] * n2) + U2[3] * n3) / U2[0];
+    double p2 = (gamma - 1.0) * (U2[4] - (0.5 * ((U2[1] * U2[1] + U2[2] * U2[2]) + U2[3] * U2[3])) / U2[0]);
+    DF[0] = U2[0] * un2 - U[0] * un1;
+    DF[1] = (U2[1] * un2 + p2 * n1) - (U[1] * un1 + p1 * n1);
+    DF[2] = (U2[2] * un2 + p2 * n2) - (U[2] * un1 + p1 * n2);
+    DF[3] = (U2[3] * un2 + p2 * n3) - (U[3] * un1 + p1 * n3);
+    DF[4] = (U2[4] + p2) * un2 - (U[4] + p1) * un1;
+    if (NV == 6) DF[5] = U2[5] * un2 - U[5] * un1;
+    else if (NV == 7) { DF[5] = U2[5] * un2 - U[5] * un1; DF[6] = U2[6] * un2 - U[6] * un1; }
+}
+
+/* du_LU_SGS (sub_LU_SGS.f90:4-129) with Sfac1 = 0 */
+static void du_lusgs(orc_ctx *c, orc_block *b) {
+    int NV = c->NVAR, nx = b->nx, ny = b->ny, nz = b->nz;
+    double w = c->w_LU, gamma = c->gamma;
+    int visc = c->If_viscous == 1;
+    for (int plane = 3; plane <= nx + ny + nz - 3; plane++)
+        for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) {
+            int i = plane - k - j;
+            if (i < 1 || i > nx - 1) continue;
+            size_t s = IX(b, i, j, k);
+            double alfa[ORC_MAXV], dui[ORC_MAXV], duj[ORC_MAXV], duk[ORC_MAXV], DF[ORC_MAXV], Unb[ORC_MAXV], DUnb[ORC_MAXV];
+            for (int m = 0; m < NV; m++) alfa[m] = b->vol[s] / b->dt[s] + w * ((b->Lci[s] + b->Lcj[s]) + b->Lck[s]);
+            if (visc) {
+                for (int m = 0; m < NV; m++) alfa[m] = alfa[m] + 2.0 * ((b->Lvi[s] + b->Lvj[s]) + b->Lvk[s]);
+                if (NV == 6) alfa[5] = alfa[5] + ((b->Lvi[s] + b->Lvj[s]) + b->Lvk[s]);
+            }
+            for (int m = 0; m < NV; m++) alfa[m] = alfa[m] + 0.0 * b->vol[s];
+            for (int dir = 0; dir < 3; dir++) {
+                double *du = dir == 0 ? dui : dir == 1 ? duj : duk;
+                int pos = dir == 0 ? i : dir == 1 ? j : k;
+                if (pos != 1) {
+                    size_t t = IX(b, i - (dir == 0), j - (dir == 1), k - (dir == 2));
+                    for (int m = 0; m < NV; m++) { Unb[m] = b->U[m][t]; DUnb[m] = b->DU[m][t]; }
+                    if (NV >= 6) for (int m = 5; m < NV; m++) if (Unb[m] < 0.0) Unb[m] = 0.0;   /* limit_vt */
+                    comput_dfn(NV, DF, Unb, DUnb, b->nv[3 * dir][s], b->nv[3 * dir + 1][s], b->nv[3 * dir + 2][s], gamma);
+                    const double *Sd = dir == 0 ? b->Si : dir == 1 ? b->Sj : b->Sk;
+                    const double *Lc = dir == 0 ? b->Lci : dir == 1 ? b->Lcj : b->Lck;
+                    const double *Lv = dir == 0 ? b->Lvi : dir == 1 ? b->Lvj : b->Lvk;
+                    for (int m = 0; m < NV; m++) du[m] = 0.5 * (DF[m] * Sd[s] + (w * Lc[t]) * DUnb[m]);
+                    if (visc) for (int m = 0; m < NV; m++) du[m] = du[m] + Lv[t] * DUnb[m];
+                } else {
+                    for (int m = 0; m < NV; m++) du[m] = 0.0;
+                }
+            }
+            for (int m = 0; m < NV; m++) b->DU[m][s] = (((b->Res[m][s] + dui[m]) + duj[m]) + duk[m]) / alfa[m];
+        }
+    for (int plane = nx + ny + nz - 3; plane >= 3; plane--)
+        for (int k = nz - 1; k >= 1; k--) for (int j = ny - 1; j >= 1; j--) {
+            int i = plane - k - j;
+            if (i < 1 || i > nx - 1) continue;
+            size_t s = IX(b, i, j, k);
+            double alfa[ORC_MAXV], dui[ORC_MAXV], duj[ORC_MAXV], duk[ORC_MAXV], DF[ORC_MAXV], Unb[ORC_MAXV], DUnb[ORC_MAXV];
+            for (int m = 0; m < NV; m++) alfa[m] = b->vol[s] / b->dt[s] + w * ((b->Lci[s] + b->Lcj[s]) + b->Lck[s]);
+            if (visc) {
+                for (int m = 0; m < NV; m++) alfa[m] = alfa[m] + 2.0 * ((b->Lvi[s] + b->Lvj[s]) + b->Lvk[s]);
+                if (NV == 6) alfa[5] = ((alfa[5] + b->Lvi[s]) + b->Lvj[s]) + b->Lvk[s];
+            }
+            for (int m = 0; m < NV; m++) alfa[m] = alfa[m] + 0.0 * b->vol[s];
+            for (int dir = 0; dir < 3; dir++) {
+                double *du = dir == 0 ? dui : dir == 1 ? duj : duk;
+                int pos = dir == 0 ? i : dir == 1 ? j : k;
+                int lim = dir == 0 ? nx - 1 : dir == 1 ? ny - 1 : nz - 1;
+                if (pos != lim) {
+                    size_t t = IX(b, i + (dir == 0), j + (dir == 1), k + (dir == 2));
+                    for (int m = 0; m < NV; m++) { Unb[m] = b->U[m][t]; DUnb[m] = b->DU[m][t]; }
+                    if (NV >= 6) for (int m = 5; m < NV; m++) if (Unb[m] < 0.0) Unb[m] = 0.0;
+                    /* own-face normal n(i) with the neighbour state (sub_LU_SGS.f90:96,105,114) */
+                    comput_dfn(NV, DF, Unb, DUnb, b->nv[3 * dir][s], b->nv[3 * dir + 1][s], b->nv[3 * dir + 2][s], gamma);
+                    const double *Sd = dir == 0 ? b->Si : dir == 1 ? b->Sj : b->Sk;
+                    const double *Lc = dir == 0 ? b->Lci : dir == 1 ? b->Lcj : b->Lck;
+                    const double *Lv = dir == 0 ? b->Lvi : dir == 1 ? b->Lvj : b->Lvk;
+                    for (int m = 0; m < NV; m++) du[m] = -0.5 * (DF[m] * Sd[t] - (w * Lc[t]) * DUnb[m]);
+                    if (visc) for (int m = 0; m < NV; m++) du[m] = du[m] + Lv[t] * DUnb[m];
+                } else {
+                    for (int m = 0; m < NV; m++) du[m] = 0.0;
+                }
+            }
+            for (int m = 0; m < NV; m++) b->DU[m][s] = b->DU[m][s] + ((dui[m] + duj[m]) + duk[m]) / alfa[m];
+        }
+}
+
+/* Comput_Residual_one_mesh (sub_Residual.f90:14-127) preceded by Set_Un (sub_time_advance.f90:635-656) */
+void orc_residual_all(orc_ctx *c) {
+    for (int mb = 0; mb < c->nblk; mb++) {
+        orc_block *b = &c->blk[mb];
+        for (int m = 0; m < c->NVAR; m++) memcpy(b->Un[m], b->U[m], b->n * sizeof(double));
+    }
+    for (int mb = 0; mb < c->nblk; mb++) {
+        orc_block *b = &c->blk[mb];
+        /* limit_vt (sub_time_advance.f90:410-443) */
+        if (c->NVAR >= 6)
+            for (int k = 0; k <= b->nz; k++) for (int j = 0; j <= b->ny; j++) for (int i = 0; i <= b->nx; i++)
+                for (int m = 5; m < c->NVAR; m++) if (b->U[m][IX(b, i, j, k)] < 0.0) b->U[m][IX(b, i, j, k)] = 0.0;
+        primitives(c, b);
+        if (c->If_viscous == 1) {
+            get_viscous(c, b);
+            if (c->turb == 2) turbulence_sa(c, b);
+            else if (c->turb == 3) turbulence_sst(c, b);
+            else if (c->turb == 0) memset(b->mut, 0, b->n * sizeof(double));
+            else { fprintf(stderr, "oracle: turbulence model %d not supported\n", c->turb); abort(); }
+            limit_mut(c, b);
+        }
+        flux_inviscid(c, b, 0);
+        flux_inviscid(c, b, 1);
+        flux_inviscid(c, b, 2);
+        if (c->If_viscous == 1) { flux_viscous(c, b, 0); flux_viscous(c, b, 1); flux_viscous(c, b, 2); }
+        comput_lijk(c, b);
+        comput_dt(c, b);
+        du_lusgs(c, b);
+    }
+}
+
+/* U = Un + DU (sub_time_advance.f90:92-101), then limit_flow (sub_limitflow.f90:3-231) */
+void orc_update_limit(orc_ctx *c) {
+    for (int mb = 0; mb < c->nblk; mb++) {
+        orc_block *b = &c->blk[mb];
+        for (int k = 1; k <= b->nz - 1; k++) for (int j = 1; j <= b->ny - 1; j++) for (int i = 1; i <= b->nx - 1; i++)
+            for (int m = 0; m < c->NVAR; m++) { size_t s = IX(b, i, j, k); b->U[m][s] = b->Un[m][s] + b->DU[m][s]; }
+    }
+    if (c->IFLAG_LIMIT_FLOW != 1) return;
+    double g = c->gamma;
+    for (int mb = 0; mb < c->nblk; mb++) {
+        orc_block *b = &c->blk[mb];
+        int nx = b->nx, ny = b->ny, nz = b->nz;
+        double *d1 = (double *)calloc(b->n, sizeof(double)), *u1 = (double *)calloc(b->n, sizeof(double));
+        double *v1 = (double *)calloc(b->n, sizeof(double)), *w1 = (double *)calloc(b->n, sizeof(double));
+        double *p1 = (double *)calloc(b->n, sizeof(double));
+        for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+            size_t s = IX(b, i, j, k);
+            d1[s] = b->U[0][s]; u1[s] = b->U[1][s] / d1[s]; v1[s] = b->U[2][s] / d1[s]; w1[s] = b->U[3][s] / d1[s];
+            p1[s] = (b->U[4][s] - (0.5 * d1[s]) * ((u1[s] * u1[s] + v1[s] * v1[s]) + w1[s] * w1[s])) * (g - 1.0);
+        }
+#define BAD(s) (d1[s] < c->Ldmin || p1[s] < c->Lpmin || d1[s] > c->Ldmax || p1[s] > c->Lpmax || fabs(u1[s]) > c->Lumax || fabs(v1[s]) > c->Lumax || fabs(w1[s]) > c->Lumax)
+        int nneg = 0;
+        for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+            size_t s = IX(b, i, j, k);
+            if (!BAD(s)) continue;
+            nneg++;
+            double d0 = 0, u0 = 0, v0 = 0, w0 = 0, p0 = 0; int kn = 0;
+            for (int ka = -1; ka <= 1; ka++) for (int ja = -1; ja <= 1; ja++) for (int ia = -1; ia <= 1; ia++) {
+                int i1 = i + ia, j1 = j + ja, k1 = k + ka;
+                if (i1 > 0 && i1 < nx && j1 > 0 && j1 < ny && k1 > 0 && k1 < nz) {
+                    size_t t = IX(b, i1, j1, k1);
+                    if (!BAD(t)) { d0 = d0 + d1[t]; u0 = u0 + u1[t]; v0 = v0 + v1[t]; w0 = w0 + w1[t]; p0 = p0 + p1[t]; kn++; }
+                }
+            }
+            if (kn == 0) {
+                d0 = d1[s]; u0 = u1[s]; v0 = v1[s]; w0 = w1[s]; p0 = p1[s];
+                if (d0 < c->Ldmin) d0 = c->Ldmin;
+                if (d0 > c->Ldmax) d0 = c->Ldmax;
+                if (p0 < c->Lpmin) p0 = c->Lpmin;
+                if (p0 > c->Lpmax) p0 = c->Lpmax;
+                if (fabs(u0) > c->Lumax) u0 = copysign(c->Lumax, u0);
+                if (fabs(v0) > c->Lumax) v0 = copysign(c->Lumax, v0);
+                if (fabs(w0) > c->Lumax) w0 = copysign(c->Lumax, w0);
+            } else {
+                double sn1 = 1.0 / (double)kn;
+                d0 = d0 * sn1; u0 = u0 * sn1; v0 = v0 * sn1; w0 = w0 * sn1; p0 = p0 * sn1;
+            }
+            b->U[0][s] = d0; b->U[1][s] = d0 * u0; b->U[2][s] = d0 * v0; b->U[3][s] = d0 * w0;
+            b->U[4][s] = p0 / (g - 1.0) + (0.5 * d0) * ((u0 * u0 + v0 * v0) + w0 * w0);
+        }
+#undef BAD
+        b->nneg_last = nneg;
+        if (nneg > 0) b->IF_OverLimit = 1;
+        free(d1); free(u1); free(v1); free(w1); free(p1);
+    }
+    if (c->NVAR == 6) {
+        const double SAmin = 1.0e-8;
+        for (int mb = 0; mb < c->nblk; mb++) {
+            orc_block *b = &c->blk[mb];
+            int nx = b->nx, ny = b->ny, nz = b->nz;
+            double *U6 = b->U[5];
+            for (int k = 1; k <= nz - 1; k++) for (int j = 1; j <= ny - 1; j++) for (int i = 1; i <= nx - 1; i++) {
+                size_t s = IX(b, i, j, k);
+                if (U6[s] < SAmin) U6[s] = SAmin;
+                if (U6[s] > c->LSAmax) {
+                    double s0 = 0.0; int kn = 0;
+                    for (int ka = -1; ka <= 1; ka++) for (int ja = -1; ja <= 1; ja++) for (int ia = -1; ia <= 1; ia++) {
+                        int i1 = i + ia, j1 = j + ja, k1 = k + ka;
+                        if (i1 > 0 && i1 < nx && j1 > 0 && j1 < ny && k1 > 0 && k1 < nz) {
+                            double t = U6[IX(b, i1, j1, k1)];
+                            if (t >= SAmin && t <= c->LSAmax) { s0 = s0 + t; kn++; }
+                        }
+                    }
+                    U6[s] = kn == 0 ? c->LSAmax : s0 / (double)kn;
+                }
+            }
+        }
+    }
+}
+
+/* force_vt_kw (sub_time_advance.f90:27-61) */
+void orc_force_vt_kw(orc_ctx *c) {
+    if (c->NVAR < 6) return;
+    for (int mb = 0; mb < c->nblk; mb++) {
+        orc_block *b = &c->blk[mb];
+        for (int k = 1; k <= b->nz - 1; k++) for (int j = 1; j <= b->ny - 1; j++) for (int i = 1; i <= b->nx - 1; i++)
+            for (int m = 5; m < c->NVAR; m++) if (b->U[m][IX(b, i, j, k)] < 0) b->U[m][IX(b, i, j, k)] = 1.0e-10;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------
+ * boundary conditions (sub_boundary.f90:6-504)
+ */
+static void wall_bound(orc_ctx *c, orc_block *b, size_t s1, size_t sg, size_t smu) {
+    int NV = c->NVAR;
+    double g = c->gamma, Ma = c->Ma;
+    double U1[ORC_MAXV], Ug[ORC_MAXV];
+    for (int m = 0; m < NV; m++) U1[m] = b->U[m][s1];
+    if (c->Twall > 0.0) {
+        double d1 = U1[0], uu1 = U1[1] / d1, v1 = U1[2] / d1, w1 = U1[3] / d1;
+        double p1 = (U1[4] - (0.5 * d1) * ((uu1 * uu1 + v1 * v1) + w1 * w1)) * (g - 1.0);
+        double T1 = ((g * Ma) * Ma) * p1 / d1;
+        double p2 = p1, T2 = 2.0 * c->Twall - T1;
+        if (T2 < 0.5 * T1) T2 = 0.5 * T1;
+        double uu2 = -uu1, v2 = -v1, w2 = -w1;
+        double d2 = ((g * Ma) * Ma) * p2 / T2;
+        Ug[0] = d2; Ug[1] = d2 * uu2; Ug[2] = d2 * v2; Ug[3] = d2 * w2;
+        Ug[4] = p2 / (g - 1.0) + (0.5 * d2) * ((uu2 * uu2 + v2 * v2) + w2 * w2);
+    } else {
+        Ug[0] = U1[0]; Ug[1] = -U1[1]; Ug[2] = -U1[2]; Ug[3] = -U1[3]; Ug[4] = U1[4];
+    }
+    if (NV == 6) Ug[5] = 0.0;
+    else if (NV == 7) {
+        const double beta1 = 0.075;
+        double dw = b->dw[smu];
+        double wt = (60.0 * b->mu[smu]) / ((((U1[0] * beta1) * (dw * dw)) * c->Re));
+        Ug[5] = -U1[5];
+        Ug[6] = (U1[0] + Ug[0]) * wt - U1[6];
+    }
+    for (int m = 0; m < NV; m++) b->U[m][sg] = Ug[m];
+}
+
+static void boundary_wall(orc_ctx *c, orc_block *b, orc_bcmsg *B) {
+    int ib = B->ib, ie = B->ie, jb = B->jb, je = B->je, kb = B->kb, ke = B->ke;
+    switch (B->face) {
+    case 1: for (int k = kb; k <= ke - 1; k++) for (int j = jb; j <= je - 1; j++) wall_bound(c, b, IX(b, 1, j, k), IX(b, 0, j, k), IX(b, 1, j, k)); break;
+    case 4: for (int k = kb; k <= ke - 1; k++) for (int j = jb; j <= je - 1; j++) wall_bound(c, b, IX(b, ie - 1, j, k), IX(b, ie, j, k), IX(b, ie - 1, j, k)); break;
+    case 2: for (int k = kb; k <= ke - 1; k++) for (int i = ib; i <= ie - 1; i++) wall_bound(c, b, IX(b, i, 1, k), IX(b, i, 0, k), IX(b, i, 1, k)); break;
+    case 5: for (int k = kb; k <= ke - 1; k++) for (int i = ib; i <= ie - 1; i++) wall_bound(c, b, IX(b, i, je - 1, k), IX(b, i, je, k), IX(b, i, je - 1, k)); break;
+    case 3: for (int j = jb; j <= je - 1; j++) for (int i = ib; i <= ie - 1; i++) wall_bound(c, b, IX(b, i, j, 1), IX(b, i, j, 0), IX(b, i, j, 1)); break;
+    case 6: for (int j = jb; j <= je - 1; j++) for (int i = ib; i <= ie - 1; i++) wall_bound(c, b, IX(b, i, j, ke - 1), IX(b, i, j, ke), IX(b, i, j, ke - 1)); break;
+    }
+}
+
+static void boundary_farfield(orc_ctx *c, orc_block *b, orc_bcmsg *B, int flag) {
+    const int FLAG_OUTLET = 1, FLAG_INLET = 2;
+    int NV = c->NVAR;
+    double g = c->gamma;
+    double d_inf = 1.0, u_inf = cos(c->A_alfa) * cos(c->A_beta), v_inf = sin(c->A_alfa) * cos(c->A_beta), w_inf = sin(c->A_beta);
+    double p_inf = 1.0 / (g * c->Ma * c->Ma);
+    for (int k = B->kb; k <= B->ke; k++) for (int j = B->jb; j <= B->je; j++) for (int i = B->ib; i <= B->ie; i++) {
+        int i1 = i, j1 = j, k1 = k, i2 = i, j2 = j, k2 = k;
+        double n1, n2, n3;
+        size_t s = IX(b, i, j, k);
+        switch (B->face) {
+        case 1: i2 = i - 1; n1 = -b->nv[0][s]; n2 = -b->nv[1][s]; n3 = -b->nv[2][s]; break;
+        case 2: j2 = j - 1; n1 = -b->nv[3][s]; n2 = -b->nv[4][s]; n3 = -b->nv[5][s]; break;
+        case 3: k2 = k - 1; n1 = -b->nv[6][s]; n2 = -b->nv[7][s]; n3 = -b->nv[8][s]; break;
+        case 4: i1 = i - 1; n1 = b->nv[0][s]; n2 = b->nv[1][s]; n3 = b->nv[2][s]; break;
+        case 5: j1 = j - 1; n1 = b->nv[3][s]; n2 = b->nv[4][s]; n3 = b->nv[5][s]; break;
+        default: k1 = k - 1; n1 = b->nv[6][s]; n2 = b->nv[7][s]; n3 = b->nv[8][s]; break;
+        }
+        size_t s1 = IX(b, i1, j1, k1), s2 = IX(b, i2, j2, k2);
+        double d1 = b->U[0][s1], u1 = b->U[1][s1] / d1, v1 = b->U[2][s1] / d1, w1 = b->U[3][s1] / d1;
+        double p1 = (b->U[4][s1] - (0.5 * d1) * ((u1 * u1 + v1 * v1) + w1 * w1)) * (g - 1.0);
+        double c1 = sqrt(g * p1 / d1);
+        double d2, u2, v2, w2, p2, Ma_n = 0.0;
+        if (flag == FLAG_OUTLET) { d2 = d1; u2 = u1; v2 = v1; w2 = w1; p2 = p1; }
+        else if (flag == FLAG_INLET) { d2 = d_inf; u2 = u_inf; v2 = v_inf; w2 = w_inf; p2 = p_inf; }
+        else {
+            if (c->P_outlet <= -1.0) Ma_n = ((u_inf * n1 + v_inf * n2) + w_inf * n3) * c->Ma;
+            else Ma_n = ((u1 * n1 + v1 * n2) + w1 * n3) / c1;
+            if (Ma_n > 1.0) { d2 = d1; u2 = u1; v2 = v1; w2 = w1; p2 = p1; }
+            else if (Ma_n > -1.0e-6) {
+                if (c->P_outlet > 0.0) {
+                    double pb = c->P_outlet;
+                    double db = d1 + (pb - p1) / (c1 * c1);
+                    double ub = u1 + ((p1 - pb) / (d1 * c1)) * n1, vb = v1 + ((p1 - pb) / (d1 * c1)) * n2, wb = w1 + ((p1 - pb) / (d1 * c1)) * n3;
+                    p2 = 2.0 * pb - p1; d2 = 2.0 * db - d1; u2 = 2.0 * ub - u1; v2 = 2.0 * vb - v1; w2 = 2.0 * wb - w1;
+                } else { d2 = d1; u2 = u1; v2 = v1; w2 = w1; p2 = p1; }
+            } else if (Ma_n > -1.0) {
+                double pb = 0.5 * ((p1 + p_inf) - (d1 * c1) * (((u_inf - u1) * n1 + (v_inf - v1) * n2) + (w_inf - w1) * n3));
+                double db = d_inf + (pb - p_inf) / (c1 * c1);
+                double ub = u_inf - ((p_inf - pb) / (d1 * c1)) * n1, vb = v_inf - ((p_inf - pb) / (d1 * c1)) * n2, wb = w_inf - ((p_inf - pb) / (d1 * c1)) * n3;
+                p2 = 2.0 * pb - p1; d2 = 2.0 * db - d1; u2 = 2.0 * ub - u1; v2 = 2.0 * vb - v1; w2 = 2.0 * wb - w1;
+            } else { d2 = d_inf; u2 = u_inf; v2 = v_inf; w2 = w_inf; p2 = p_inf; }
+        }
+        b->U[0][s2] = d2; b->U[1][s2] = d2 * u2; b->U[2][s2] = d2 * v2; b->U[3][s2] = d2 * w2;
+        b->U[4][s2] = p2 / (g - 1.0) + (0.5 * d2) * ((u2 * u2 + v2 * v2) + w2 * w2);
+        if (Ma_n > 0.0 || flag == FLAG_OUTLET) {
+            if (NV == 6) b->U[5][s2] = b->U[5][s1];
+            else if (NV == 7) { b->U[5][s2] = b->U[5][s1]; b->U[6][s2] = b->U[6][s1]; }
+        } else {
+            if (NV == 6) b->U[5][s2] = 5.0;
+            else if (NV == 7) { b->U[5][s2] = d_inf * c->Kt_inf; b->U[6][s2] = d_inf * c->Wt_inf; }
+        }
+    }
+}
+
+static void boundary_symmetry(orc_ctx *c, orc_block *b, orc_bcmsg *B) {
+    int NV = c->NVAR;
+    int ib = B->ib, ie = B->ie, jb = B->jb, je = B->je, kb = B->kb, ke = B->ke;
+    if (B->face == 1 || B->face == 4) {
+        int i = B->face == 1 ? ib : ie, i1 = B->face == 1 ? ib : ie - 1, i2 = B->face == 1 ? ib - 1 : ie;
+        for (int k = kb; k <= ke - 1; k++) for (int j = jb; j <= je - 1; j++) {
+            size_t s1 = IX(b, i1, j, k), s2 = IX(b, i2, j, k), sn = IX(b, i, j, k);
+            for (int m = 0; m < NV; m++) b->U[m][s2] = b->U[m][s1];
+            double n1 = b->nv[0][sn], n2 = b->nv[1][sn], n3 = b->nv[2][sn];
+            double Vn = (b->U[1][s1] * n1 + b->U[2][s1] * n2) + b->U[3][s1] * n3;
+            b->U[1][s2] = b->U[1][s2] - (2.0 * Vn) * n1; b->U[2][s2] = b->U[2][s2] - (2.0 * Vn) * n2; b->U[3][s2] = b->U[3][s2] - (2.0 * Vn) * n3;
+        }
+    } else if (B->face == 2 || B->face == 5) {
+        int j = B->face == 2 ? jb : je, j1 = B->face == 2 ? jb : je - 1, j2 = B->face == 2 ? jb - 1 : je;
+        for (int k = kb; k <= ke - 1; k++) for (int i = ib; i <= ie - 1; i++) {
+            size_t s1 = IX(b, i, j1, k), s2 = IX(b, i, j2, k), sn = IX(b, i, j, k);
+            for (int m = 0; m < NV; m++) b->U[m][s2] = b->U[m][s1];
+            double n1 = b->nv[3][sn], n2 = b->nv[4][sn], n3 = b->nv[5][sn];
+            double Vn = (b->U[1][s1] * n1 + b->U[2][s1] * n2) + b->U[3][s1] * n3;
+            b->U[1][s2] = b->U[1][s2] - (2.0 * Vn) * n1; b->U[2][s2] = b->U[2][s2] - (2.0 * Vn) * n2; b->U[3][s2] = b->U[3][s2] - (2.0 * Vn) * n3;
+        }
+    } else {
+        int k = B->face == 3 ? kb : ke, k1 = B->face == 3 ? kb : ke - 1, k2 = B->face == 3 ? kb - 1 : ke;
+        for (int j = jb; j <= je - 1; j++) for (int i = ib; i <= ie - 1; i++) {
+            size_t s1 = IX(b, i, j, k1), s2 = IX(b, i, j, k2), sn = IX(b, i, j, k);
+            for (int m = 0; m < NV; m++) b->U[m][s2] = b->U[m][s1];
+            double n1 = b->nv[6][sn], n2 = b->nv[7][sn], n3 = b->nv[8][sn];
+            double Vn = (b->U[1][s1] * n1 + b->U[2][s1] * n2) + b->U[3][s1] * n3;
+            b->U[1][s2] = b->U[1][s2] - (2.0 * Vn) * n1; b->U[2][s2] = b->U[2][s2] - (2.0 * Vn) * n2; b->U[3][s2] = b->U[3][s2] - (2.0 * Vn) * n3;
+        }
+    }
+}
+
+static void boundary_extrapolate(orc_ctx *c, orc_block *b, orc_bcmsg *B) {
+    for (int k = B->kb; k <= B->ke; k++) for (int j = B->jb; j <= B->je; j++) for (int i = B->ib; i <= B->ie; i++) {
+        int i1 = i, j1 = j, k1 = k, i2 = i, j2 = j, k2 = k;
+        switch (B->face) {
+        case 1: i2 = i - 1; break; case 2: j2 = j - 1; break; case 3: k2 = k - 1; break;
+        case 4: i1 = i - 1; break; case 5: j1 = j - 1; break; default: k1 = k - 1; break;
+        }
+        for (int m = 0; m < c->NVAR; m++) b->U[m][IX(b, i2, j2, k2)] = b->U[m][IX(b, i1, j1, k1)];
+    }
+}
+
+void orc_boundary(orc_ctx *c) {
+    for (int mb = 0; mb < c->nblk; mb++) {
+        orc_block *b = &c->blk[mb];
+        for (int q = 0; q < b->nsub; q++) {
+            orc_bcmsg *B = &b->bc[q];
+            if (B->bc <= 0) continue;
+            if (B->bc == BCW && c->If_viscous == 1) boundary_wall(c, b, B);
+            else if (B->bc == BCFAR) boundary_farfield(c, b, B, 0);
+            else if (B->bc == BCIN) boundary_farfield(c, b, B, 2);
+            else if (B->bc == BCOUT) boundary_farfield(c, b, B, 1);
+            else if (B->bc == BCSYM || (B->bc == BCW && c->If_viscous == 0)) boundary_symmetry(c, b, B);
+            else if (B->bc == BCEXT) boundary_extrapolate(c, b, B);
+            else { fprintf(stderr, "oracle: boundary condition %d not supported\n", B->bc); abort(); }
+        }
+    }
+}
+
+/* get_U_conner (sub_update_buffer_mpi.f90:515-549) */
+static void u_corner(orc_block *b, double *U) {
+    int nx = b->nx, ny = b->ny, nz = b->nz;
+#define V(i, j, k) U[IX(b, i, j, k)]
+    for (int k = 1; k <= nz - 1; k++) {
+        V(0, 0, k) = (V(1, 0, k) + V(0, 1, k)) * 0.5;
+        V(nx, 0, k) = (V(nx - 1, 0, k) + V(nx, 1, k)) * 0.5;
+        V(0, ny, k) = (V(1, ny, k) + V(0, ny - 1, k)) * 0.5;
+        V(nx, ny, k) = (V(nx - 1, ny, k) + V(nx, ny - 1, k)) * 0.5;
+    }
+    for (int j = 1; j <= ny - 1; j++) {
+        V(0, j, 0) = (V(1, j, 0) + V(0, j, 1)) * 0.5;
+        V(nx, j, 0) = (V(nx - 1, j, 0) + V(nx, j, 1)) * 0.5;
+        V(0, j, nz) = (V(1, j, nz) + V(0, j, nz - 1)) * 0.5;
+        V(nx, j, nz) = (V(nx - 1, j, nz) + V(nx, j, nz - 1)) * 0.5;
+    }
+    for (int i = 1; i <= nx - 1; i++) {
+        V(i, 0, 0) = (V(i, 1, 0) + V(i, 0, 1)) * 0.5;
+        V(i, ny, 0) = (V(i, ny - 1, 0) + V(i, ny, 1)) * 0.5;
+        V(i, 0, nz) = (V(i, 1, nz) + V(i, 0, nz - 1)) * 0.5;
+        V(i, ny, nz) = (V(i, ny - 1, nz) + V(i, ny, nz - 1)) * 0.5;
+    }
+    V(0, 0, 0) = (2.0 * ((V(1, 0, 0) + V(0, 1, 0)) + V(0, 0, 1)) - ((V(1, 1, 0) + V(1, 0, 1)) + V(0, 1, 1))) / 3.0;
+    V(nx, 0, 0) = (2.0 * ((V(nx - 1, 0, 0) + V(nx, 1, 0)) + V(nx, 0, 1)) - ((V(nx - 1, 1, 0) + V(nx - 1, 0, 1)) + V(nx, 1, 1))) / 3.0;
+    V(0, ny, 0) = (2.0 * ((V(1, ny, 0) + V(0, ny - 1, 0)) + V(0, ny, 1)) - ((V(1, ny - 1, 0) + V(1, ny, 1)) + V(0, ny - 1, 1))) / 3.0;
+    V(nx, ny, 0) = (2.0 * ((V(nx - 1, ny, 0) + V(nx, ny - 1, 0)) + V(nx, ny, 1)) - ((V(nx - 1, ny - 1, 0) + V(nx - 1, ny, 1)) + V(nx, ny - 1, 1))) / 3.0;
+    V(0, 0, nz) = (2.0 * ((V(1, 0, nz) + V(0, 1, nz)) + V(0, 0, nz - 1)) - ((V(1, 1, nz) + V(1, 0, nz - 1)) + V(0, 1, nz - 1))) / 3.0;
+    V(nx, 0, nz) = (2.0 * ((V(nx - 1, 0, nz) + V(nx, 1, nz)) + V(nx, 0, nz - 1)) - ((V(nx - 1, 1, nz) + V(nx - 1, 0, nz - 1)) + V(nx, 1, nz - 1))) / 3.0;
+    V(0, ny, nz) = (2.0 * ((V(1, ny, nz) + V(0, ny - 1, nz)) + V(0, ny, nz - 1)) - ((V(1, ny - 1, nz) + V(1, ny, nz - 1)) + V(0, ny - 1, nz - 1))) / 3.0;
+    V(nx, ny, nz) = (2.0 * ((V(nx - 1, ny, nz) + V(nx, ny - 1, nz)) + V(nx, ny, nz - 1)) - ((V(nx - 1, ny - 1, nz) + V(nx - 1, ny, nz - 1)) + V(nx, ny - 1, nz - 1))) / 3.0;
+#undef V
+}
+
+/* update_buffer_onemesh: Umessage_send_mpi local copies (sub_update_buffer_mpi.f90:42-141)
+ * then Umessage_corner (:197-211). */
+void orc_halo(orc_ctx *c) {
+    for (int mb = 0; mb < c->nblk; mb++) {
+        orc_block *b = &c->blk[mb];
+        for (int q = 0; q < b->nsub; q++) {
+            orc_bcmsg *B = &b->bc[q];
+            if (B->bc >= 0) continue;
+            int kb[3], ke[3], kb1[3], ke1[3], L[3], P[3], ks[3];
+            orc_halo_map(B, LAP, kb, ke, kb1, ke1, L, P, ks);
+            orc_block *b1 = &c->blk[B->nb1 - 1];
+            for (int k = kb1[2]; k <= ke1[2]; k++) for (int j = kb1[1]; j <= ke1[1]; j++) for (int i = kb1[0]; i <= ke1[0]; i++) {
+                int ka[3] = {i - kb1[0], j - kb1[1], k - kb1[2]};
+                int i1 = ks[0] + ka[L[0] - 1] * P[0], j1 = ks[1] + ka[L[1] - 1] * P[1], k1 = ks[2] + ka[L[2] - 1] * P[2];
+                for (int m = 0; m < c->NVAR; m++) b1->U[m][IX(b1, i, j, k)] = b->U[m][IX(b, i1, j1, k1)];
+            }
+        }
+    }
+    for (int mb = 0; mb < c->nblk; mb++)
+        for (int m = 0; m < c->NVAR; m++) u_corner(&c->blk[mb], c->blk[mb].U[m]);
+}
+
+/* comput_max_Res_onemesh (sub_time_advance.f90:330-380), serial order.  Returns 1 on NaN. */
+int orc_max_res(orc_ctx *c, int *where) {
+    double mx[ORC_MAXV] = {0}, rms[ORC_MAXV] = {0};
+    for (int mb = 0; mb < c->nblk; mb++) {
+        orc_block *b = &c->blk[mb];
+        for (int k = 1; k <= b->nz - 1; k++) for (int j = 1; j <= b->ny - 1; j++) for (int i = 1; i <= b->nx - 1; i++)
+            for (int m = 0; m < c->NVAR; m++) {
+                double r = b->Res[m][IX(b, i, j, k)];
+                if (r != r) { if (where) { where[0] = mb + 1; where[1] = i; where[2] = j; where[3] = k; where[4] = m + 1; } return 1; }
+                if (fabs(r) > mx[m]) mx[m] = fabs(r);
+                rms[m] = rms[m] + r * r;
+            }
+    }
+    for (int m = 0; m < c->NVAR; m++) { c->Res_max[m] = mx[m]; c->Res_rms[m] = sqrt(rms[m] / (double)c->Num_Cell); }
+    return 0;
+}
+
+/* comput_force (sub_Post.f90:4-265) */
+void orc_force(orc_ctx *c) {
+    double g = c->gamma, p_inf = 1.0 / (g * c->Ma * c->Ma);
+    double Ft[9] = {0};
+    double *Px1 = calloc((size_t)c->nblk * 9, sizeof(double));
+    for (int mb = 0; mb < c->nblk; mb++) {
+        orc_block *b = &c->blk[mb];
+        int nx = b->nx, ny = b->ny, nz = b->nz;
+        double *A = Px1 + 9 * mb;   /* Px Py Pz Cfx Cfy Cfz Mx My Mz */
+        for (int q = 0; q < b->nsub; q++) {
+            orc_bcmsg *B = &b->bc[q];
+            if (B->bc != BCW) continue;
+            int f = B->face;
+            int d = (f - 1) % 3;
+            int lo1, hi1, lo2, hi2;   /* loop ranges over the two tangential dims (outer, inner) */
+            if (d == 0) { lo1 = B->kb; hi1 = B->ke - 1; lo2 = B->jb; hi2 = B->je - 1; }
+            else if (d == 1) { lo1 = B->kb; hi1 = B->ke - 1; lo2 = B->ib; hi2 = B->ie - 1; }
+            else { lo1 = B->jb; hi1 = B->je - 1; lo2 = B->ib; hi2 = B->ie - 1; }
+            for (int o = lo1; o <= hi1; o++) for (int in = lo2; in <= hi2; in++) {
+                int i, j, k;
+                size_t s1, s0, sf, sm;
+                double *Sf;
+                int sgn = f <= 3 ? -1 : 1;
+                if (d == 0) { j = in; k = o; int ia = f == 1 ? 1 : nx - 1, ig = f == 1 ? 0 : nx, iface = f == 1 ? 1 : nx;
+                    s1 = IX(b, ia, j, k); s0 = IX(b, ig, j, k); sf = IX(b, iface, j, k); sm = JK(b, j, k); Sf = b->Si; i = 0; (void)i; }
+                else if (d == 1) { i = in; k = o; int ja = f == 2 ? 1 : ny - 1, jg = f == 2 ? 0 : ny, jface = f == 2 ? 1 : ny;
+                    s1 = IX(b, i, ja, k); s0 = IX(b, i, jg, k); sf = IX(b, i, jface, k); sm = IK(b, i, k); Sf = b->Sj; j = 0; (void)j; }
+                else { i = in; j = o; int ka = f == 3 ? 1 : nz - 1, kg = f == 3 ? 0 : nz, kface = f == 3 ? 1 : nz;
+                    s1 = IX(b, i, j, ka); s0 = IX(b, i, j, kg); sf = IX(b, i, j, kface); sm = IJ(b, i, j); Sf = b->Sk; k = 0; (void)k; }
+                double p1 = (g - 1.0) * (b->U[4][s1] - ((b->U[1][s1] * b->U[1][s1] + b->U[2][s1] * b->U[2][s1]) + b->U[3][s1] * b->U[3][s1]) / b->U[0][s1]);
+                double p2 = (g - 1.0) * (b->U[4][s0] - ((b->U[1][s0] * b->U[1][s0] + b->U[2][s0] * b->U[2][s0]) + b->U[3][s0] * b->U[3][s0]) / b->U[0][s0]);
+                double Pw = (f <= 3) ? -(0.5 * (p1 + p2) - p_inf) * Sf[sf] : (0.5 * (p1 + p2) - p_inf) * Sf[sf];
+                double n1 = b->nv[3 * d][sf], n2 = b->nv[3 * d + 1][sf], n3 = b->nv[3 * d + 2][sf];
+                double *const *Sv = b->Surf[f - 1];
+                A[0] = A[0] + Pw * n1; A[1] = A[1] + Pw * n2; A[2] = A[2] + Pw * n3;
+                double fx0, fy0, fz0;
+                if (sgn < 0) {
+                    A[3] = A[3] + Sv[0][sm]; A[4] = A[4] + Sv[1][sm]; A[5] = A[5] + Sv[2][sm];
+                    fx0 = Pw * n1 + Sv[0][sm]; fy0 = Pw * n2 + Sv[1][sm]; fz0 = Pw * n3 + Sv[2][sm];
+                } else {
+                    A[3] = A[3] - Sv[0][sm]; A[4] = A[4] - Sv[1][sm]; A[5] = A[5] - Sv[2][sm];
+                    fx0 = Pw * n1 - Sv[0][sm]; fy0 = Pw * n2 - Sv[1][sm]; fz0 = Pw * n3 - Sv[2][sm];
+                }
+                double xc = (b->xc[s1] + b->xc[s0]) * 0.5 - c->Centroid[0];
+                double yc = (b->yc[s1] + b->yc[s0]) * 0.5 - c->Centroid[1];
+                double zc = (b->zc[s1] + b->zc[s0]) * 0.5 - c->Centroid[2];
+                if (f <= 3) {   /* i-/j-/k- faces list the ghost cell second: (xc(1)+xc(0)) */
+                    xc = (b->xc[s1] + b->xc[s0]) * 0.5 - c->Centroid[0];
+                } else {        /* i+/j+/k+ faces: (xc(n-1)+xc(n)) */
+                    xc = (b->xc[s1] + b->xc[s0]) * 0.5 - c->Centroid[0];
+                }
+                A[6] = A[6] + (yc * fz0 - zc * fy0);
+                A[7] = A[7] + (zc * fx0 - xc * fz0);
+                A[8] = A[8] + (xc * fy0 - yc * fx0);
+            }
+        }
+    }
+    double Px = 0, Py = 0, Pz = 0, Cfx = 0, Cfy = 0, Cfz = 0, Mx = 0, My = 0, Mz = 0;
+    for (int mb = 0; mb < c->nblk; mb++) {
+        double *A = Px1 + 9 * mb;
+        Px = Px + A[0]; Py = Py + A[1]; Pz = Pz + A[2];
+        Cfx = Cfx + A[3]; Cfy = Cfy + A[4]; Cfz = Cfz + A[5];
+        Mx = Mx + A[6]; My = My + A[7]; Mz = Mz + A[8];
+    }
+    free(Px1);
+    Ft[0] = Px; Ft[1] = Py; Ft[2] = Pz; Ft[3] = Cfx; Ft[4] = Cfy; Ft[5] = Cfz; Ft[6] = Mx; Ft[7] = My; Ft[8] = Mz;
+    for (int q = 0; q < 6; q++) Ft[q] = 2.0 * Ft[q] / c->Ref_S;
+    for (int q = 6; q < 9; q++) Ft[q] = 2.0 * Ft[q] / (c->Ref_S * c->Ref_L);
+    double Fx = Ft[0] + Ft[3], Fy = Ft[1] + Ft[4], Fz = Ft[2] + Ft[5];
+    double CL, CD, CS;
+    if (c->Cood_Y_UP == 1) { CL = Fy * cos(c->AoA) - Fx * sin(c->AoA); CD = Fx * cos(c->AoA) + Fy * sin(c->AoA); CS = Fz; }
+    else { CL = Fz * cos(c->AoA) - Fx * sin(c->AoA); CD = Fx * cos(c->AoA) + Fz * sin(c->AoA); CS = Fy; }
+    for (int q = 0; q < 9; q++) c->force[q] = Ft[q];
+    c->force[9] = CL; c->force[10] = CD; c->force[11] = CS;
+}
+
+/* NS_Time_advance(1) with Time_Method = LU-SGS (sub_time_advance.f90:6-23,75-118) */
+int orc_step(orc_ctx *c) {
+    orc_residual_all(c);
+    orc_update_limit(c);
+    orc_boundary(c);
+    orc_halo(c);
+    c->tt = c->tt + c->dt_global;
+    c->Kstep = c->Kstep + 1;
+    orc_force_vt_kw(c);
+    return 0;
+}
+
+int orc_monitor(orc_ctx *c) {
+    orc_force(c);
+    int where[5];
+    if (orc_max_res(c, where)) return -1;
+    return 0;
+}

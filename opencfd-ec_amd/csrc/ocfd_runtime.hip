@@ -1,0 +1,785 @@
+// Runtime and C ABI of libocfd_ec.so: block storage in HBM, Fortran-layout transfers, the
+// per-iteration launch sequence (optionally captured in a hipGraph), residual norms and forces.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ocfd_ec.h"
+#include "ocfd_dev.h"
+#include "ocfd_launch.h"
+
+using namespace ocfd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) { g_err = msg; return code; }
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) return fail(OCFD_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct HostBlock {
+    int no = 0, nx = 0, ny = 0, nz = 0;
+    std::vector<Sub> subs;
+    Blk d{};
+    std::vector<void*> allocs;
+    double* force_acc = nullptr;     // 12 doubles: 9 accumulators + centroid
+    std::vector<double> nodes[3];    // node coordinates awaiting ocfd_setup_geometry
+    size_t n() const { return (size_t)d.NI * d.NJ * d.NK; }
+};
+
+}  // namespace
+
+struct ocfd_ctx {
+    ocfd_params hp{};
+    Par p{};
+    std::vector<HostBlock> blocks;   // index block_no - 1
+    hipStream_t st = nullptr;
+    double* norm_part = nullptr;
+    size_t norm_part_n = 0;
+    int kstep = 0;
+    double tt = 0.0;
+    bool use_graph = false;
+    hipGraphExec_t exec[OCFD_NSTAGES] = {};   // one captured graph per stage of the iteration
+    hipGraph_t graph[OCFD_NSTAGES] = {};
+    std::vector<hipEvent_t> events;           // stage boundaries of ocfd_step_timed
+    int device = 0;
+};
+
+namespace {
+
+int dalloc(HostBlock& hb, double** p, size_t n) {
+    void* q = nullptr;
+    HIPCHK(hipMalloc(&q, n * sizeof(double)));
+    HIPCHK(hipMemset(q, 0, n * sizeof(double)));
+    hb.allocs.push_back(q);
+    *p = (double*)q;
+    return OCFD_OK;
+}
+
+int balloc(HostBlock& hb, uint8_t** p, size_t n) {
+    void* q = nullptr;
+    HIPCHK(hipMalloc(&q, n));
+    HIPCHK(hipMemset(q, 0, n));
+    hb.allocs.push_back(q);
+    *p = (uint8_t*)q;
+    return OCFD_OK;
+}
+
+// allocate all device arrays of one block (the same padded box for every array)
+int alloc_block(ocfd_ctx* c, HostBlock& hb) {
+    Blk& b = hb.d;
+    b.nx = hb.nx; b.ny = hb.ny; b.nz = hb.nz;
+    b.NI = hb.nx + 7; b.NJ = hb.ny + 7; b.NK = hb.nz + 7;
+    size_t n = hb.n();
+    int NV = c->p.NV, r;
+    bool visc = c->p.visc == 1, turb = c->p.turb != 0;
+    for (int m = 0; m < NV; m++) {
+        if ((r = dalloc(hb, &b.U[m], n)) || (r = dalloc(hb, &b.Res[m], n)) || (r = dalloc(hb, &b.DU[m], n))) return r;
+    }
+    double** one[] = {&b.dt, &b.mu, &b.muf, &b.mut, &b.vol, &b.dtm, &b.x, &b.y, &b.z, &b.xc, &b.yc, &b.zc, &b.aux};
+    for (double** q : one) if ((r = dalloc(hb, q, n))) return r;
+    for (int q = 0; q < 7; q++) if ((r = dalloc(hb, &b.prim[q], n))) return r;
+    for (int q = 0; q < 3; q++) if ((r = dalloc(hb, &b.Lc[q], n)) || (r = dalloc(hb, &b.Lv[q], n)) || (r = dalloc(hb, &b.S[q], n))) return r;
+    for (int q = 0; q < 9; q++) if ((r = dalloc(hb, &b.nv[q], n))) return r;
+    for (int s = 1; s < 4; s++) for (int q = 0; q < 9; q++) if ((r = dalloc(hb, &b.met[s][q], n))) return r;
+    if (turb) {
+        for (int q = 0; q < 9; q++) if ((r = dalloc(hb, &b.met[0][q], n))) return r;
+        if ((r = dalloc(hb, &b.dw, n))) return r;
+    }
+    for (int dd = 0; dd < 3; dd++) for (int m = 0; m < 5; m++) if ((r = dalloc(hb, &b.F[dd][m], n))) return r;
+    if (visc) for (int dd = 0; dd < 3; dd++) for (int m = 0; m < 4; m++) if ((r = dalloc(hb, &b.Fv[dd][m], n))) return r;
+    size_t jk = (size_t)b.NJ * b.NK, ik = (size_t)b.NI * b.NK, ij = (size_t)b.NI * b.NJ;
+    if ((r = balloc(hb, &b.bcf[0], jk)) || (r = balloc(hb, &b.bcf[1], ik)) || (r = balloc(hb, &b.bcf[2], ij))) return r;
+    size_t fsz[6] = {jk, ik, ij, jk, ik, ij};
+    for (int f = 0; f < 6; f++) for (int q = 0; q < 3; q++) if ((r = dalloc(hb, &b.surf[f][q], fsz[f]))) return r;
+    void* fl = nullptr;
+    HIPCHK(hipMalloc(&fl, 16 * sizeof(int)));
+    HIPCHK(hipMemset(fl, 0, 16 * sizeof(int)));
+    hb.allocs.push_back(fl);
+    b.flags = (int*)fl;
+    if ((r = dalloc(hb, &hb.force_acc, 12))) return r;
+    // initial values of allocate_mem_Blocks (sub_init.f90:496,507-508)
+    std::vector<double> h(n, 1.0 / c->p.Re);
+    HIPCHK(hipMemcpy(b.mu, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.muf, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    std::fill(h.begin(), h.end(), 1.0);
+    HIPCHK(hipMemcpy(b.dtm, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    return OCFD_OK;
+}
+
+// copy a Fortran array with inclusive bounds lo..hi into the padded box of a block
+int put_box(const HostBlock& hb, double* dev, const double* src, const int lo[3], const int hi[3]) {
+    size_t n = hb.n();
+    std::vector<double> h(n, 0.0);
+    const Blk& b = hb.d;
+    long n0 = hi[0] - lo[0] + 1, n1 = hi[1] - lo[1] + 1;
+    for (int k = lo[2]; k <= hi[2]; k++)
+        for (int j = lo[1]; j <= hi[1]; j++)
+            for (int i = lo[0]; i <= hi[0]; i++)
+                h[IX(b, i, j, k)] = src[(i - lo[0]) + n0 * ((j - lo[1]) + n1 * (long)(k - lo[2]))];
+    HIPCHK(hipMemcpy(dev, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    return OCFD_OK;
+}
+
+int get_box(const HostBlock& hb, const double* dev, std::vector<double>& h) {
+    h.resize(hb.n());
+    HIPCHK(hipMemcpy(h.data(), dev, hb.n() * sizeof(double), hipMemcpyDeviceToHost));
+    return OCFD_OK;
+}
+
+HostBlock* find_block(ocfd_ctx* c, int no) {
+    if (!c || no < 1 || no > (int)c->blocks.size() || c->blocks[no - 1].no != no) return nullptr;
+    return &c->blocks[no - 1];
+}
+
+Par to_par(const ocfd_params& h) {
+    Par p{};
+    p.Ma = h.Ma; p.Re = h.Re; p.gamma = h.gamma; p.Cv = h.Cv; p.Cp = h.Cp; p.PrL = h.PrL; p.PrT = h.PrT;
+    p.T_inf = h.T_inf; p.Twall = h.Twall; p.AoA = h.AoA; p.A_alfa = h.A_alfa; p.A_beta = h.A_beta;
+    p.P_outlet = h.P_outlet; p.CFL = h.CFL; p.dtmax = h.dtmax; p.dtmin = h.dtmin; p.dt_global = h.dt_global;
+    p.w_LU = h.w_LU; p.MUT_MAX = h.MUT_MAX; p.Kt_inf = h.Kt_inf; p.Wt_inf = h.Wt_inf; p.Ldmin = h.Ldmin;
+    p.Ldmax = h.Ldmax; p.Lpmin = h.Lpmin; p.Lpmax = h.Lpmax; p.Lumax = h.Lumax; p.LSAmax = h.LSAmax;
+    p.NV = h.NVAR; p.visc = h.If_viscous; p.turb = h.If_viscous == 1 ? h.Iflag_turbulence_model : 0;
+    p.scheme = h.Iflag_Scheme; p.flux = h.Iflag_Flux; p.recon = h.IFlag_Reconstruction; p.bscheme = h.Bound_Scheme;
+    p.local_dt = h.Iflag_local_dt; p.dtime_mesh = h.If_dtime_mesh; p.positivity = h.IF_Scheme_Positivity;
+    p.limit_flow = h.IFLAG_LIMIT_FLOW;
+    return p;
+}
+
+// boundary indicators BcI/BcJ/BcK (set_BcK, sub_init.f90:545-577) as per-row bit flags
+int upload_bcflags(HostBlock& hb) {
+    const Blk& b = hb.d;
+    std::vector<uint8_t> fi((size_t)b.NJ * b.NK, 0), fj((size_t)b.NI * b.NK, 0), fk((size_t)b.NI * b.NJ, 0);
+    for (const Sub& B : hb.subs) {
+        if (B.bc < 0) continue;
+        uint8_t bit = B.face <= 3 ? 1 : 2;
+        if (B.face == 1 || B.face == 4) { for (int k = B.kb; k <= B.ke - 1; k++) for (int j = B.jb; j <= B.je - 1; j++) fi[JK(b, j, k)] |= bit; }
+        else if (B.face == 2 || B.face == 5) { for (int k = B.kb; k <= B.ke - 1; k++) for (int i = B.ib; i <= B.ie - 1; i++) fj[IK(b, i, k)] |= bit; }
+        else { for (int j = B.jb; j <= B.je - 1; j++) for (int i = B.ib; i <= B.ie - 1; i++) fk[IJ(b, i, j)] |= bit; }
+    }
+    HIPCHK(hipMemcpy(b.bcf[0], fi.data(), fi.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.bcf[1], fj.data(), fj.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.bcf[2], fk.data(), fk.size(), hipMemcpyHostToDevice));
+    return OCFD_OK;
+}
+
+int check_subs(ocfd_ctx* c, const HostBlock& hb) {
+    for (const Sub& B : hb.subs) {
+        if (B.face < 1 || B.face > 6) return fail(OCFD_EARG, "subface with face id outside 1..6");
+        if (B.ib < 1 || B.ie > hb.nx || B.jb < 1 || B.je > hb.ny || B.kb < 1 || B.ke > hb.nz || B.ib > B.ie || B.jb > B.je || B.kb > B.ke)
+            return fail(OCFD_EARG, "subface range outside block " + std::to_string(hb.no));
+        if (B.bc < 0) {
+            if (B.nb1 < 1 || B.nb1 > (int)c->blocks.size()) return fail(OCFD_EARG, "inner subface links to unknown block");
+            if (B.face1 < 1 || B.face1 > 6) return fail(OCFD_EARG, "inner subface with linked face id outside 1..6");
+            int La[3] = {std::abs(B.L1), std::abs(B.L2), std::abs(B.L3)};
+            for (int q = 0; q < 3; q++) if (La[q] < 1 || La[q] > 3) return fail(OCFD_EARG, "inner subface with L outside +-1..3");
+        } else if (!(B.bc == 2 || B.bc == 3 || B.bc == 4 || B.bc == 5 || B.bc == 6 || B.bc == 401)) {
+            return fail(OCFD_EUNSUPPORTED, "The boundary condition is not supported!!! Block_no is " + std::to_string(hb.no) +
+                                               " bc=" + std::to_string(B.bc));
+        }
+    }
+    return OCFD_OK;
+}
+
+void read_subs(HostBlock& hb, int nsub, const int32_t* a) {
+    hb.subs.resize(nsub);
+    for (int q = 0; q < nsub; q++) {
+        const int32_t* v = a + 21 * q;
+        Sub& B = hb.subs[q];
+        B.ib = v[0]; B.ie = v[1]; B.jb = v[2]; B.je = v[3]; B.kb = v[4]; B.ke = v[5]; B.bc = v[6]; B.face = v[7]; B.f_no = v[8];
+        B.ib1 = v[9]; B.ie1 = v[10]; B.jb1 = v[11]; B.je1 = v[12]; B.kb1 = v[13]; B.ke1 = v[14]; B.nb1 = v[15]; B.face1 = v[16];
+        B.f_no1 = v[17]; B.L1 = v[18]; B.L2 = v[19]; B.L3 = v[20];
+    }
+}
+
+// ---- iteration -------------------------------------------------------------------------------
+// The iteration is enqueued stage by stage over all blocks; blocks are independent inside the
+// residual and update (the halo was filled at the end of the previous iteration), so the
+// stage-major order gives the results of the reference's block-major loop.
+void enqueue_stage(ocfd_ctx* c, int stage) {
+    switch (stage) {
+    case OCFD_STAGE_PRE:      // Set_Un is implicit (U is not modified before the update)
+        for (HostBlock& hb : c->blocks) {
+            const Blk& b = hb.d;
+            launch_reset_flags(b, c->st);
+            launch_prims(b, c->p, c->st);
+            if (c->p.visc == 1) {
+                launch_viscosity(b, c->p, c->st);
+                if (c->p.turb == 2) launch_sa(b, c->p, hb.subs.data(), (int)hb.subs.size(), c->st);
+                else if (c->p.turb == 3) launch_sst(b, c->p, hb.subs.data(), (int)hb.subs.size(), c->st);
+                else launch_laminar_amut(b, hb.subs.data(), (int)hb.subs.size(), c->st);
+            }
+        }
+        break;
+    case OCFD_STAGE_INVISCID:
+        for (HostBlock& hb : c->blocks) launch_inv_fluxes(hb.d, c->p, hb.subs.data(), (int)hb.subs.size(), c->st);
+        break;
+    case OCFD_STAGE_VISCOUS_DT:
+        for (HostBlock& hb : c->blocks) launch_visc_assemble_dt(hb.d, c->p, c->st);
+        break;
+    case OCFD_STAGE_LU_FORWARD:
+        for (HostBlock& hb : c->blocks) launch_lusgs_sweep(hb.d, c->p, true, c->st);
+        break;
+    case OCFD_STAGE_LU_BACKWARD:
+        for (HostBlock& hb : c->blocks) launch_lusgs_sweep(hb.d, c->p, false, c->st);
+        break;
+    case OCFD_STAGE_UPDATE:
+        for (HostBlock& hb : c->blocks) launch_update_limit(hb.d, c->p, c->st);
+        break;
+    case OCFD_STAGE_BC_HALO:
+        for (HostBlock& hb : c->blocks)
+            for (const Sub& B : hb.subs)
+                if (B.bc > 0) launch_bc(hb.d, c->p, B, c->st);
+        for (HostBlock& hb : c->blocks)
+            for (const Sub& B : hb.subs)
+                if (B.bc < 0) launch_halo(hb.d, c->blocks[B.nb1 - 1].d, B, c->p.NV, c->st);
+        for (HostBlock& hb : c->blocks) launch_corners(hb.d, c->p.NV, c->st);
+        for (HostBlock& hb : c->blocks) launch_force_vt_kw(hb.d, c->p, c->st);
+        break;
+    }
+}
+
+void enqueue_residual(ocfd_ctx* c) {
+    for (int s = OCFD_STAGE_PRE; s <= OCFD_STAGE_LU_BACKWARD; s++) enqueue_stage(c, s);
+}
+
+void enqueue_bc(ocfd_ctx* c) {
+    for (HostBlock& hb : c->blocks)
+        for (const Sub& B : hb.subs)
+            if (B.bc > 0) launch_bc(hb.d, c->p, B, c->st);
+}
+
+void enqueue_halo(ocfd_ctx* c) {
+    for (HostBlock& hb : c->blocks)
+        for (const Sub& B : hb.subs)
+            if (B.bc < 0) launch_halo(hb.d, c->blocks[B.nb1 - 1].d, B, c->p.NV, c->st);
+    for (HostBlock& hb : c->blocks) launch_corners(hb.d, c->p.NV, c->st);
+}
+
+void enqueue_force_vt_kw(ocfd_ctx* c) {
+    for (HostBlock& hb : c->blocks) launch_force_vt_kw(hb.d, c->p, c->st);
+}
+
+void drop_graphs(ocfd_ctx* c) {
+    for (int s = 0; s < OCFD_NSTAGES; s++) {
+        if (c->exec[s]) { (void)hipGraphExecDestroy(c->exec[s]); c->exec[s] = nullptr; }
+        if (c->graph[s]) { (void)hipGraphDestroy(c->graph[s]); c->graph[s] = nullptr; }
+    }
+}
+
+// One stage of the iteration: eager launches, or the stage's graph (captured on first use).
+int run_stage(ocfd_ctx* c, int stage) {
+    if (!c->use_graph) { enqueue_stage(c, stage); return OCFD_OK; }
+    if (!c->exec[stage]) {
+        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        enqueue_stage(c, stage);
+        HIPCHK(hipStreamEndCapture(c->st, &c->graph[stage]));
+        HIPCHK(hipGraphInstantiate(&c->exec[stage], c->graph[stage], nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(c->exec[stage], c->st));
+    return OCFD_OK;
+}
+
+int ready(ocfd_ctx* c) {
+    if (!c) return fail(OCFD_EARG, "null context");
+    if (c->blocks.empty()) return fail(OCFD_EARG, "no blocks added");
+    for (size_t q = 0; q < c->blocks.size(); q++)
+        if (c->blocks[q].no != (int)q + 1) return fail(OCFD_EARG, "block " + std::to_string(q + 1) + " missing");
+    return OCFD_OK;
+}
+
+int finish(ocfd_ctx* c) {
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    return OCFD_OK;
+}
+
+}  // namespace
+
+// ================================ C ABI ========================================================
+extern "C" {
+
+const char* ocfd_last_error(void) { return g_err.c_str(); }
+
+int ocfd_create(const ocfd_params* p, ocfd_ctx** out) {
+    if (!p || !out) return fail(OCFD_EARG, "null argument");
+    if (p->NVAR < 5 || p->NVAR > 7) return fail(OCFD_EARG, "NVAR must be 5, 6 or 7");
+    if (p->If_viscous == 1 && !(p->Iflag_turbulence_model == 0 || p->Iflag_turbulence_model == 2 || p->Iflag_turbulence_model == 3))
+        return fail(OCFD_EUNSUPPORTED, "turbulence model outside SA (2) and SST (3)");
+    if (p->Iflag_Scheme == 9 || p->Bound_Scheme == 9) return fail(OCFD_EUNSUPPORTED, "WENO7 reconstruction is not on this path");
+    if (p->Iflag_Flux < 1 || p->Iflag_Flux > 6) return fail(OCFD_EARG, "Iflag_Flux outside 1..6");
+    if (p->precision != 0) return fail(OCFD_EUNSUPPORTED, "precision mode other than all-FP64 is selected through ocfd_set_precision");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) return fail(OCFD_EHIP, "no HIP device");
+    ocfd_ctx* c = new ocfd_ctx();
+    c->hp = *p;
+    c->p = to_par(*p);
+    HIPCHK(hipGetDevice(&c->device));
+    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    *out = c;
+    return OCFD_OK;
+}
+
+int ocfd_destroy(ocfd_ctx* c) {
+    if (!c) return OCFD_OK;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    drop_graphs(c);
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    for (HostBlock& hb : c->blocks) for (void* q : hb.allocs) (void)hipFree(q);
+    if (c->norm_part) (void)hipFree(c->norm_part);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return OCFD_OK;
+}
+
+static int add_block_common(ocfd_ctx* c, int32_t no, int32_t nx, int32_t ny, int32_t nz, int32_t nsub, const int32_t* bc21, HostBlock** out) {
+    if (!c || !bc21) return fail(OCFD_EARG, "null argument");
+    if (no < 1 || nx < 3 || ny < 3 || nz < 3 || nsub < 1) return fail(OCFD_EARG, "block number or size out of range (need nx,ny,nz >= 3)");
+    if ((int)c->blocks.size() < no) c->blocks.resize(no);
+    HostBlock& hb = c->blocks[no - 1];
+    if (hb.no != 0) return fail(OCFD_EARG, "block " + std::to_string(no) + " added twice");
+    hb.no = no; hb.nx = nx; hb.ny = ny; hb.nz = nz;
+    read_subs(hb, nsub, bc21);
+    int r = alloc_block(c, hb);
+    if (r) return r;
+    drop_graphs(c);
+    *out = &hb;
+    return OCFD_OK;
+}
+
+int ocfd_add_block(ocfd_ctx* c, int32_t no, int32_t nx, int32_t ny, int32_t nz, int32_t nsub, const int32_t* bc21,
+                   const ocfd_block_geom* g) {
+    if (!g) return fail(OCFD_EARG, "null geometry");
+    HostBlock* hb;
+    int r = add_block_common(c, no, nx, ny, nz, nsub, bc21, &hb);
+    if (r) return r;
+    Blk& b = hb->d;
+    const int L = LAP;
+    int n0[3] = {0, 0, 0}, n1[3] = {nx + 1, ny + 1, nz + 1};
+    int c0[3] = {1 - L, 1 - L, 1 - L}, c1[3] = {nx + L - 1, ny + L - 1, nz + L - 1};
+    int v0[3] = {1, 1, 1}, v1[3] = {nx - 1, ny - 1, nz - 1};
+    int f1[3] = {nx, ny, nz};
+    if (!g->x || !g->y || !g->z || !g->xc || !g->yc || !g->zc || !g->vol || !g->dtime_mesh || !g->Si || !g->Sj || !g->Sk)
+        return fail(OCFD_EARG, "geometry pointer missing");
+    if ((r = put_box(*hb, b.x, g->x, n0, n1)) || (r = put_box(*hb, b.y, g->y, n0, n1)) || (r = put_box(*hb, b.z, g->z, n0, n1))) return r;
+    if ((r = put_box(*hb, b.xc, g->xc, c0, c1)) || (r = put_box(*hb, b.yc, g->yc, c0, c1)) || (r = put_box(*hb, b.zc, g->zc, c0, c1))) return r;
+    if ((r = put_box(*hb, b.vol, g->vol, v0, v1)) || (r = put_box(*hb, b.dtm, g->dtime_mesh, v0, v1))) return r;
+    if ((r = put_box(*hb, b.S[0], g->Si, v0, f1)) || (r = put_box(*hb, b.S[1], g->Sj, v0, f1)) || (r = put_box(*hb, b.S[2], g->Sk, v0, f1))) return r;
+    for (int q = 0; q < 9; q++) {
+        if (!g->n[q]) return fail(OCFD_EARG, "normal pointer missing");
+        if ((r = put_box(*hb, b.nv[q], g->n[q], v0, f1))) return r;
+    }
+    for (int s = 1; s < 4; s++) for (int q = 0; q < 9; q++) {
+        if (!g->met[s][q]) return fail(OCFD_EARG, "metric pointer missing");
+        if ((r = put_box(*hb, b.met[s][q], g->met[s][q], v0, f1))) return r;
+    }
+    if (c->p.turb != 0) {
+        if (!g->dw) return fail(OCFD_EARG, "wall distance required by the turbulence model");
+        if ((r = put_box(*hb, b.dw, g->dw, v0, v1))) return r;
+        for (int q = 0; q < 9; q++) {
+            if (!g->met[0][q]) return fail(OCFD_EARG, "metric set 0 required by the turbulence model");
+            if ((r = put_box(*hb, b.met[0][q], g->met[0][q], v0, f1))) return r;
+        }
+    }
+    return upload_bcflags(*hb);
+}
+
+int ocfd_add_block_nodes(ocfd_ctx* c, int32_t no, int32_t nx, int32_t ny, int32_t nz, int32_t nsub, const int32_t* bc21,
+                         const double* x, const double* y, const double* z) {
+    if (!x || !y || !z) return fail(OCFD_EARG, "null coordinates");
+    HostBlock* hb;
+    int r = add_block_common(c, no, nx, ny, nz, nsub, bc21, &hb);
+    if (r) return r;
+    size_t nn = (size_t)nx * ny * nz;
+    const double* src[3] = {x, y, z};
+    for (int q = 0; q < 3; q++) hb->nodes[q].assign(src[q], src[q] + nn);
+    return upload_bcflags(*hb);
+}
+
+// ghost node layer: Coordinate_send_mpi, Coordinate_Periodic, coordinate_boundary_and_corner and
+// get_xyz_conner (sub_update_buffer_mpi.f90:250-512) on host copies of the node boxes
+static void coordinate_ghosts(ocfd_ctx* c, std::vector<std::vector<double>> (&X)[3], double pd[3]) {
+    for (size_t m = 0; m < c->blocks.size(); m++) {
+        HostBlock& hb = c->blocks[m];
+        for (const Sub& B : hb.subs) {
+            if (B.bc >= 0) continue;
+            int kb[3] = {B.ib, B.jb, B.kb}, ke[3] = {B.ie, B.je, B.ke};
+            int kd = (B.face - 1) % 3;
+            if (B.face <= 3) kb[kd] += 1; else kb[kd] -= 1;
+            ke[kd] = kb[kd];
+            int kb1[3] = {B.ib1, B.jb1, B.kb1}, ke1[3] = {B.ie1, B.je1, B.ke1};
+            int kd1 = (B.face1 - 1) % 3;
+            if (B.face1 <= 3) kb1[kd1] -= 1; else kb1[kd1] += 1;
+            ke1[kd1] = kb1[kd1];
+            int Ls[3] = {B.L1, B.L2, B.L3}, L[3], P[3], ks[3];
+            for (int q = 0; q < 3; q++) { L[q] = std::abs(Ls[q]); P[q] = Ls[q] >= 0 ? 1 : -1; ks[q] = P[q] > 0 ? kb[q] : ke[q]; }
+            HostBlock& t = c->blocks[B.nb1 - 1];
+            for (int k = kb1[2]; k <= ke1[2]; k++) for (int j = kb1[1]; j <= ke1[1]; j++) for (int i = kb1[0]; i <= ke1[0]; i++) {
+                int ka[3] = {i - kb1[0], j - kb1[1], k - kb1[2]};
+                int i1 = ks[0] + ka[L[0] - 1] * P[0], j1 = ks[1] + ka[L[1] - 1] * P[1], k1 = ks[2] + ka[L[2] - 1] * P[2];
+                for (int a = 0; a < 3; a++) X[a][B.nb1 - 1][IX(t.d, i, j, k)] = X[a][m][IX(hb.d, i1, j1, k1)];
+            }
+        }
+    }
+    for (size_t m = 0; m < c->blocks.size(); m++) {
+        HostBlock& hb = c->blocks[m];
+        for (const Sub& B : hb.subs) {
+            if (B.bc != -2 && B.bc != -3) continue;
+            double sgn = B.bc == -2 ? -1.0 : 1.0;
+            int kb[3] = {B.ib, B.jb, B.kb}, ke[3] = {B.ie, B.je, B.ke};
+            int kd = (B.face - 1) % 3;
+            if (B.face <= 3) kb[kd] -= 1; else kb[kd] += 1;
+            ke[kd] = kb[kd];
+            for (int k = kb[2]; k <= ke[2]; k++) for (int j = kb[1]; j <= ke[1]; j++) for (int i = kb[0]; i <= ke[0]; i++)
+                for (int a = 0; a < 3; a++) { double& v = X[a][m][IX(hb.d, i, j, k)]; v = v + (sgn < 0 ? -pd[a] : pd[a]); }
+        }
+    }
+    for (size_t m = 0; m < c->blocks.size(); m++) {
+        HostBlock& hb = c->blocks[m];
+        const Blk& b = hb.d;
+        for (const Sub& B : hb.subs) {
+            if (B.bc < 0) continue;
+            for (int a = 0; a < 3; a++) {
+                std::vector<double>& x = X[a][m];
+                for (int k = B.kb; k <= B.ke; k++) for (int j = B.jb; j <= B.je; j++) for (int i = B.ib; i <= B.ie; i++) {
+                    switch (B.face) {
+                    case 1: if (i == B.ib) x[IX(b, i - 1, j, k)] = 2.0 * x[IX(b, i, j, k)] - x[IX(b, i + 1, j, k)]; break;
+                    case 2: if (j == B.jb) x[IX(b, i, j - 1, k)] = 2.0 * x[IX(b, i, j, k)] - x[IX(b, i, j + 1, k)]; break;
+                    case 3: if (k == B.kb) x[IX(b, i, j, k - 1)] = 2.0 * x[IX(b, i, j, k)] - x[IX(b, i, j, k + 1)]; break;
+                    case 4: if (i == B.ie) x[IX(b, i + 1, j, k)] = 2.0 * x[IX(b, i, j, k)] - x[IX(b, i - 1, j, k)]; break;
+                    case 5: if (j == B.je) x[IX(b, i, j + 1, k)] = 2.0 * x[IX(b, i, j, k)] - x[IX(b, i, j - 1, k)]; break;
+                    default: if (k == B.ke) x[IX(b, i, j, k + 1)] = 2.0 * x[IX(b, i, j, k)] - x[IX(b, i, j, k - 1)]; break;
+                    }
+                }
+            }
+        }
+        int nx = b.nx, ny = b.ny, nz = b.nz;
+        for (int a = 0; a < 3; a++) {
+            std::vector<double>& x = X[a][m];
+            auto V = [&](int i, int j, int k) -> double& { return x[IX(b, i, j, k)]; };
+            for (int k = 1; k <= nz; k++) {
+                V(0, 0, k) = (V(1, 0, k) + V(0, 1, k)) * 0.5; V(nx + 1, 0, k) = (V(nx, 0, k) + V(nx + 1, 1, k)) * 0.5;
+                V(0, ny + 1, k) = (V(1, ny + 1, k) + V(0, ny, k)) * 0.5; V(nx + 1, ny + 1, k) = (V(nx, ny + 1, k) + V(nx + 1, ny, k)) * 0.5;
+            }
+            for (int j = 1; j <= ny; j++) {
+                V(0, j, 0) = (V(1, j, 0) + V(0, j, 1)) * 0.5; V(nx + 1, j, 0) = (V(nx, j, 0) + V(nx + 1, j, 1)) * 0.5;
+                V(0, j, nz + 1) = (V(1, j, nz + 1) + V(0, j, nz)) * 0.5; V(nx + 1, j, nz + 1) = (V(nx, j, nz + 1) + V(nx + 1, j, nz)) * 0.5;
+            }
+            for (int i = 1; i <= nx; i++) {
+                V(i, 0, 0) = (V(i, 1, 0) + V(i, 0, 1)) * 0.5; V(i, ny + 1, 0) = (V(i, ny, 0) + V(i, ny + 1, 1)) * 0.5;
+                V(i, 0, nz + 1) = (V(i, 1, nz + 1) + V(i, 0, nz)) * 0.5; V(i, ny + 1, nz + 1) = (V(i, ny, nz + 1) + V(i, ny + 1, nz)) * 0.5;
+            }
+            for (int cidx = 0; cidx < 8; cidx++) {
+                int I = (cidx & 1) ? nx + 1 : 0, J = (cidx & 2) ? ny + 1 : 0, K = (cidx & 4) ? nz + 1 : 0;
+                int Ii = (cidx & 1) ? nx : 1, Jj = (cidx & 2) ? ny : 1, Kk = (cidx & 4) ? nz : 1;
+                V(I, J, K) = (2.0 * ((V(Ii, J, K) + V(I, Jj, K)) + V(I, J, Kk)) - ((V(Ii, Jj, K) + V(Ii, J, Kk)) + V(I, Jj, Kk))) / 3.0;
+            }
+        }
+    }
+}
+
+int ocfd_setup_geometry(ocfd_ctx* c) {
+    int r = ready(c);
+    if (r) return r;
+    for (HostBlock& hb : c->blocks) {
+        if (hb.nodes[0].empty()) return fail(OCFD_EARG, "ocfd_setup_geometry needs every block added with ocfd_add_block_nodes");
+        if ((r = check_subs(c, hb))) return r;
+    }
+    std::vector<std::vector<double>> X[3];
+    for (int a = 0; a < 3; a++) {
+        X[a].resize(c->blocks.size());
+        for (size_t m = 0; m < c->blocks.size(); m++) {
+            HostBlock& hb = c->blocks[m];
+            X[a][m].assign(hb.n(), 0.0);
+            for (int k = 1; k <= hb.nz; k++) for (int j = 1; j <= hb.ny; j++) for (int i = 1; i <= hb.nx; i++)
+                X[a][m][IX(hb.d, i, j, k)] = hb.nodes[a][(size_t)(i - 1) + (size_t)hb.nx * ((size_t)(j - 1) + (size_t)hb.ny * (k - 1))];
+        }
+    }
+    double pd[3] = {c->hp.Periodic_d[0], c->hp.Periodic_d[1], c->hp.Periodic_d[2]};
+    coordinate_ghosts(c, X, pd);
+    for (size_t m = 0; m < c->blocks.size(); m++) {
+        HostBlock& hb = c->blocks[m];
+        HIPCHK(hipMemcpy(hb.d.x, X[0][m].data(), hb.n() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(hb.d.y, X[1][m].data(), hb.n() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(hb.d.z, X[2][m].data(), hb.n() * sizeof(double), hipMemcpyHostToDevice));
+        launch_geometry(hb.d, hb.subs.data(), (int)hb.subs.size(), c->st);
+    }
+    // update_Mesh_Center: cell-centre ghosts of inner faces, then the periodic shift
+    // (sub_udate_Meshlink_mpi.f90:12-249), through the U-halo kernel on (xc, yc, zc)
+    for (HostBlock& hb : c->blocks)
+        for (const Sub& B : hb.subs) {
+            if (B.bc >= 0) continue;
+            Blk s = hb.d, t = c->blocks[B.nb1 - 1].d;
+            s.U[0] = hb.d.xc; s.U[1] = hb.d.yc; s.U[2] = hb.d.zc;
+            t.U[0] = t.xc; t.U[1] = t.yc; t.U[2] = t.zc;
+            launch_halo(s, t, B, 3, c->st);
+        }
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (HostBlock& hb : c->blocks) {
+        bool any = false;
+        for (const Sub& B : hb.subs) any = any || B.bc == -2 || B.bc == -3;
+        if (!any) continue;
+        std::vector<double> C[3];
+        double* dev[3] = {hb.d.xc, hb.d.yc, hb.d.zc};
+        for (int a = 0; a < 3; a++) if ((r = get_box(hb, dev[a], C[a]))) return r;
+        for (const Sub& B : hb.subs) {
+            if (B.bc != -2 && B.bc != -3) continue;
+            int kb[3] = {B.ib, B.jb, B.kb}, ke[3] = {B.ie - 1, B.je - 1, B.ke - 1};
+            int kd = (B.face - 1) % 3;
+            if (B.face <= 3) kb[kd] -= LAP;
+            ke[kd] = kb[kd] + LAP - 1;
+            for (int k = kb[2]; k <= ke[2]; k++) for (int j = kb[1]; j <= ke[1]; j++) for (int i = kb[0]; i <= ke[0]; i++)
+                for (int a = 0; a < 3; a++) { double& v = C[a][IX(hb.d, i, j, k)]; v = v + (B.bc == -2 ? -pd[a] : pd[a]); }
+        }
+        for (int a = 0; a < 3; a++) HIPCHK(hipMemcpy(dev[a], C[a].data(), hb.n() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    // wall distance (sub_comput_dw.f90:43-168): wall nodes of every block, in block/subface order
+    if (c->p.turb != 0) {
+        std::vector<double> W;
+        for (size_t m = 0; m < c->blocks.size(); m++) {
+            HostBlock& hb = c->blocks[m];
+            for (const Sub& B : hb.subs) {
+                if (B.bc != 2) continue;
+                for (int k = B.kb; k <= B.ke; k++) for (int j = B.jb; j <= B.je; j++) for (int i = B.ib; i <= B.ie; i++)
+                    for (int a = 0; a < 3; a++) W.push_back(X[a][m][IX(hb.d, i, j, k)]);
+            }
+        }
+        double* dW = nullptr;
+        HIPCHK(hipMalloc(&dW, std::max<size_t>(W.size(), 3) * sizeof(double)));
+        if (!W.empty()) HIPCHK(hipMemcpy(dW, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice));
+        for (HostBlock& hb : c->blocks) launch_wall_distance(hb.d, dW, (long)(W.size() / 3), c->st);
+        HIPCHK(hipStreamSynchronize(c->st));
+        HIPCHK(hipFree(dW));
+    }
+    for (HostBlock& hb : c->blocks) launch_mesh_quality(hb.d, c->st);
+    for (HostBlock& hb : c->blocks) for (int q = 0; q < 3; q++) hb.nodes[q].clear();
+    return finish(c);
+}
+
+int ocfd_upload_U(ocfd_ctx* c, int32_t no, const double* U) {
+    HostBlock* hb = find_block(c, no);
+    if (!hb || !U) return fail(OCFD_EARG, "unknown block or null U");
+    int NV = c->p.NV;
+    const Blk& b = hb->d;
+    std::vector<double> h(hb->n());
+    long ni = b.NI, nj = b.NJ;
+    for (int m = 0; m < NV; m++) {
+        for (long s = 0; s < (long)hb->n(); s++) h[s] = U[m + (long)NV * s];   // U(m,i,j,k) over the same LAP box
+        HIPCHK(hipMemcpy(b.U[m], h.data(), hb->n() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    (void)ni; (void)nj;
+    return OCFD_OK;
+}
+
+int ocfd_download_U(ocfd_ctx* c, int32_t no, double* U) {
+    HostBlock* hb = find_block(c, no);
+    if (!hb || !U) return fail(OCFD_EARG, "unknown block or null U");
+    HIPCHK(hipStreamSynchronize(c->st));
+    int NV = c->p.NV, r;
+    std::vector<double> h;
+    for (int m = 0; m < NV; m++) {
+        if ((r = get_box(*hb, hb->d.U[m], h))) return r;
+        for (long s = 0; s < (long)hb->n(); s++) U[m + (long)NV * s] = h[s];
+    }
+    return OCFD_OK;
+}
+
+int ocfd_download_field(ocfd_ctx* c, int32_t no, int32_t field, double* out) {
+    HostBlock* hb = find_block(c, no);
+    if (!hb || !out) return fail(OCFD_EARG, "unknown block or null output");
+    HIPCHK(hipStreamSynchronize(c->st));
+    const Blk& b = hb->d;
+    int nv = (field == OCFD_F_RES || field == OCFD_F_DU) ? c->p.NV : 1;
+    long n0 = b.nx + 3, n1 = b.ny + 3, n2 = b.nz + 3;
+    std::vector<double> h;
+    int r;
+    for (int m = 0; m < nv; m++) {
+        const double* src;
+        switch (field) {
+        case OCFD_F_RES: src = b.Res[m]; break;
+        case OCFD_F_DU: src = b.DU[m]; break;
+        case OCFD_F_DT: src = b.dt; break;
+        case OCFD_F_MU: src = b.muf; break;
+        case OCFD_F_MUT: src = b.mut; break;
+        default: return fail(OCFD_EARG, "unknown field id");
+        }
+        if ((r = get_box(*hb, src, h))) return r;
+        for (int k = -1; k <= b.nz + 1; k++) for (int j = -1; j <= b.ny + 1; j++) for (int i = -1; i <= b.nx + 1; i++)
+            out[m + nv * ((i + 1) + n0 * ((j + 1) + n1 * (long)(k + 1)))] = h[IX(b, i, j, k)];
+    }
+    (void)n2;
+    return OCFD_OK;
+}
+
+int ocfd_init_flow(ocfd_ctx* c) {
+    int r = ready(c);
+    if (r) return r;
+    double g = c->p.gamma, Ma = c->p.Ma;
+    double d0 = 1.0, p0 = 1.0 / (g * Ma * Ma);
+    double u0 = std::cos(c->p.A_alfa) * std::cos(c->p.A_beta), v0 = std::sin(c->p.A_alfa) * std::cos(c->p.A_beta), w0 = std::sin(c->p.A_beta);
+    double q[7] = {d0, d0 * u0, d0 * v0, d0 * w0, p0 / (g - 1.0) + (0.5 * d0) * ((u0 * u0 + v0 * v0) + w0 * w0), 0.0, 0.0};
+    if (c->p.NV == 6) q[5] = 5.0;
+    if (c->p.NV == 7) { q[5] = 10.0 * c->p.Kt_inf; q[6] = c->p.Wt_inf; }
+    for (HostBlock& hb : c->blocks) {
+        if ((r = check_subs(c, hb))) return r;
+        std::vector<double> h(hb.n());
+        for (int m = 0; m < c->p.NV; m++) {
+            std::fill(h.begin(), h.end(), q[m]);
+            HIPCHK(hipMemcpy(hb.d.U[m], h.data(), hb.n() * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    enqueue_bc(c);
+    enqueue_halo(c);
+    return finish(c);
+}
+
+int ocfd_step_lusgs(ocfd_ctx* c, int32_t niter) {
+    int r = ready(c);
+    if (r) return r;
+    for (HostBlock& hb : c->blocks) if ((r = check_subs(c, hb))) return r;
+    for (int it = 0; it < niter; it++) {
+        for (int s = 0; s < OCFD_NSTAGES; s++) if ((r = run_stage(c, s))) return r;
+        c->tt = c->tt + c->p.dt_global;
+        c->kstep = c->kstep + 1;
+    }
+    return finish(c);
+}
+
+int ocfd_step_timed(ocfd_ctx* c, int32_t niter, double* stage_ms) {
+    int r = ready(c);
+    if (r) return r;
+    if (!stage_ms || niter < 1) return fail(OCFD_EARG, "null stage_ms or niter < 1");
+    for (HostBlock& hb : c->blocks) if ((r = check_subs(c, hb))) return r;
+    size_t need = (size_t)niter * (OCFD_NSTAGES + 1);
+    while (c->events.size() < need) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->events.push_back(e);
+    }
+    for (int it = 0; it < niter; it++) {
+        hipEvent_t* ev = c->events.data() + (size_t)it * (OCFD_NSTAGES + 1);
+        HIPCHK(hipEventRecord(ev[0], c->st));
+        for (int s = 0; s < OCFD_NSTAGES; s++) {
+            if ((r = run_stage(c, s))) return r;
+            HIPCHK(hipEventRecord(ev[s + 1], c->st));
+        }
+        c->tt = c->tt + c->p.dt_global;
+        c->kstep = c->kstep + 1;
+    }
+    if ((r = finish(c))) return r;
+    for (int s = 0; s < OCFD_NSTAGES; s++) stage_ms[s] = 0.0;
+    for (int it = 0; it < niter; it++) {
+        hipEvent_t* ev = c->events.data() + (size_t)it * (OCFD_NSTAGES + 1);
+        for (int s = 0; s < OCFD_NSTAGES; s++) {
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
+            stage_ms[s] += (double)ms;
+        }
+    }
+    return OCFD_OK;
+}
+
+int ocfd_residual(ocfd_ctx* c) { int r = ready(c); if (r) return r; enqueue_residual(c); return finish(c); }
+int ocfd_update_limit(ocfd_ctx* c) { int r = ready(c); if (r) return r; enqueue_stage(c, OCFD_STAGE_UPDATE); return finish(c); }
+int ocfd_bc(ocfd_ctx* c) {
+    int r = ready(c); if (r) return r;
+    for (HostBlock& hb : c->blocks) if ((r = check_subs(c, hb))) return r;
+    enqueue_bc(c); return finish(c);
+}
+int ocfd_halo(ocfd_ctx* c) { int r = ready(c); if (r) return r; enqueue_halo(c); return finish(c); }
+int ocfd_force_vt_kw(ocfd_ctx* c) { int r = ready(c); if (r) return r; enqueue_force_vt_kw(c); return finish(c); }
+
+int ocfd_res_norms(ocfd_ctx* c, double* res_max, double* res_rms) {
+    int r = ready(c);
+    if (r) return r;
+    if (!res_max || !res_rms) return fail(OCFD_EARG, "null output");
+    int NV = c->p.NV, P = norms_parts();
+    size_t need = (size_t)c->blocks.size() * NV * P * 3;
+    if (c->norm_part_n < need) {
+        if (c->norm_part) HIPCHK(hipFree(c->norm_part));
+        HIPCHK(hipMalloc(&c->norm_part, need * sizeof(double)));
+        c->norm_part_n = need;
+    }
+    for (size_t m = 0; m < c->blocks.size(); m++) launch_norms(c->blocks[m].d, NV, c->norm_part + m * NV * P * 3, c->st);
+    std::vector<double> h(need);
+    HIPCHK(hipMemcpyAsync(h.data(), c->norm_part, need * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double mx[MAXV] = {0}, sm[MAXV] = {0};
+    bool nan = false;
+    long ncell = 0;
+    for (size_t m = 0; m < c->blocks.size(); m++) {
+        const HostBlock& hb = c->blocks[m];
+        ncell += (long)(hb.nx - 1) * (hb.ny - 1) * (hb.nz - 1);
+        for (int v = 0; v < NV; v++) for (int q = 0; q < P; q++) {
+            const double* o = h.data() + 3 * ((m * NV + v) * P + q);
+            mx[v] = std::max(mx[v], o[0]); sm[v] = sm[v] + o[1]; nan = nan || o[2] != 0.0;
+        }
+    }
+    for (int v = 0; v < NV; v++) { res_max[v] = mx[v]; res_rms[v] = std::sqrt(sm[v] / (double)ncell); }
+    if (nan) return fail(OCFD_ENAN, "NaN in Residual is found !");
+    return OCFD_OK;
+}
+
+int ocfd_forces(ocfd_ctx* c, double* out) {
+    int r = ready(c);
+    if (r) return r;
+    if (!out) return fail(OCFD_EARG, "null output");
+    std::vector<double> tot(9, 0.0);
+    for (HostBlock& hb : c->blocks) {
+        double init[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, c->hp.Centroid[0], c->hp.Centroid[1], c->hp.Centroid[2]};
+        HIPCHK(hipMemcpyAsync(hb.force_acc, init, sizeof(init), hipMemcpyHostToDevice, c->st));
+        for (const Sub& B : hb.subs) if (B.bc == 2) launch_force_sub(hb.d, c->p, B, hb.force_acc, c->st);
+        double acc[12];
+        HIPCHK(hipMemcpyAsync(acc, hb.force_acc, sizeof(acc), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (int q = 0; q < 9; q++) tot[q] = tot[q] + acc[q];
+    }
+    for (int q = 0; q < 6; q++) tot[q] = 2.0 * tot[q] / c->hp.Ref_S;
+    for (int q = 6; q < 9; q++) tot[q] = 2.0 * tot[q] / (c->hp.Ref_S * c->hp.Ref_L);
+    double Fx = tot[0] + tot[3], Fy = tot[1] + tot[4], Fz = tot[2] + tot[5], AoA = c->hp.AoA;
+    for (int q = 0; q < 9; q++) out[q] = tot[q];
+    if (c->hp.Cood_Y_UP == 1) { out[9] = Fy * std::cos(AoA) - Fx * std::sin(AoA); out[10] = Fx * std::cos(AoA) + Fy * std::sin(AoA); out[11] = Fz; }
+    else { out[9] = Fz * std::cos(AoA) - Fx * std::sin(AoA); out[10] = Fx * std::cos(AoA) + Fz * std::sin(AoA); out[11] = Fy; }
+    return OCFD_OK;
+}
+
+int ocfd_block_status(ocfd_ctx* c, int32_t no, int32_t* overlimit, int32_t* nlimited) {
+    HostBlock* hb = find_block(c, no);
+    if (!hb || !overlimit || !nlimited) return fail(OCFD_EARG, "unknown block or null output");
+    int f[16];
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(f, hb->d.flags, sizeof(f), hipMemcpyDeviceToHost));
+    *overlimit = f[0]; *nlimited = f[1];
+    return OCFD_OK;
+}
+
+int ocfd_get_counters(ocfd_ctx* c, int32_t* kstep, double* tt) {
+    if (!c || !kstep || !tt) return fail(OCFD_EARG, "null argument");
+    *kstep = c->kstep; *tt = c->tt;
+    return OCFD_OK;
+}
+
+int ocfd_set_graph(ocfd_ctx* c, int32_t enable) {
+    if (!c) return fail(OCFD_EARG, "null context");
+    c->use_graph = enable != 0;
+    return OCFD_OK;
+}
+
+int ocfd_sync(ocfd_ctx* c) {
+    if (!c) return fail(OCFD_EARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->st));
+    return OCFD_OK;
+}
+
+}  // extern "C"

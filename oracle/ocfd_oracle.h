@@ -35,6 +35,11 @@ int orc_init_flow(orc_ctx *c);
 int orc_set_interior_U(orc_ctx *c, int blk, const double *U);
 int orc_bc_halo(orc_ctx *c);
 int orc_step(orc_ctx *c);
+int orc_residual(orc_ctx *c);
+int orc_update(orc_ctx *c);
+int orc_bc(orc_ctx *c);
+int orc_halo_only(orc_ctx *c);
+int orc_vt_kw(orc_ctx *c);
 int orc_monitor(orc_ctx *c);
 void orc_block_dims(orc_ctx *c, int blk, int *out9);
 int orc_get_array(orc_ctx *c, int blk, const char *name, double *out);

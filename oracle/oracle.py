@@ -41,7 +41,8 @@ def lib():
         L.orc_add_block.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp]
         L.orc_add_subface.argtypes = [vp, C.c_int, ip, C.c_int, ip]
         L.orc_get_inc.argtypes = [vp, C.c_int, ip]
-        for f in ("orc_setup", "orc_init_flow", "orc_bc_halo", "orc_step", "orc_monitor"):
+        for f in ("orc_setup", "orc_init_flow", "orc_bc_halo", "orc_step", "orc_monitor", "orc_residual",
+                  "orc_update", "orc_bc", "orc_halo_only", "orc_vt_kw"):
             getattr(L, f).argtypes = [vp]
         L.orc_set_interior_U.argtypes = [vp, C.c_int, dp]
         L.orc_block_dims.argtypes = [vp, C.c_int, ip]
@@ -109,6 +110,21 @@ class Oracle:
 
     def bc_halo(self):
         lib().orc_bc_halo(self._c)
+
+    def residual(self):
+        lib().orc_residual(self._c)
+
+    def update_limit(self):
+        lib().orc_update(self._c)
+
+    def bc(self):
+        lib().orc_bc(self._c)
+
+    def halo(self):
+        lib().orc_halo_only(self._c)
+
+    def force_vt_kw(self):
+        lib().orc_vt_kw(self._c)
 
     def step(self, n: int = 1):
         for _ in range(n):

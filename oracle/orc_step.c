@@ -1453,3 +1453,10 @@ int orc_monitor(orc_ctx *c) {
     if (orc_max_res(c, where)) return -1;
     return 0;
 }
+
+/* routine-level entry points for stage-by-stage comparisons */
+int orc_residual(orc_ctx *c) { orc_residual_all(c); return 0; }
+int orc_update(orc_ctx *c) { orc_update_limit(c); return 0; }
+int orc_bc(orc_ctx *c) { orc_boundary(c); return 0; }
+int orc_halo_only(orc_ctx *c) { orc_halo(c); return 0; }
+int orc_vt_kw(orc_ctx *c) { orc_force_vt_kw(c); c->tt = c->tt + c->dt_global; c->Kstep = c->Kstep + 1; return 0; }

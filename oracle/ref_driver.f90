@@ -13,7 +13,8 @@
 !   stateN.bin - the same after the last iteration
 !   norms.txt  - Kstep, Res_max(1:NVAR), Res_rms(1:NVAR) every kshow steps (comput_max_Res_onemesh)
 !   force.log  - written by the reference's comput_force every kshow steps
-! Usage: ref_driver NSTEPS [KSHOW]
+!   timing.txt - nsteps and the wall-clock seconds of the NS_Time_advance loop (system_clock)
+! Usage: ref_driver NSTEPS [KSHOW [DUMP]]   (DUMP=0 writes no array dumps, for timing runs)
 module dump_util
   implicit none
 contains
@@ -67,7 +68,8 @@ program ref_driver
   use Flow_Var
   use dump_util
   implicit none
-  integer :: ierr, nsteps, kshow, n, mb, i, j, k, m, u
+  integer :: ierr, nsteps, kshow, dump, n, mb, i, j, k, m, u
+  integer(8) :: c0, c1, crate
   character(len=64) :: arg
   logical :: have_u
   type(Block_TYPE), pointer :: B
@@ -76,12 +78,15 @@ program ref_driver
   call mpi_init(ierr)
   call mpi_comm_rank(MPI_COMM_WORLD, my_id, ierr)
   call mpi_comm_size(MPI_COMM_WORLD, Total_proc, ierr)
-  nsteps = 1; kshow = 1
+  nsteps = 1; kshow = 1; dump = 1
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg); read(arg, *) nsteps
   end if
   if (command_argument_count() >= 2) then
     call get_command_argument(2, arg); read(arg, *) kshow
+  end if
+  if (command_argument_count() >= 3) then
+    call get_command_argument(3, arg); read(arg, *) dump
   end if
 
   call read_parameter
@@ -115,21 +120,28 @@ program ref_driver
     call update_buffer_onemesh(1)
   end if
 
-  call dump_geometry('geom.bin')
-  call dump_state('state0.bin')
+  if (dump /= 0) then
+    call dump_geometry('geom.bin')
+    call dump_state('state0.bin')
+  end if
 
   open(newunit=u, file='norms.txt', status='replace')
+  call system_clock(c0, crate)
   do n = 1, nsteps
     call NS_Time_advance(1)
-    if (n == 1) call dump_state('state1.bin')
+    if (n == 1 .and. dump /= 0) call dump_state('state1.bin')
     if (mod(Mesh(1)%Kstep, kshow) == 0) then
       call comput_force
       call comput_max_Res_onemesh(1)
       write(u, '(I8,14ES26.17E3)') Mesh(1)%Kstep, Mesh(1)%Res_max(1:NVAR), Mesh(1)%Res_rms(1:NVAR)
     end if
   end do
+  call system_clock(c1)
   close(u)
-  call dump_state('stateN.bin')
+  open(newunit=u, file='timing.txt', status='replace')
+  write(u, '(I8,ES26.17E3)') nsteps, dble(c1 - c0) / dble(crate)
+  close(u)
+  if (dump /= 0) call dump_state('stateN.bin')
   call mpi_finalize(ierr)
 
 contains

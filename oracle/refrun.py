@@ -89,8 +89,11 @@ def read_forcelog(path: str):
 
 
 def run_case(case, nsteps: int = 1, kshow: int = 1, U_init=None, omp_threads: int = 0,
-             keep: Optional[str] = None, timeout: float = 1800.0) -> Dict[str, object]:
-    """Run the reference driver on ``case`` (an ocfd_amd.cases.Case) and return its dumps."""
+             keep: Optional[str] = None, timeout: float = 1800.0, dump: bool = True) -> Dict[str, object]:
+    """Run the reference driver on ``case`` (an ocfd_amd.cases.Case) and return its dumps.
+
+    With ``dump=False`` the driver writes no arrays; the result then holds the wall-clock seconds
+    of the iteration loop (``loop_seconds``) and the norms/force logs only."""
     import sys
     sys.path.insert(0, os.path.join(os.path.dirname(HERE), "opencfd-ec_amd"))
     from ocfd_amd import cases as C  # noqa: E402
@@ -107,7 +110,7 @@ def run_case(case, nsteps: int = 1, kshow: int = 1, U_init=None, omp_threads: in
         env = dict(os.environ)
         if omp_threads > 0:
             env["OMP_NUM_THREADS"] = str(omp_threads)
-        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {exe} {nsteps} {kshow}"],
+        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {exe} {nsteps} {kshow} {1 if dump else 0}"],
                            cwd=d, capture_output=True, text=True, timeout=timeout, env=env)
         if r.returncode != 0:
             raise RuntimeError(f"reference driver failed ({r.returncode}):\n{r.stdout[-3000:]}\n{r.stderr[-3000:]}")
@@ -116,8 +119,9 @@ def run_case(case, nsteps: int = 1, kshow: int = 1, U_init=None, omp_threads: in
             p = os.path.join(d, name + ".bin")
             if os.path.exists(p):
                 out[name] = read_dump(p)
-        nvar = out["geom"][0]["nvar"]
-        out["norms"] = read_norms(os.path.join(d, "norms.txt"), nvar)
+        with open(os.path.join(d, "timing.txt")) as f:
+            out["loop_seconds"] = float(f.read().split()[1])
+        out["norms"] = read_norms(os.path.join(d, "norms.txt"), C.nvar_of(case.params))
         out["force"] = read_forcelog(os.path.join(d, "force.log"))
         with open(os.path.join(d, "bc3d.inc")) as f:
             out["bc3d_inc"] = f.read()

@@ -100,6 +100,7 @@ def main() -> int:
                     help="Iflag_Flux (default 4 Roe; 3 = the reference's HLLC, which diverges on "
                          "the C-grid within a few iterations)")
     ap.add_argument("--graph", type=int, default=1, help="replay each stage as a captured hipGraph")
+    ap.add_argument("--lu", choices=("pipelined", "hyperplane"), default="pipelined", help="LU-SGS sweep schedule")
     ap.add_argument("--cpu-iters", type=int, default=20, help="iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
@@ -115,6 +116,7 @@ def main() -> int:
     s = A.solver_from_case(case)          # geometry, wall distance, freestream: all on the device
     nv = s.nv
     s.set_graph(bool(args.graph))
+    s.set_lu_schedule(A.LU_PIPELINED if args.lu == "pipelined" else A.LU_HYPERPLANE)
     for _ in range(args.warmup):
         s.step(1)
     s.sync()
@@ -163,6 +165,7 @@ def main() -> int:
             "scheme": SCHEME_NAMES.get(int(case.params["Iflag_Scheme"]), str(case.params["Iflag_Scheme"])),
             "flux": FLUX_NAMES[int(case.params["Iflag_Flux"])],
             "graph": bool(args.graph),
+            "lu_schedule": args.lu,
         },
         "roofline": roofline,
         "iteration_hbm": {"alg_bytes_per_cell": sum(words) * 8, "achieved": round(iter_bytes / (ms_per_step * 1e-3) / 1e9, 2),

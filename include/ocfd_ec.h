@@ -156,6 +156,15 @@ int ocfd_step_timed(ocfd_ctx *ctx, int32_t niter, double *stage_ms);
  * eagerly (0). */
 int ocfd_set_graph(ocfd_ctx *ctx, int32_t enable);
 
+/* Schedule of the LU-SGS sweeps; both give the reference's DU bit for bit.
+ *   OCFD_LU_PIPELINED  (default) one persistent launch per sweep: 8x8 (j,k) tiles march along i,
+ *                      tiles synchronised by progress counters
+ *   OCFD_LU_HYPERPLANE one launch per i+j+k hyperplane, the reference's own order
+ *                      (sub_LU_SGS.f90:20-126) */
+#define OCFD_LU_PIPELINED 0
+#define OCFD_LU_HYPERPLANE 1
+int ocfd_set_lu_schedule(ocfd_ctx *ctx, int32_t schedule);
+
 /* Wait for the context's stream. */
 int ocfd_sync(ocfd_ctx *ctx);
 

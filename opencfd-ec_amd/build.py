@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT, "libocfd_ec.so")
-SOURCES = ["ocfd_kernels.hip", "ocfd_geometry.hip", "ocfd_runtime.hip"]
+SOURCES = ["ocfd_kernels.hip", "ocfd_geometry.hip", "ocfd_lusgs.hip", "ocfd_runtime.hip"]
 ARCH = os.environ.get("OCFD_OFFLOAD_ARCH", "gfx950")
 # FP64 drop-in: no contraction, so device results follow the reference evaluation order.
 # Packed FP32 instructions stay off for every HIP object.

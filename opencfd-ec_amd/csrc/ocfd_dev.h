@@ -48,4 +48,22 @@ struct Sub {
     int L1, L2, L3;
 };
 
+// one tile of the pipelined LU-SGS sweep: cells j0..j0+tj-1, k0..k0+tk-1 of block `blk`, all i
+struct LuTask {
+    int blk, j0, k0, tj, tk;
+    int dep_j, off_j, last_j;    // upstream tile along j: task index (-1: none), step offset, its last step
+    int dep_k, off_k, last_k;    // upstream tile along k
+};
+
+// device tables of the pipelined sweeps of one context
+struct LuPlan {
+    const Blk* blks = nullptr;
+    const LuTask* fwd = nullptr;
+    const LuTask* bwd = nullptr;
+    int n_fwd = 0, n_bwd = 0;
+    int* prog_fwd = nullptr;     // last completed step per tile, -1 before the sweep
+    int* prog_bwd = nullptr;
+    int* err = nullptr;          // set when a wait exceeds its poll limit
+};
+
 }  // namespace ocfd

@@ -1,5 +1,7 @@
 // Host launchers of the hot-path kernels (ocfd_kernels.hip), called by the runtime.
 #pragma once
+#include <vector>
+
 #include "ocfd_dev.h"
 
 namespace ocfd {
@@ -12,6 +14,8 @@ void launch_laminar_amut(const Blk& b, const Sub* subs, int nsub, hipStream_t st
 void launch_inv_fluxes(const Blk& b, const Par& p, const Sub* subs, int nsub, hipStream_t st);
 void launch_visc_assemble_dt(const Blk& b, const Par& p, hipStream_t st);
 void launch_lusgs_sweep(const Blk& b, const Par& p, bool forward, hipStream_t st);
+std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward);
+void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, hipStream_t st);
 void launch_update_limit(const Blk& b, const Par& p, hipStream_t st);
 void launch_bc(const Blk& b, const Par& p, const Sub& B, hipStream_t st);
 void launch_halo(const Blk& src, const Blk& dst, const Sub& B, int NV, hipStream_t st);

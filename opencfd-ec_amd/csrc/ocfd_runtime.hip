@@ -53,6 +53,9 @@ struct ocfd_ctx {
     hipGraph_t graph[OCFD_NSTAGES] = {};
     std::vector<hipEvent_t> events;           // stage boundaries of ocfd_step_timed
     int device = 0;
+    int lu_schedule = OCFD_LU_PIPELINED;     // LU-SGS sweep schedule
+    LuPlan lu{};                              // pipelined-sweep tables (built on first use)
+    std::vector<void*> lu_allocs;
 };
 
 namespace {
@@ -228,10 +231,11 @@ void enqueue_stage(ocfd_ctx* c, int stage) {
         for (HostBlock& hb : c->blocks) launch_visc_assemble_dt(hb.d, c->p, c->st);
         break;
     case OCFD_STAGE_LU_FORWARD:
-        for (HostBlock& hb : c->blocks) launch_lusgs_sweep(hb.d, c->p, true, c->st);
-        break;
     case OCFD_STAGE_LU_BACKWARD:
-        for (HostBlock& hb : c->blocks) launch_lusgs_sweep(hb.d, c->p, false, c->st);
+        if (c->lu_schedule == OCFD_LU_PIPELINED)
+            launch_lusgs_pipelined(c->lu, c->p, stage == OCFD_STAGE_LU_FORWARD, c->st);
+        else
+            for (HostBlock& hb : c->blocks) launch_lusgs_sweep(hb.d, c->p, stage == OCFD_STAGE_LU_FORWARD, c->st);
         break;
     case OCFD_STAGE_UPDATE:
         for (HostBlock& hb : c->blocks) launch_update_limit(hb.d, c->p, c->st);
@@ -290,18 +294,67 @@ int run_stage(ocfd_ctx* c, int stage) {
     return OCFD_OK;
 }
 
+void drop_lu_plan(ocfd_ctx* c) {
+    for (void* q : c->lu_allocs) (void)hipFree(q);
+    c->lu_allocs.clear();
+    c->lu = LuPlan{};
+}
+
+// Device tables of the pipelined LU-SGS sweeps: the block table, the tile lists of both sweep
+// directions, one progress counter per tile and direction (-1 = nothing done) and the error flag.
+int build_lu_plan(ocfd_ctx* c) {
+    if (c->lu.fwd) return OCFD_OK;
+    std::vector<Blk> blks;
+    for (HostBlock& hb : c->blocks) blks.push_back(hb.d);
+    std::vector<LuTask> f = lusgs_tasks(blks, true), b = lusgs_tasks(blks, false);
+    auto put = [&](const void* src, size_t bytes, void** out) -> int {
+        void* q = nullptr;
+        HIPCHK(hipMalloc(&q, bytes));
+        c->lu_allocs.push_back(q);
+        HIPCHK(hipMemcpy(q, src, bytes, hipMemcpyHostToDevice));
+        *out = q;
+        return OCFD_OK;
+    };
+    std::vector<int> minus(std::max(f.size(), b.size()), -1), zero(4, 0);
+    void *pb, *pf, *pw, *qf, *qb, *pe;
+    int r;
+    if ((r = put(blks.data(), blks.size() * sizeof(Blk), &pb)) || (r = put(f.data(), f.size() * sizeof(LuTask), &pf)) ||
+        (r = put(b.data(), b.size() * sizeof(LuTask), &pw)) || (r = put(minus.data(), f.size() * sizeof(int), &qf)) ||
+        (r = put(minus.data(), b.size() * sizeof(int), &qb)) || (r = put(zero.data(), zero.size() * sizeof(int), &pe)))
+        return r;
+    c->lu.blks = (const Blk*)pb; c->lu.fwd = (const LuTask*)pf; c->lu.bwd = (const LuTask*)pw;
+    c->lu.n_fwd = (int)f.size(); c->lu.n_bwd = (int)b.size();
+    c->lu.prog_fwd = (int*)qf; c->lu.prog_bwd = (int*)qb; c->lu.err = (int*)pe;
+    return OCFD_OK;
+}
+
+// After a sweep gave up waiting: clear the flag and re-arm both counter sets.
+int check_lu_error(ocfd_ctx* c) {
+    if (!c->lu.err) return OCFD_OK;
+    int e = 0;
+    HIPCHK(hipMemcpy(&e, c->lu.err, sizeof(int), hipMemcpyDeviceToHost));
+    if (e == 0) return OCFD_OK;
+    std::vector<int> minus(std::max(c->lu.n_fwd, c->lu.n_bwd), -1);
+    HIPCHK(hipMemcpy(c->lu.prog_fwd, minus.data(), c->lu.n_fwd * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->lu.prog_bwd, minus.data(), c->lu.n_bwd * sizeof(int), hipMemcpyHostToDevice));
+    int z = 0;
+    HIPCHK(hipMemcpy(c->lu.err, &z, sizeof(int), hipMemcpyHostToDevice));
+    return fail(OCFD_EHIP, "LU-SGS sweep: a tile waited beyond its poll limit; the iteration is invalid");
+}
+
 int ready(ocfd_ctx* c) {
     if (!c) return fail(OCFD_EARG, "null context");
     if (c->blocks.empty()) return fail(OCFD_EARG, "no blocks added");
     for (size_t q = 0; q < c->blocks.size(); q++)
         if (c->blocks[q].no != (int)q + 1) return fail(OCFD_EARG, "block " + std::to_string(q + 1) + " missing");
+    if (c->lu_schedule == OCFD_LU_PIPELINED) return build_lu_plan(c);
     return OCFD_OK;
 }
 
 int finish(ocfd_ctx* c) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
-    return OCFD_OK;
+    return check_lu_error(c);
 }
 
 }  // namespace
@@ -335,6 +388,7 @@ int ocfd_destroy(ocfd_ctx* c) {
     if (!c) return OCFD_OK;
     if (c->st) (void)hipStreamSynchronize(c->st);
     drop_graphs(c);
+    drop_lu_plan(c);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     for (HostBlock& hb : c->blocks) for (void* q : hb.allocs) (void)hipFree(q);
     if (c->norm_part) (void)hipFree(c->norm_part);
@@ -354,6 +408,7 @@ static int add_block_common(ocfd_ctx* c, int32_t no, int32_t nx, int32_t ny, int
     int r = alloc_block(c, hb);
     if (r) return r;
     drop_graphs(c);
+    drop_lu_plan(c);
     *out = &hb;
     return OCFD_OK;
 }
@@ -773,6 +828,13 @@ int ocfd_get_counters(ocfd_ctx* c, int32_t* kstep, double* tt) {
 int ocfd_set_graph(ocfd_ctx* c, int32_t enable) {
     if (!c) return fail(OCFD_EARG, "null context");
     c->use_graph = enable != 0;
+    return OCFD_OK;
+}
+
+int ocfd_set_lu_schedule(ocfd_ctx* c, int32_t schedule) {
+    if (!c) return fail(OCFD_EARG, "null context");
+    if (schedule != OCFD_LU_PIPELINED && schedule != OCFD_LU_HYPERPLANE) return fail(OCFD_EARG, "unknown LU-SGS schedule");
+    if (schedule != c->lu_schedule) { drop_graphs(c); c->lu_schedule = schedule; }
     return OCFD_OK;
 }
 

@@ -56,8 +56,9 @@ EXPORTS = [
     "ocfd_setup_geometry", "ocfd_upload_U", "ocfd_download_U", "ocfd_download_field", "ocfd_init_flow",
     "ocfd_step_lusgs", "ocfd_residual", "ocfd_update_limit", "ocfd_bc", "ocfd_halo", "ocfd_force_vt_kw",
     "ocfd_res_norms", "ocfd_forces", "ocfd_block_status", "ocfd_get_counters", "ocfd_set_graph", "ocfd_sync",
-    "ocfd_step_timed",
+    "ocfd_step_timed", "ocfd_set_lu_schedule",
 ]
+LU_PIPELINED, LU_HYPERPLANE = 0, 1
 
 # stages of one iteration (OCFD_STAGE_* in include/ocfd_ec.h)
 STAGES = ("pre", "inviscid", "viscous_dt", "lu_forward", "lu_backward", "update", "bc_halo")
@@ -90,6 +91,7 @@ def lib():
         L.ocfd_get_counters.argtypes = [vp, ip, dp]
         L.ocfd_set_graph.argtypes = [vp, i32]
         L.ocfd_step_timed.argtypes = [vp, i32, dp]
+        L.ocfd_set_lu_schedule.argtypes = [vp, i32]
         _lib = L
     return _lib
 
@@ -310,6 +312,7 @@ class Solver:
         _check(lib().ocfd_step_timed(self._h, n, _dp(ms)))
         return dict(zip(STAGES, ms.tolist()))
     def set_graph(self, on: bool): _check(lib().ocfd_set_graph(self._h, 1 if on else 0))
+    def set_lu_schedule(self, schedule: int): _check(lib().ocfd_set_lu_schedule(self._h, schedule))
 
     def norms(self):
         mx = np.zeros(self.nv); rms = np.zeros(self.nv)

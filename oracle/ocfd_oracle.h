@@ -45,6 +45,10 @@ void orc_block_dims(orc_ctx *c, int blk, int *out9);
 int orc_get_array(orc_ctx *c, int blk, const char *name, double *out);
 int orc_set_array(orc_ctx *c, int blk, const char *name, const double *in);
 int orc_get_state(orc_ctx *c, int *kstep, double *tt, double *res_max, double *res_rms, double *force12);
+/* face-level hooks for closed-form checks: 1-D flux of rotated primitive states (rho, un, ut1, ut2,
+ * p), and the left/right reconstruction of u(-3..4) around face i-1/2 */
+int orc_face_flux(int iflux, const double *QL, const double *QR, double gamma, double *F);
+int orc_face_recon(int scheme, const double *u8, double *LR);
 
 #ifdef __cplusplus
 }

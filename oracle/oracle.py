@@ -49,6 +49,8 @@ def lib():
         L.orc_get_array.argtypes = [vp, C.c_int, cp, dp]
         L.orc_set_array.argtypes = [vp, C.c_int, cp, dp]
         L.orc_get_state.argtypes = [vp, ip, dp, dp, dp, dp]
+        L.orc_face_flux.argtypes = [C.c_int, dp, dp, C.c_double, dp]
+        L.orc_face_recon.argtypes = [C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -165,6 +167,22 @@ class Oracle:
         out = np.zeros(21 * 64, dtype=np.int32)
         n = lib().orc_get_inc(self._c, blk, _iptr(out))
         return out[:21 * n].reshape(n, 21)
+
+
+def face_flux(iflux: int, QL, QR, gamma: float = 1.4) -> np.ndarray:
+    """1-D numerical flux of the rotated primitive states (rho, u_n, u_t1, u_t2, p)."""
+    ql = np.ascontiguousarray(QL, dtype=np.float64); qr = np.ascontiguousarray(QR, dtype=np.float64)
+    F = np.zeros(5)
+    lib().orc_face_flux(int(iflux), _dptr(ql), _dptr(qr), float(gamma), _dptr(F))
+    return F
+
+
+def face_recon(scheme: int, u8) -> np.ndarray:
+    """(left, right) values at face i-1/2 from u(-3..4) with the given Iflag_Scheme."""
+    u = np.ascontiguousarray(u8, dtype=np.float64)
+    LR = np.zeros(2)
+    lib().orc_face_recon(int(scheme), _dptr(u), _dptr(LR))
+    return LR
 
 
 def pad_view(a: np.ndarray, lo, hi):

@@ -1460,3 +1460,18 @@ int orc_update(orc_ctx *c) { orc_update_limit(c); return 0; }
 int orc_bc(orc_ctx *c) { orc_boundary(c); return 0; }
 int orc_halo_only(orc_ctx *c) { orc_halo(c); return 0; }
 int orc_vt_kw(orc_ctx *c) { orc_force_vt_kw(c); c->tt = c->tt + c->dt_global; c->Kstep = c->Kstep + 1; return 0; }
+
+/* ------------------------------------------------------------------------------------------
+ * Face-level hooks, used by the tests' closed-form checks of the 1-D fluxes and the
+ * reconstructions: u8 = u(-3..4) around face i-1/2, LR[0] = left state, LR[1] = right state.
+ */
+int orc_face_flux(int iflux, const double *QL, const double *QR, double gamma, double *F) {
+    riemann(QL, QR, F, gamma, iflux);
+    return 0;
+}
+
+int orc_face_recon(int scheme, const double *u8, double *LR) {
+    LR[0] = scheme_fP(u8, scheme);
+    LR[1] = scheme_fm(u8, scheme);
+    return 0;
+}

@@ -8,6 +8,9 @@
 # on line 109 of that file that flang rejects.
 #
 #   make -f oracle/ref.mk            (from the repository root)
+#   make -f oracle/ref.mk dropin     the reference host with libocfd_ec.so dropped in through the
+#                                    Fortran binding (opencfd-ec_amd/fortran/ocfd_ec.f90); needs
+#                                    opencfd-ec_amd/lib/libocfd_ec.so
 #
 REF      ?= /root/reference
 OUT      := oracle/_ref
@@ -26,7 +29,7 @@ SRCS := sub_modules.f90 sub_interfaces.f90 sub_scheme.f90 sub_flux_split.f90 \
         sub_boundary_user.f90 sub_udate_Meshlink_mpi.f90 sub_Finite_Difference1.f90 \
         sub_Finite_Difference2.f90 sub_Post_timeAverage.f90 sub_limitflow.f90
 
-.PHONY: all serial omp clean
+.PHONY: all serial omp dropin clean
 all: serial omp
 
 serial: $(OUT)/ref_driver
@@ -52,6 +55,14 @@ endef
 
 $(eval $(call BUILD_VARIANT,obj_serial,,ref_driver))
 $(eval $(call BUILD_VARIANT,obj_omp,-fopenmp,ref_driver_omp))
+
+HIPLIB := opencfd-ec_amd/lib
+dropin: $(OUT)/dropin_driver
+$(OUT)/dropin_driver: $(OUT)/obj_serial/.stamp oracle/dropin_driver.f90 opencfd-ec_amd/fortran/ocfd_ec.f90 $(HIPLIB)/libocfd_ec.so
+	cd $(OUT)/obj_serial && $(FC) $(FFLAGS) -c ../../../opencfd-ec_amd/fortran/ocfd_ec.f90 -o ocfd_ec_binding.o && \
+	  $(FC) $(FFLAGS) -c ../../../oracle/dropin_driver.f90 -o dropin_driver.o && \
+	  $(FC) -o ../dropin_driver dropin_driver.o ocfd_ec_binding.o flow_var.o $(SRCS:.f90=.o) $(MPILIB) \
+	    -L../../../$(HIPLIB) -locfd_ec '-Wl,-rpath,$$ORIGIN/../../$(HIPLIB)'
 
 clean:
 	rm -rf $(OUT)

@@ -13,6 +13,10 @@ Workloads:
                              cells), 2e-6 wall spacing, 20-chord far field; SA RANS, Re 6e6,
                              Mach 0.8, AoA 1.25 deg, WENO5 reconstruction, FP64
   channel_bump_97x33x9       inviscid channel with a 10% bump, 97x33x9 nodes, MUSCL3 + Roe
+  naca4412_48block           577x193x129 C-grid around NACA 4412 cut 6x4x2 at seeded planes into 48
+                             uneven blocks (14,155,776 cells); SST, Re 3e6, Mach 0.7, MUSCL3 + AUSMPW
+  swept_wing_24block         1025x257x233 swept (30 deg), tapered (0.5) wing grid cut 4x3x2 into 24
+                             blocks (60,817,408 cells); SA, Re 1.2e7, Mach 0.84, AoA 3, WENO5 + Roe
 
 cpu_baseline: the compiled reference (oracle/_ref/ref_driver_omp, the reference's Fortran built
 from its own sources with amdflang) on a bounded number of iterations of the same workload,
@@ -48,7 +52,21 @@ def build_workload(name: str, flux: int):
         return C.naca_cgrid(257, 97, 33, Iflag_Flux=flux)
     if name == "channel_bump_97x33x9":
         return C.channel_bump(97, 33, 9)
+    if name == "naca4412_48block":
+        return C.multiblock_airfoil()
+    if name == "swept_wing_24block":
+        return C.swept_wing()
     raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_sample_case(name: str, flux: int):
+    """The CPU baseline's sample: the workload itself where the reference finishes setup and a few
+    iterations within seconds, else the same generator at 1/8 of the cells."""
+    if name == "naca4412_48block":
+        return C.multiblock_airfoil(scale=0.5), "the same 48-block generator at half resolution per direction"
+    if name == "swept_wing_24block":
+        return C.swept_wing(scale=0.5), "the same 24-block generator at half resolution per direction"
+    return build_workload(name, flux), "the benchmark workload"
 
 
 def ncells(case) -> int:
@@ -66,12 +84,13 @@ def host_cores() -> int:
     return max(1, n)
 
 
-def cpu_baseline(case, cells: int, iters: int):
-    """Reference CPU throughput on `iters` iterations of the same workload (rank 0, N=1 only)."""
+def cpu_baseline(case, what: str, iters: int):
+    """Reference CPU throughput on `iters` iterations of the workload sample (rank 0, N=1 only)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     cores = host_cores()
+    cells = ncells(case)
     import refrun  # noqa: E402
-    sample = f"{iters} iterations of the benchmark workload from freestream"
+    sample = f"{iters} iterations of {what} ({cells} cells) from freestream"
     if refrun.available(omp=True):
         try:
             r = refrun.run_case(case, nsteps=iters, kshow=10 ** 8, omp_threads=cores, dump=False, timeout=900)
@@ -100,7 +119,7 @@ def main() -> int:
                     help="Iflag_Flux (default 4 Roe; 3 = the reference's HLLC, which diverges on "
                          "the C-grid within a few iterations)")
     ap.add_argument("--graph", type=int, default=1, help="replay each stage as a captured hipGraph")
-    ap.add_argument("--lu", choices=("pipelined", "hyperplane"), default="pipelined", help="LU-SGS sweep schedule")
+    ap.add_argument("--lu", choices=("pipelined", "hyperplane", "fenced"), default="pipelined", help="LU-SGS sweep schedule")
     ap.add_argument("--cpu-iters", type=int, default=20, help="iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
@@ -116,7 +135,7 @@ def main() -> int:
     s = A.solver_from_case(case)          # geometry, wall distance, freestream: all on the device
     nv = s.nv
     s.set_graph(bool(args.graph))
-    s.set_lu_schedule(A.LU_PIPELINED if args.lu == "pipelined" else A.LU_HYPERPLANE)
+    s.set_lu_schedule({"pipelined": A.LU_PIPELINED, "hyperplane": A.LU_HYPERPLANE, "fenced": A.LU_PIPELINED_FENCED}[args.lu])
     for _ in range(args.warmup):
         s.step(1)
     s.sync()
@@ -175,7 +194,8 @@ def main() -> int:
         "state": {"res_rms1": float(rms[0]), "overlimit": int(overlimit)},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(case, cells, args.cpu_iters)
+        ccase, what = cpu_sample_case(args.workload, args.flux)
+        out["cpu_baseline"] = cpu_baseline(ccase, what, args.cpu_iters)
         out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 4)
     print(json.dumps(out), flush=True)
     return 0

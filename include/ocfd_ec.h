@@ -156,13 +156,17 @@ int ocfd_step_timed(ocfd_ctx *ctx, int32_t niter, double *stage_ms);
  * eagerly (0). */
 int ocfd_set_graph(ocfd_ctx *ctx, int32_t enable);
 
-/* Schedule of the LU-SGS sweeps; both give the reference's DU bit for bit.
- *   OCFD_LU_PIPELINED  (default) one persistent launch per sweep: 8x8 (j,k) tiles march along i,
- *                      tiles synchronised by progress counters
- *   OCFD_LU_HYPERPLANE one launch per i+j+k hyperplane, the reference's own order
- *                      (sub_LU_SGS.f90:20-126) */
+/* Schedule of the LU-SGS sweeps; all give the reference's DU bit for bit.
+ *   OCFD_LU_PIPELINED        (default) one persistent launch per sweep: 8x8 (j,k) tiles march
+ *                            along i; tile-edge DU and progress counters exchanged with
+ *                            device-coherent atomics
+ *   OCFD_LU_HYPERPLANE       one launch per i+j+k hyperplane, the reference's own order
+ *                            (sub_LU_SGS.f90:20-126)
+ *   OCFD_LU_PIPELINED_FENCED the pipelined schedule with agent-scope release/acquire fences at
+ *                            every step (L2 write-back and invalidate) */
 #define OCFD_LU_PIPELINED 0
 #define OCFD_LU_HYPERPLANE 1
+#define OCFD_LU_PIPELINED_FENCED 2
 int ocfd_set_lu_schedule(ocfd_ctx *ctx, int32_t schedule);
 
 /* Wait for the context's stream. */

@@ -28,6 +28,7 @@ struct Blk {
     double *vol, *S[3], *nv[9], *met[4][9], *dw, *dtm, *x, *y, *z, *xc, *yc, *zc;
     double *F[3][5];                     // inviscid face flux, m = 1..5
     double *Fv[3][4];                    // viscous face flux, m = 2..5
+    double *Fsa[3][2];                   // SA face terms: convection+diffusion flux, grad(vt).n S
     double *aux;                         // turbulence scratch (SST f1)
     uint8_t *bcf[3];                     // face-row flags: bit0 lower face physical, bit1 upper face physical
     double *surf[6][3];                  // Surf1..6 (wall viscous force of the i-/j-/k-/i+/j+/k+ faces)

@@ -73,60 +73,72 @@ __global__ void k_sa_mut(Blk b, Par p) {
     b.mut[s] = m2;
 }
 
-// Res(6): face fluxes in i, j, k then the source, in that order
-__global__ void k_sa_res(Blk b, Par p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > b.nx - 1) return;
-    const double SA_sigma = 2.0 / 3.0, Cb1 = 0.1355, Cb2 = 0.622, SA_k = 0.41;
-    const double Cw1 = Cb1 / (SA_k * SA_k) + (1.0 + Cb2) / SA_sigma, Cw2 = 0.3, Cw3 = 2.0, Ct3 = 1.2, Ct4 = 0.5;
-    const double Cv1_3 = pw3(7.1), Cw3_6 = pw6(Cw3);
-    const double c_v0 = (0.5 * (1.0 + Cb2)) / SA_sigma;
+// SA constants (sub_turbulence_SA.f90:18-33)
+struct SAc {
+    static constexpr double sigma = 2.0 / 3.0, Cb1 = 0.1355, Cb2 = 0.622, k = 0.41;
+    static constexpr double Cw2 = 0.3, Cw3 = 2.0, Ct3 = 1.2, Ct4 = 0.5;
+};
+
+// nu-tilde face flux of the SA equation on face (fi,fj,fk) of direction DIR (I-1/2 convention):
+// fv = (-upwind convection + (1+cb2)/sigma (vt+mu/rho)/Re grad(vt).n) S and f2 = grad(vt).n S
+// (sub_turbulence_SA.f90:60-174)
+template <int DIR>
+__device__ __forceinline__ void sa_face(const Blk& b, const Par& p, int fi, int fj, int fk, double& fv, double& f2) {
+    constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
+    const double c_v0 = (0.5 * (1.0 + SAc::Cb2)) / SAc::sigma;
     const double Re = p.Re;
     const double* d = b.prim[0];
     const double *uu = b.prim[1], *vv = b.prim[2], *ww = b.prim[3];
-    size_t s = IX(b, i, j, k);
-    double vt_c = vt_of(b, s), mu_c = b.mu[s] * Re;
-    double v0c = ((vt_c + mu_c / d[s]) * Cb2) / SA_sigma;
-    double res = 0.0;
-    for (int dir = 0; dir < 3; dir++) {
-        int di = dir == 0, dj = dir == 1, dk = dir == 2;
-        double fv[2], f2[2];
-        for (int side = 0; side < 2; side++) {
-            int fi = i + side * di, fj = j + side * dj, fk = k + side * dk;   // face index (I-1/2 convention)
-            size_t f = IX(b, fi, fj, fk), fm = IX(b, fi - di, fj - dj, fk - dk);
-            double s1x = b.nv[3 * dir][f], s1y = b.nv[3 * dir + 1][f], s1z = b.nv[3 * dir + 2][f];
-            double gi, gj, gk;
-            auto VT = [&](int a, int bb, int cc) { return vt_of(b, IX(b, a, bb, cc)); };
-            if (dir == 0) {
-                gi = VT(fi, fj, fk) - VT(fi - 1, fj, fk);
-                gj = 0.25 * (((VT(fi, fj + 1, fk) - VT(fi, fj - 1, fk)) + VT(fi - 1, fj + 1, fk)) - VT(fi - 1, fj - 1, fk));
-                gk = 0.25 * (((VT(fi, fj, fk + 1) - VT(fi, fj, fk - 1)) + VT(fi - 1, fj, fk + 1)) - VT(fi - 1, fj, fk - 1));
-            } else if (dir == 1) {
-                gi = 0.25 * (((VT(fi + 1, fj, fk) - VT(fi - 1, fj, fk)) + VT(fi + 1, fj - 1, fk)) - VT(fi - 1, fj - 1, fk));
-                gj = VT(fi, fj, fk) - VT(fi, fj - 1, fk);
-                gk = 0.25 * (((VT(fi, fj, fk + 1) - VT(fi, fj, fk - 1)) + VT(fi, fj - 1, fk + 1)) - VT(fi, fj - 1, fk - 1));
-            } else {
-                gi = 0.25 * (((VT(fi + 1, fj, fk) - VT(fi - 1, fj, fk)) + VT(fi + 1, fj, fk - 1)) - VT(fi - 1, fj, fk - 1));
-                gj = 0.25 * (((VT(fi, fj + 1, fk) - VT(fi, fj - 1, fk)) + VT(fi, fj + 1, fk - 1)) - VT(fi, fj - 1, fk - 1));
-                gk = VT(fi, fj, fk) - VT(fi, fj, fk - 1);
-            }
-            double* const* mt = b.met[dir + 1];
-            double vtx = (gi * mt[0][f] + gj * mt[3][f]) + gk * mt[6][f];
-            double vty = (gi * mt[1][f] + gj * mt[4][f]) + gk * mt[7][f];
-            double vtz = (gi * mt[2][f] + gj * mt[5][f]) + gk * mt[8][f];
-            double v0 = c_v0 * (((vt_of(b, f) + (b.mu[f] * Re) / d[f]) + vt_of(b, fm)) + (b.mu[fm] * Re) / d[fm]);
-            double vn1 = (uu[fm] * s1x + vv[fm] * s1y) + ww[fm] * s1z;
-            double vn2 = (uu[f] * s1x + vv[f] * s1y) + ww[f] * s1z;
-            double vfi = 0.5 * ((vn1 + fabs(vn1)) * vt_of(b, fm) + (vn2 - fabs(vn2)) * vt_of(b, f));
-            double gn = (vtx * s1x + vty * s1y) + vtz * s1z;
-            double Sf = b.S[dir][f];
-            fv[side] = (-vfi + (v0 / Re) * gn) * Sf;
-            f2[side] = gn * Sf;
-        }
-        if (dir == 0) res = (fv[1] - fv[0]) - (v0c / Re) * (f2[1] - f2[0]);
-        else res = ((res + fv[1]) - fv[0]) - (v0c / Re) * (f2[1] - f2[0]);
+    size_t f = IX(b, fi, fj, fk), fm = IX(b, fi - di, fj - dj, fk - dk);
+    double s1x = b.nv[3 * DIR][f], s1y = b.nv[3 * DIR + 1][f], s1z = b.nv[3 * DIR + 2][f];
+    double gi, gj, gk;
+    auto VT = [&](int a, int bb, int cc) { return vt_of(b, IX(b, a, bb, cc)); };
+    if (DIR == 0) {
+        gi = VT(fi, fj, fk) - VT(fi - 1, fj, fk);
+        gj = 0.25 * (((VT(fi, fj + 1, fk) - VT(fi, fj - 1, fk)) + VT(fi - 1, fj + 1, fk)) - VT(fi - 1, fj - 1, fk));
+        gk = 0.25 * (((VT(fi, fj, fk + 1) - VT(fi, fj, fk - 1)) + VT(fi - 1, fj, fk + 1)) - VT(fi - 1, fj, fk - 1));
+    } else if (DIR == 1) {
+        gi = 0.25 * (((VT(fi + 1, fj, fk) - VT(fi - 1, fj, fk)) + VT(fi + 1, fj - 1, fk)) - VT(fi - 1, fj - 1, fk));
+        gj = VT(fi, fj, fk) - VT(fi, fj - 1, fk);
+        gk = 0.25 * (((VT(fi, fj, fk + 1) - VT(fi, fj, fk - 1)) + VT(fi, fj - 1, fk + 1)) - VT(fi, fj - 1, fk - 1));
+    } else {
+        gi = 0.25 * (((VT(fi + 1, fj, fk) - VT(fi - 1, fj, fk)) + VT(fi + 1, fj, fk - 1)) - VT(fi - 1, fj, fk - 1));
+        gj = 0.25 * (((VT(fi, fj + 1, fk) - VT(fi, fj - 1, fk)) + VT(fi, fj + 1, fk - 1)) - VT(fi, fj - 1, fk - 1));
+        gk = VT(fi, fj, fk) - VT(fi, fj, fk - 1);
     }
-    // source (:181-249), CFL3D form
+    double* const* mt = b.met[DIR + 1];
+    double vtx = (gi * mt[0][f] + gj * mt[3][f]) + gk * mt[6][f];
+    double vty = (gi * mt[1][f] + gj * mt[4][f]) + gk * mt[7][f];
+    double vtz = (gi * mt[2][f] + gj * mt[5][f]) + gk * mt[8][f];
+    double v0 = c_v0 * (((vt_of(b, f) + (b.mu[f] * Re) / d[f]) + vt_of(b, fm)) + (b.mu[fm] * Re) / d[fm]);
+    double vn1 = (uu[fm] * s1x + vv[fm] * s1y) + ww[fm] * s1z;
+    double vn2 = (uu[f] * s1x + vv[f] * s1y) + ww[f] * s1z;
+    double vfi = 0.5 * ((vn1 + fabs(vn1)) * vt_of(b, fm) + (vn2 - fabs(vn2)) * vt_of(b, f));
+    double gn = (vtx * s1x + vty * s1y) + vtz * s1z;
+    double Sf = b.S[DIR][f];
+    fv = (-vfi + (v0 / Re) * gn) * Sf;
+    f2 = gn * Sf;
+}
+
+// Res(6) of cell s from its six stored face fluxes (i, j, k in that order) and the source in the
+// CFL3D form (sub_turbulence_SA.f90:103,138,174,181-249)
+__device__ __forceinline__ double sa_residual(const Blk& b, const Par& p, int i, int j, int k, size_t s) {
+    const double Cw1 = SAc::Cb1 / (SAc::k * SAc::k) + (1.0 + SAc::Cb2) / SAc::sigma;
+    const double Cv1_3 = pw3(7.1), Cw3_6 = pw6(SAc::Cw3);
+    const double Re = p.Re;
+    const double* d = b.prim[0];
+    const double *uu = b.prim[1], *vv = b.prim[2], *ww = b.prim[3];
+    double vt_c = vt_of(b, s), mu_c = b.mu[s] * Re;
+    double v0c = ((vt_c + mu_c / d[s]) * SAc::Cb2) / SAc::sigma;
+    const size_t sp[3] = {s + 1, s + (size_t)b.NI, s + (size_t)b.NI * b.NJ};
+    double res = 0.0;
+#pragma unroll
+    for (int dir = 0; dir < 3; dir++) {
+        double fv0 = b.Fsa[dir][0][s], fv1 = b.Fsa[dir][0][sp[dir]];
+        double f20 = b.Fsa[dir][1][s], f21 = b.Fsa[dir][1][sp[dir]];
+        if (dir == 0) res = (fv1 - fv0) - (v0c / Re) * (f21 - f20);
+        else res = ((res + fv1) - fv0) - (v0c / Re) * (f21 - f20);
+    }
     auto UU = [&](int a, int bb, int cc) { return uu[IX(b, a, bb, cc)]; };
     auto VV = [&](int a, int bb, int cc) { return vv[IX(b, a, bb, cc)]; };
     auto WW = [&](int a, int bb, int cc) { return ww[IX(b, a, bb, cc)]; };
@@ -143,16 +155,16 @@ __global__ void k_sa_res(Blk b, Par p) {
     double X = d[s] * vt_c / mu_c;
     double fv1 = pw3(X) / (pw3(X) + Cv1_3);
     double fv2 = 1.0 - X / (1.0 + X * fv1);
-    double ft2 = Ct3 * exp(-((Ct4 * X) * X));
-    double S1 = S + (fv2 * vt_c) / (Re * sq(SA_k * dw));
+    double ft2 = SAc::Ct3 * exp(-((SAc::Ct4 * X) * X));
+    double S1 = S + (fv2 * vt_c) / (Re * sq(SAc::k * dw));
     S1 = S1 > 0.0 ? S1 : 0.0;
-    double r = vt_c / (((((Re * S1) * SA_k) * SA_k) * dw) * dw);
+    double r = vt_c / (((((Re * S1) * SAc::k) * SAc::k) * dw) * dw);
     if (r > 10.0) r = 10.0;
-    double g = r + Cw2 * (pw6(r) - r);
+    double g = r + SAc::Cw2 * (pw6(r) - r);
     double fw = g * pow((1.0 + Cw3_6) / (pw6(g) + Cw3_6), 1.0 / 6.0);
-    double Q3 = (Cb1 * ((1.0 - ft2) * fv2 + ft2)) / (SA_k * SA_k) - Cw1 * fw;
-    double Q_SA = ((Cb1 * (1.0 - ft2)) * S1) * vt_c + (Q3 * sq(vt_c / dw)) / Re;
-    b.Res[5][s] = res + Q_SA * b.vol[s];
+    double Q3 = (SAc::Cb1 * ((1.0 - ft2) * fv2 + ft2)) / (SAc::k * SAc::k) - Cw1 * fw;
+    double Q_SA = ((SAc::Cb1 * (1.0 - ft2)) * S1) * vt_c + (Q3 * sq(vt_c / dw)) / Re;
+    return res + Q_SA * b.vol[s];
 }
 
 // Amut_boundary (sub_turbulence_SA.f90:282-368) on one wall subface
@@ -404,6 +416,12 @@ __global__ void k_inv_flux(Blk b, Par p, int overlimit_host) {
     b.F[DIR][2][s] = -((F0[1] * s1y + F0[2] * s2y) + F0[3] * s3y) * Sf;
     b.F[DIR][3][s] = -((F0[1] * s1z + F0[2] * s2z) + F0[3] * s3z) * Sf;
     b.F[DIR][4][s] = -F0[4] * Sf;
+    if (p.turb == 2) {
+        double fv, f2;
+        sa_face<DIR>(b, p, i, j, k, fv, f2);
+        b.Fsa[DIR][0][s] = fv;
+        b.Fsa[DIR][1][s] = f2;
+    }
 }
 
 // explicit average-state flux on a physical k subface (sub_Residual.f90:667-688)
@@ -508,7 +526,8 @@ __global__ void k_visc_flux(Blk b, Par p) {
 
 // Res(1:5): inviscid i, j, k then viscous i, j, k face differences, in the reference order
 // (sub_Residual.f90:346-354,507-516,692-701,864-872,1039-1048,1219-1229)
-__global__ void k_assemble(Blk b, Par p) {
+// followed by Res(6) of SA, then comput_Lijk + comput_dt (sub_time_acceleraction.f90:7-112)
+__global__ void k_cell(Blk b, Par p) {
     int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
     if (i > b.nx - 1) return;
     size_t s = IX(b, i, j, k);
@@ -526,18 +545,11 @@ __global__ void k_assemble(Blk b, Par p) {
         }
         b.Res[m][s] = r;
     }
-}
-
-// comput_Lijk + comput_dt (sub_time_acceleraction.f90:7-112)
-__global__ void k_lijk_dt(Blk b, Par p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > b.nx - 1) return;
-    size_t s = IX(b, i, j, k);
+    if (p.turb == 2) b.Res[5][s] = sa_residual(b, p, i, j, k, s);
     const double g43 = p.gamma > (double)(4.0f / 3.0f) ? p.gamma : (double)(4.0f / 3.0f);
     double d = b.prim[0][s], u = b.prim[1][s], v = b.prim[2][s], w = b.prim[3][s], c = b.prim[6][s];
     double D0 = p.visc == 1 ? (g43 / d) * (b.muf[s] / p.PrL + b.mut[s] / p.PrT) : 0.0;
     double vol = b.vol[s], vol1 = 1.0 / vol;
-    size_t sp[3] = {s + 1, s + (size_t)b.NI, s + (size_t)b.NI * b.NJ};
     double Lc[3], Lv[3];
 #pragma unroll
     for (int dir = 0; dir < 3; dir++) {
@@ -1031,7 +1043,6 @@ void launch_viscosity(const Blk& b, const Par& p, hipStream_t st) {
 
 void launch_sa(const Blk& b, const Par& p, const Sub* subs, int nsub, hipStream_t st) {
     hipLaunchKernelGGL(k_sa_mut, g3(b.nx + 1, b.ny + 1, b.nz + 1, 64), dim3(64), 0, st, b, p);
-    hipLaunchKernelGGL(k_sa_res, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
     for (int q = 0; q < nsub; q++) {
         if (subs[q].bc != 2) continue;
         int mx = std::max(b.nx, std::max(b.ny, b.nz));
@@ -1083,8 +1094,7 @@ void launch_visc_assemble_dt(const Blk& b, const Par& p, hipStream_t st) {
         hipLaunchKernelGGL(k_visc_flux<1>, g3(b.nx - 1, b.ny, b.nz - 1, 64), dim3(64), 0, st, b, p);
         hipLaunchKernelGGL(k_visc_flux<2>, g3(b.nx - 1, b.ny - 1, b.nz, 64), dim3(64), 0, st, b, p);
     }
-    hipLaunchKernelGGL(k_assemble, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
-    hipLaunchKernelGGL(k_lijk_dt, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
+    hipLaunchKernelGGL(k_cell, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
 }
 
 template <int NV>

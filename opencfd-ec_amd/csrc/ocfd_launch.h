@@ -15,7 +15,7 @@ void launch_inv_fluxes(const Blk& b, const Par& p, const Sub* subs, int nsub, hi
 void launch_visc_assemble_dt(const Blk& b, const Par& p, hipStream_t st);
 void launch_lusgs_sweep(const Blk& b, const Par& p, bool forward, hipStream_t st);
 std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward);
-void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, hipStream_t st);
+void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st);
 void launch_update_limit(const Blk& b, const Par& p, hipStream_t st);
 void launch_bc(const Blk& b, const Par& p, const Sub& B, hipStream_t st);
 void launch_halo(const Blk& src, const Blk& dst, const Sub& B, int NV, hipStream_t st);

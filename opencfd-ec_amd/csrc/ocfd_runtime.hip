@@ -101,6 +101,7 @@ int alloc_block(ocfd_ctx* c, HostBlock& hb) {
     }
     for (int dd = 0; dd < 3; dd++) for (int m = 0; m < 5; m++) if ((r = dalloc(hb, &b.F[dd][m], n))) return r;
     if (visc) for (int dd = 0; dd < 3; dd++) for (int m = 0; m < 4; m++) if ((r = dalloc(hb, &b.Fv[dd][m], n))) return r;
+    if (visc && c->p.turb == 2) for (int dd = 0; dd < 3; dd++) for (int m = 0; m < 2; m++) if ((r = dalloc(hb, &b.Fsa[dd][m], n))) return r;
     size_t jk = (size_t)b.NJ * b.NK, ik = (size_t)b.NI * b.NK, ij = (size_t)b.NI * b.NJ;
     if ((r = balloc(hb, &b.bcf[0], jk)) || (r = balloc(hb, &b.bcf[1], ik)) || (r = balloc(hb, &b.bcf[2], ij))) return r;
     size_t fsz[6] = {jk, ik, ij, jk, ik, ij};
@@ -232,8 +233,8 @@ void enqueue_stage(ocfd_ctx* c, int stage) {
         break;
     case OCFD_STAGE_LU_FORWARD:
     case OCFD_STAGE_LU_BACKWARD:
-        if (c->lu_schedule == OCFD_LU_PIPELINED)
-            launch_lusgs_pipelined(c->lu, c->p, stage == OCFD_STAGE_LU_FORWARD, c->st);
+        if (c->lu_schedule != OCFD_LU_HYPERPLANE)
+            launch_lusgs_pipelined(c->lu, c->p, stage == OCFD_STAGE_LU_FORWARD, c->lu_schedule == OCFD_LU_PIPELINED_FENCED, c->st);
         else
             for (HostBlock& hb : c->blocks) launch_lusgs_sweep(hb.d, c->p, stage == OCFD_STAGE_LU_FORWARD, c->st);
         break;
@@ -347,7 +348,7 @@ int ready(ocfd_ctx* c) {
     if (c->blocks.empty()) return fail(OCFD_EARG, "no blocks added");
     for (size_t q = 0; q < c->blocks.size(); q++)
         if (c->blocks[q].no != (int)q + 1) return fail(OCFD_EARG, "block " + std::to_string(q + 1) + " missing");
-    if (c->lu_schedule == OCFD_LU_PIPELINED) return build_lu_plan(c);
+    if (c->lu_schedule != OCFD_LU_HYPERPLANE) return build_lu_plan(c);
     return OCFD_OK;
 }
 
@@ -833,7 +834,8 @@ int ocfd_set_graph(ocfd_ctx* c, int32_t enable) {
 
 int ocfd_set_lu_schedule(ocfd_ctx* c, int32_t schedule) {
     if (!c) return fail(OCFD_EARG, "null context");
-    if (schedule != OCFD_LU_PIPELINED && schedule != OCFD_LU_HYPERPLANE) return fail(OCFD_EARG, "unknown LU-SGS schedule");
+    if (schedule != OCFD_LU_PIPELINED && schedule != OCFD_LU_HYPERPLANE && schedule != OCFD_LU_PIPELINED_FENCED)
+        return fail(OCFD_EARG, "unknown LU-SGS schedule");
     if (schedule != c->lu_schedule) { drop_graphs(c); c->lu_schedule = schedule; }
     return OCFD_OK;
 }

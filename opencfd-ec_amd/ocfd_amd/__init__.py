@@ -58,7 +58,7 @@ EXPORTS = [
     "ocfd_res_norms", "ocfd_forces", "ocfd_block_status", "ocfd_get_counters", "ocfd_set_graph", "ocfd_sync",
     "ocfd_step_timed", "ocfd_set_lu_schedule",
 ]
-LU_PIPELINED, LU_HYPERPLANE = 0, 1
+LU_PIPELINED, LU_HYPERPLANE, LU_PIPELINED_FENCED = 0, 1, 2
 
 # stages of one iteration (OCFD_STAGE_* in include/ocfd_ec.h)
 STAGES = ("pre", "inviscid", "viscous_dt", "lu_forward", "lu_backward", "update", "bc_halo")

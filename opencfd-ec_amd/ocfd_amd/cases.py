@@ -292,7 +292,7 @@ def solve_stretch(h1: float, total: float, ncell: int) -> float:
 
 def naca_cgrid(ni: int = 257, nj: int = 97, nk: int = 33, m: float = 0.0, p: float = 0.0,
                t: float = 0.12, span: float = 1.0, wall_spacing: float = 2e-6,
-               farfield: float = 20.0, **params) -> Case:
+               farfield: float = 20.0, sweep_deg: float = 0.0, taper: float = 1.0, **params) -> Case:
     """Single-block C-grid around an extruded NACA 4-digit section.
 
     i runs wake(lower) -> lower surface -> LE -> upper surface -> wake(upper); j runs from the wall
@@ -300,6 +300,8 @@ def naca_cgrid(ni: int = 257, nj: int = 97, nk: int = 33, m: float = 0.0, p: flo
     (bc=-1), the airfoil is a no-slip wall (bc=2), the outer and outflow faces are far field
     (bc=4), the span ends are symmetry planes (bc=3).  The wall-normal distribution is geometric
     from ``wall_spacing`` with the ratio solved so the outer boundary sits at ``farfield`` chords.
+    With ``sweep_deg``/``taper`` every spanwise station z is the section scaled by the local chord
+    c(z) = 1 - (1 - taper) z / span and shifted by z tan(sweep) (a swept, tapered extruded wing).
     """
     nw = int(round((ni - 1) * 0.15625)) + 1          # wake nodes per side incl. the TE node
     na = (ni - 2 * (nw - 1) - 1) // 2 + 1             # nodes per surface incl. LE and TE
@@ -335,8 +337,13 @@ def naca_cgrid(ni: int = 257, nj: int = 97, nk: int = 33, m: float = 0.0, p: flo
     X2 = xi[:, None] + s[None, :] * (xo - xi)[:, None]
     Y2 = yi[:, None] + s[None, :] * (yo - yi)[:, None]
     zz = np.linspace(0.0, span, nk)
-    x = np.repeat(X2[:, :, None], nk, axis=2)
-    y = np.repeat(Y2[:, :, None], nk, axis=2)
+    if sweep_deg == 0.0 and taper == 1.0:
+        x = np.repeat(X2[:, :, None], nk, axis=2)
+        y = np.repeat(Y2[:, :, None], nk, axis=2)
+    else:
+        c = 1.0 - (1.0 - taper) * zz / span
+        x = X2[:, :, None] * c[None, None, :] + (zz * math.tan(math.radians(sweep_deg)))[None, None, :]
+        y = Y2[:, :, None] * c[None, None, :]
     z = np.broadcast_to(zz[None, None, :], (ni, nj, nk)).copy()
     dims = (ni, nj, nk)
     iw0, iw1 = nw, ni - nw + 1                          # airfoil node range on j=1
@@ -508,6 +515,55 @@ def cut_blocks(case: Case, cuts: Sequence[Sequence[Sequence[int]]]) -> Case:
                             # the fixed dim's sign convention in the target follows Gridgen (positive)
                             child.subfaces.append(Subface(src, sf.bc, tuple(tgt) + (tcid,)))
     return Case(case.name + "_cut", newblocks, dict(case.params))
+
+
+def seeded_cuts(n_nodes: int, parts: int, rng: np.random.Generator, lo: int = 17, hi: int = 161) -> List[int]:
+    """Interior cut node indices splitting 1..n_nodes into `parts` pieces whose node counts lie in
+    [lo, hi]: piece sizes are the even split perturbed by seeded noise, then clipped to the range."""
+    cells = n_nodes - 1
+    if not (parts * (lo - 1) <= cells <= parts * (hi - 1)):
+        raise ValueError("no split of %d cells into %d pieces within [%d, %d] nodes" % (cells, parts, lo, hi))
+    for _ in range(1000):
+        w = rng.uniform(0.6, 1.4, parts)
+        sz = np.floor(w / w.sum() * cells).astype(int)
+        sz[-1] = cells - sz[:-1].sum()
+        if np.all(sz >= lo - 1) and np.all(sz <= hi - 1):
+            return [int(v) + 1 for v in np.cumsum(sz)[:-1]]
+    raise RuntimeError("no admissible split found")
+
+
+def multiblock_airfoil(seed: int = 3, scale: float = 1.0) -> Case:
+    """48-block uneven grid: the 577x193x129 C-grid around a NACA 4412 section (span 1) cut 6x4x2
+    at seeded planes, every block dim within [17, 161] nodes (14,155,776 cells); SST, Re 3e6,
+    Mach 0.7, MUSCL3 + AUSMPW (flux 6).  `scale` < 1 shrinks the parent for tests."""
+    def n(v):
+        return v if scale == 1.0 else max(33, int(round((v - 1) * scale)) // 32 * 32 + 1)
+    ni, nj, nk = n(577), n(193), n(129)
+    parent = naca_cgrid(ni, nj, nk, m=0.04, p=0.4, t=0.12, span=1.0, Iflag_turbulence_model=3, Re=3e6,
+                        Ma=0.7, Iflag_Scheme=5, Iflag_Flux=6)
+    rng = np.random.default_rng(seed)
+    lo, hi = (17, 161) if scale == 1.0 else (5, 161)
+    cuts = [[seeded_cuts(ni, 6, rng, lo, hi), seeded_cuts(nj, 4, rng, lo, hi), seeded_cuts(nk, 2, rng, lo, hi)]]
+    case = cut_blocks(parent, cuts)
+    case.name = "naca4412_48block"
+    return case
+
+
+def swept_wing(seed: int = 4, scale: float = 1.0) -> Case:
+    """24-block swept, tapered wing: the 1025x257x233 C-grid around NACA 0012 extruded over a span of
+    3 chords with 30 deg sweep and taper 0.5, wall spacing 1e-6, cut 4x3x2 at seeded planes
+    (60,817,408 cells); SA, Mach 0.84, AoA 3 deg, Re 1.2e7, WENO5 + Roe.  The wall-normal ratio is
+    solved for a 20-chord far field (a 1.1 ratio from 1e-6 over 256 cells would reach 4e5 chords)."""
+    def n(v):
+        return v if scale == 1.0 else max(33, int(round((v - 1) * scale)) // 32 * 32 + 1)
+    ni, nj, nk = n(1025), n(257), n(233)
+    parent = naca_cgrid(ni, nj, nk, span=3.0, wall_spacing=1e-6, sweep_deg=30.0, taper=0.5,
+                        Iflag_turbulence_model=2, Re=1.2e7, Ma=0.84, AoA=3.0, Iflag_Scheme=7, Iflag_Flux=4)
+    rng = np.random.default_rng(seed)
+    cuts = [[seeded_cuts(ni, 4, rng, 5, 10 ** 6), seeded_cuts(nj, 3, rng, 5, 10 ** 6), seeded_cuts(nk, 2, rng, 5, 10 ** 6)]]
+    case = cut_blocks(parent, cuts)
+    case.name = "swept_wing_24block"
+    return case
 
 
 # ---------------------------------------------------------------------------------------------

@@ -14,6 +14,8 @@
 !   norms.txt  - Kstep, Res_max(1:NVAR), Res_rms(1:NVAR) every kshow steps (comput_max_Res_onemesh)
 !   force.log  - written by the reference's comput_force every kshow steps
 !   timing.txt - nsteps and the wall-clock seconds of the NS_Time_advance loop (system_clock)
+!   flow3d.dat, SA3d.dat/SST3d.dat, Step_mess.dat, wall_dist.dat - the reference's own restart and
+!                wall-distance writers (output_flow, write_dw) after the last iteration
 ! Usage: ref_driver NSTEPS [KSHOW [DUMP]]   (DUMP=0 writes no array dumps, for timing runs)
 module dump_util
   implicit none
@@ -141,7 +143,11 @@ program ref_driver
   open(newunit=u, file='timing.txt', status='replace')
   write(u, '(I8,ES26.17E3)') nsteps, dble(c1 - c0) / dble(crate)
   close(u)
-  if (dump /= 0) call dump_state('stateN.bin')
+  if (dump /= 0) then
+    call dump_state('stateN.bin')
+    call output_flow                                   ! flow3d.dat (+ SA3d/SST3d), Step_mess.dat
+    if (Mesh(1)%Iflag_turbulence_model /= 0) call write_dw   ! wall_dist.dat
+  end if
   call mpi_finalize(ierr)
 
 contains

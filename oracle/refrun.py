@@ -125,6 +125,11 @@ def run_case(case, nsteps: int = 1, kshow: int = 1, U_init=None, omp_threads: in
         out["force"] = read_forcelog(os.path.join(d, "force.log"))
         with open(os.path.join(d, "bc3d.inc")) as f:
             out["bc3d_inc"] = f.read()
+        for name in ("flow3d.dat", "SA3d.dat", "SST3d.dat", "wall_dist.dat", "Step_mess.dat"):
+            p = os.path.join(d, name)
+            if os.path.exists(p):
+                with open(p, "rb") as f:
+                    out[name] = f.read()
         return out
     finally:
         if keep:

@@ -146,9 +146,10 @@ def test_c_grid_257x97x33(flux, iters, path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("schedule", ["pipelined", "fenced"])
 @pytest.mark.parametrize("name", ["channel_roe_muscl3", "channel_cut6", "cgrid_sa_limiter", "cgrid_sst_muscl3_ausm",
                                   "cgrid_laminar_vl", "cgrid_sa_cut12"])
-def test_lu_schedules_agree(name):
+def test_lu_schedules_agree(name, schedule):
     """The pipelined tile schedule of the LU-SGS sweeps and the reference's hyperplane order give
     the same state bit for bit, iteration after iteration (graph replay included)."""
     print(f"\n[{name}]", flush=True)
@@ -158,7 +159,7 @@ def test_lu_schedules_agree(name):
     a = A.solver_from_case(case, states=U0)
     b = A.solver_from_case(case, states=U0)
     a.set_lu_schedule(A.LU_HYPERPLANE)
-    b.set_lu_schedule(A.LU_PIPELINED)
+    b.set_lu_schedule(A.LU_PIPELINED if schedule == "pipelined" else A.LU_PIPELINED_FENCED)
     b.set_graph(True)
     for it in range(4):
         a.step(1); b.step(1)

@@ -54,6 +54,9 @@ struct LuTask {
     int blk, j0, k0, tj, tk;
     int dep_j, off_j, last_j;    // upstream tile along j: task index (-1: none), step offset, its last step
     int dep_k, off_k, last_k;    // upstream tile along k
+    int pofs;                    // sweep-ordered storage of this tile: element (step q, lane) at pofs + 64 q + lane
+    int pofs_j, tj_up;           // the j-upstream tile's storage offset and j extent
+    int pofs_k, tk_up;           // the k-upstream tile's storage offset and k extent
 };
 
 // device tables of the pipelined sweeps of one context
@@ -65,6 +68,10 @@ struct LuPlan {
     int* prog_fwd = nullptr;     // last completed step per tile, -1 before the sweep
     int* prog_bwd = nullptr;
     int* err = nullptr;          // set when a wait exceeds its poll limit
+    // sweep-ordered copies of the sweep's inputs and of DU (field f of element e at pk[f * P + e])
+    double* pk = nullptr;
+    long long P = 0;
+    int nwin = 0;                // i-windows per tile of the pack/unpack transposes
 };
 
 }  // namespace ocfd

@@ -14,7 +14,10 @@ void launch_laminar_amut(const Blk& b, const Sub* subs, int nsub, hipStream_t st
 void launch_inv_fluxes(const Blk& b, const Par& p, const Sub* subs, int nsub, hipStream_t st);
 void launch_visc_assemble_dt(const Blk& b, const Par& p, hipStream_t st);
 void launch_lusgs_sweep(const Blk& b, const Par& p, bool forward, hipStream_t st);
-std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward);
+std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, long long* total_elements);
+int lusgs_fields(int NV);
+int lusgs_windows(int max_cells_i);
+void launch_lusgs_pack_geometry(const LuPlan& L, int NV, hipStream_t st);
 void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st);
 void launch_update_limit(const Blk& b, const Par& p, hipStream_t st);
 void launch_bc(const Blk& b, const Par& p, const Sub& B, hipStream_t st);

@@ -221,27 +221,6 @@ struct LuRemote {
     double Lc, Lv, S;
 };
 
-template <int NV, bool FWD>
-__device__ __forceinline__ void lu_load_own(const Blk& b, size_t s, LuOwn<NV>& o) {
-#pragma unroll
-    for (int m = 0; m < NV; m++) { o.R[m] = FWD ? b.Res[m][s] : b.DU[m][s]; o.U[m] = b.U[m][s]; }
-#pragma unroll
-    for (int d = 0; d < 3; d++) { o.Lc[d] = b.Lc[d][s]; o.Lv[d] = b.Lv[d][s]; o.S[d] = b.S[d][s]; }
-#pragma unroll
-    for (int q = 0; q < 9; q++) o.n[q] = b.nv[q][s];
-    o.vol = b.vol[s];
-    o.dt = b.dt[s];
-}
-
-template <int NV, bool FWD>
-__device__ __forceinline__ void lu_load_remote(const Blk& b, size_t t, int dir, LuRemote<NV>& r) {
-#pragma unroll
-    for (int m = 0; m < NV; m++) { r.U[m] = b.U[m][t]; r.DU[m] = ld_coherent(b.DU[m] + t); }
-    r.Lc = b.Lc[dir][t];
-    r.Lv = b.Lv[dir][t];
-    r.S = FWD ? 0.0 : b.S[dir][t];
-}
-
 // du contribution of one upstream neighbour (sub_LU_SGS.f90:42-60 forward, :94-119 backward)
 template <int NV, bool FWD>
 __device__ __forceinline__ void lu_neighbour(double* du, const double* Un_in, const double* DUn, double Lcn, double Lvn,
@@ -267,10 +246,121 @@ __device__ __forceinline__ void lu_neighbour(double* du, const double* Un_in, co
     }
 }
 
-template <int NV, bool FWD, int PUB>
+// Fields of the sweep-ordered storage: for a tile, element (q, lane) holds the cell that lane
+// handles at forward step q (the backward sweep visits the same cells at step nsteps-1-q), so
+// every step reads and writes 64 consecutive doubles per field.
+template <int NV> struct Pk {
+    static constexpr int U = 0, R = NV, DU = 2 * NV, LC = 3 * NV, LV = 3 * NV + 3, DT = 3 * NV + 6, VOL = 3 * NV + 7,
+                         S = 3 * NV + 8, N = 3 * NV + 11, NF = 3 * NV + 20;
+};
+
+template <int NV>
+__device__ __forceinline__ double* box_field(const Blk& b, int f) {
+    if (f < NV) return b.U[f];
+    f -= NV;
+    if (f < NV) return b.Res[f];
+    f -= NV;
+    if (f < NV) return b.DU[f];
+    f -= NV;
+    if (f < 3) return b.Lc[f];
+    f -= 3;
+    if (f < 3) return b.Lv[f];
+    f -= 3;
+    if (f == 0) return b.dt;
+    if (f == 1) return b.vol;
+    f -= 2;
+    if (f < 3) return b.S[f];
+    f -= 3;
+    return b.nv[f];
+}
+
+constexpr int PK_W = 32;      // i-window of one transpose workgroup
+constexpr int PK_T = 256;
+
+// box <-> sweep order for fields [f0,f1) and [f2,f3) of one tile and one i-window, through LDS so
+// that both sides are read and written in contiguous runs
+template <int NV, bool TO_PACKED>
+__global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
+                                                       double* __restrict__ pk, long long P, int f0, int f1, int f2, int f3) {
+    __shared__ double buf[LU_NT][PK_W + 1];
+    const LuTask T = tiles[blockIdx.x];
+    const Blk& b = blks[T.blk];
+    const int nxm = b.nx - 1;
+    const int i0 = 1 + (int)blockIdx.y * PK_W;
+    if (i0 > nxm) return;
+    const int W = min(PK_W, nxm - i0 + 1);
+    const int nsteps = nxm + T.tj + T.tk - 2;
+    const int t = threadIdx.x;
+    for (int f = f0; f < f3; f++) {
+        if (f == f1) { f = f2 - 1; continue; }
+        double* box = box_field<NV>(b, f);
+        double* pf = pk + (size_t)f * (size_t)P + T.pofs;
+        if (TO_PACKED) {
+            for (int e = t; e < LU_NT * PK_W; e += PK_T) {
+                int r = e / PK_W, c = e % PK_W, jl = r % LU_TJ, kl = r / LU_TJ;
+                if (jl < T.tj && kl < T.tk && c < W) buf[r][c] = box[IX(b, i0 + c, T.j0 + jl, T.k0 + kl)];
+            }
+            __syncthreads();
+            for (int e = t; e < (W + LU_TJ + LU_TK - 2) * LU_NT; e += PK_T) {
+                int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % LU_TJ, kl = lane / LU_TJ;
+                int i = q - jl - kl + 1;
+                if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * LU_NT + lane] = buf[lane][i - i0];
+            }
+        } else {
+            for (int e = t; e < (W + LU_TJ + LU_TK - 2) * LU_NT; e += PK_T) {
+                int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % LU_TJ, kl = lane / LU_TJ;
+                int i = q - jl - kl + 1;
+                if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) buf[lane][i - i0] = pf[(size_t)q * LU_NT + lane];
+            }
+            __syncthreads();
+            for (int e = t; e < LU_NT * PK_W; e += PK_T) {
+                int r = e / PK_W, c = e % PK_W, jl = r % LU_TJ, kl = r / LU_TJ;
+                if (jl < T.tj && kl < T.tk && c < W) box[IX(b, i0 + c, T.j0 + jl, T.k0 + kl)] = buf[r][c];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int NV, bool FWD>
+__device__ __forceinline__ void lu_load_own_pk(const double* pk, long long P, size_t e, LuOwn<NV>& o) {
+    using F = Pk<NV>;
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+        o.R[m] = pk[(size_t)((FWD ? F::R : F::DU) + m) * P + e];
+        o.U[m] = pk[(size_t)(F::U + m) * P + e];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        o.Lc[d] = pk[(size_t)(F::LC + d) * P + e];
+        o.Lv[d] = pk[(size_t)(F::LV + d) * P + e];
+        o.S[d] = pk[(size_t)(F::S + d) * P + e];
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) o.n[q] = pk[(size_t)(F::N + q) * P + e];
+    o.vol = pk[(size_t)F::VOL * P + e];
+    o.dt = pk[(size_t)F::DT * P + e];
+}
+
+template <int NV, bool FWD>
+__device__ __forceinline__ void lu_load_remote_pk(const double* pk, long long P, size_t e, int dir, LuRemote<NV>& r) {
+    using F = Pk<NV>;
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+        r.U[m] = pk[(size_t)(F::U + m) * P + e];
+        r.DU[m] = ld_coherent(pk + (size_t)(F::DU + m) * P + e);
+    }
+    r.Lc = pk[(size_t)(F::LC + dir) * P + e];
+    r.Lv = pk[(size_t)(F::LV + dir) * P + e];
+    r.S = FWD ? 0.0 : pk[(size_t)(F::S + dir) * P + e];
+}
+
+template <int NV, bool FWD>
 __global__ void __launch_bounds__(LU_NT) k_lusgs_pipe(const Blk* __restrict__ blks, const LuTask* __restrict__ tasks,
+                                                      double* __restrict__ pk, long long P,
                                                       int* __restrict__ prog, int* __restrict__ prog_other,
                                                       int n_other, int* __restrict__ err, Par p) {
+    using F = Pk<NV>;
     __shared__ LuSlot<NV> slot[2][LU_NT];
     __shared__ int bail;
     const int w = blockIdx.x, lane = threadIdx.x;
@@ -283,29 +373,28 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_pipe(const Blk* __restrict__ bl
     const int jo = FWD ? jl : T.tj - 1 - jl, ko = FWD ? kl : T.tk - 1 - kl;
     const int j = T.j0 + jl, k = T.k0 + kl;
     const int nxm = b.nx - 1;
-    const size_t sj = (size_t)b.NI, sk = (size_t)b.NI * b.NJ;
     const double wlu = p.w_LU, gamma = p.gamma;
     const bool visc = p.visc == 1;
     const int nsteps = nxm + T.tj + T.tk - 2;
-    // neighbour sources of the j and k directions: none (block face), same tile (LDS) or upstream tile
     const bool has_j = FWD ? j != 1 : j != b.ny - 1, has_k = FWD ? k != 1 : k != b.nz - 1;
     const bool rem_j = live && has_j && jo == 0, rem_k = live && has_k && ko == 0;
-    // edge lanes whose DU a downstream tile reads
     const bool pub_lane = live && (jo == T.tj - 1 || ko == T.tk - 1);
     const bool deps = T.dep_j >= 0 || T.dep_k >= 0;
-    int known_j = -1, known_k = -1;
+    // the upstream lanes this lane reads across the tile edges
+    const int jlu = FWD ? T.tj_up - 1 : 0, klu = FWD ? T.tk_up - 1 : 0;
+    int known_j = -1, known_k = -1, got_j = -1, got_k = -1;
     if (lane == 0) bail = 0;
 
     auto cell_i = [&](int q) { return FWD ? q - jo - ko + 1 : nxm - (q - jo - ko); };
-    // lane 0: wait until the upstream tiles have published what step q reads
+    auto own_e = [&](int q) { return (size_t)T.pofs + (size_t)(FWD ? q : nsteps - 1 - q) * LU_NT + lane; };
+    auto need_j = [&](int q) { return T.dep_j >= 0 ? min(q + T.off_j, T.last_j) : -1; };
+    auto need_k = [&](int q) { return T.dep_k >= 0 ? min(q + T.off_k, T.last_k) : -1; };
     auto wait_for = [&](int q) {
-        int need_j = T.dep_j >= 0 ? min(q + T.off_j, T.last_j) : -1;
-        int need_k = T.dep_k >= 0 ? min(q + T.off_k, T.last_k) : -1;
         long polls = 0;
-        while (known_j < need_j || known_k < need_k) {
-            if (known_j < need_j) known_j = __hip_atomic_load(prog + T.dep_j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (known_k < need_k) known_k = __hip_atomic_load(prog + T.dep_k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (known_j >= need_j && known_k >= need_k) break;
+        while (known_j < need_j(q) || known_k < need_k(q)) {
+            if (known_j < need_j(q)) known_j = __hip_atomic_load(prog + T.dep_j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (known_k < need_k(q)) known_k = __hip_atomic_load(prog + T.dep_k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (known_j >= need_j(q) && known_k >= need_k(q)) break;
             if (++polls > LU_POLL_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
                 __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 bail = 1;
@@ -314,25 +403,26 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_pipe(const Blk* __restrict__ bl
             __builtin_amdgcn_s_sleep(1);
         }
     };
+    auto load_remote = [&](int q, LuRemote<NV>& rj, LuRemote<NV>& rk) {
+        const int iq = cell_i(q);
+        if (live && iq >= 1 && iq <= nxm && q < nsteps) {
+            if (rem_j) lu_load_remote_pk<NV, FWD>(pk, P, (size_t)T.pofs_j + (size_t)(iq - 1 + jlu + kl) * LU_NT + jlu + LU_TJ * kl, 1, rj);
+            if (rem_k) lu_load_remote_pk<NV, FWD>(pk, P, (size_t)T.pofs_k + (size_t)(iq - 1 + jl + klu) * LU_NT + jl + LU_TJ * klu, 2, rk);
+        }
+    };
 
     LuOwn<NV> own, nxt;
-    LuRemote<NV> rj, rk;
+    LuRemote<NV> rj, rk, rjn, rkn;
     double pU[NV], pDU[NV], pLc = 0.0, pLv = 0.0, pS = 0.0;
 #pragma unroll
     for (int m = 0; m < NV; m++) { pU[m] = 0.0; pDU[m] = 0.0; }
-
-    // step 0 inputs
     {
         int i0 = cell_i(0);
-        if (live && i0 >= 1 && i0 <= nxm) lu_load_own<NV, FWD>(b, IX(b, i0, j, k), own);
-        if (deps && lane == 0) wait_for(0);
+        if (live && i0 >= 1 && i0 <= nxm) lu_load_own_pk<NV, FWD>(pk, P, own_e(0), own);
+        if (deps && lane == 0) wait_for(min(1, nsteps - 1));
         __syncthreads();
         if (bail) return;
-        if (live && i0 >= 1 && i0 <= nxm) {
-            size_t s = IX(b, i0, j, k);
-            if (rem_j) lu_load_remote<NV, FWD>(b, FWD ? s - sj : s + sj, 1, rj);
-            if (rem_k) lu_load_remote<NV, FWD>(b, FWD ? s - sk : s + sk, 2, rk);
-        }
+        load_remote(0, rj, rk);
     }
 
     for (int q = 0; q < nsteps; q++) {
@@ -340,11 +430,15 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_pipe(const Blk* __restrict__ bl
         const int i = cell_i(q), i1 = cell_i(q + 1);
         const bool act = live && i >= 1 && i <= nxm;
         const bool act1 = live && i1 >= 1 && i1 <= nxm && q + 1 < nsteps;
-        // next step's static inputs
-        if (act1) lu_load_own<NV, FWD>(b, IX(b, i1, j, k), nxt);
+        if (act1) lu_load_own_pk<NV, FWD>(pk, P, own_e(q + 1), nxt);
+        load_remote(q + 1, rjn, rkn);
+        const bool look = deps && lane == 0 && q + 2 < nsteps && (known_j < need_j(q + 2) || known_k < need_k(q + 2));
+        if (look) {
+            if (T.dep_j >= 0) got_j = __hip_atomic_load(prog + T.dep_j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (T.dep_k >= 0) got_k = __hip_atomic_load(prog + T.dep_k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         double oDU[NV];
         if (act) {
-            const size_t s = IX(b, i, j, k);
             double sumLc = (own.Lc[0] + own.Lc[1]) + own.Lc[2];
             double sumLv = (own.Lv[0] + own.Lv[1]) + own.Lv[2];
             double alfa0 = own.vol / own.dt + wlu * sumLc;
@@ -361,7 +455,6 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_pipe(const Blk* __restrict__ bl
             }
 #pragma unroll
             for (int m = 0; m < NV; m++) alfa[m] = alfa[m] + 0.0 * own.vol;
-            // neighbours from this lane and this tile first, the upstream tile's last
             double du0[NV], du1[NV], du2[NV];
             const bool has_i = FWD ? i != 1 : i != nxm;
             if (has_i) lu_neighbour<NV, FWD>(du0, pU, pDU, pLc, pLv, pS, own, 0, wlu, gamma, visc);
@@ -372,15 +465,17 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_pipe(const Blk* __restrict__ bl
             // (indices clamped: they are only read when the neighbour lane exists)
             const LuSlot<NV>& oj = slot[buf ^ 1][min(max(FWD ? lane - 1 : lane + 1, 0), LU_NT - 1)];
             const LuSlot<NV>& ok = slot[buf ^ 1][min(max(FWD ? lane - LU_TJ : lane + LU_TJ, 0), LU_NT - 1)];
-            if (has_j && !rem_j) lu_neighbour<NV, FWD>(du1, oj.U, oj.DU, oj.Lc[0], oj.Lv[0], oj.S[0], own, 1, wlu, gamma, visc);
-            if (has_k && !rem_k) lu_neighbour<NV, FWD>(du2, ok.U, ok.DU, ok.Lc[1], ok.Lv[1], ok.S[1], own, 2, wlu, gamma, visc);
-            if (rem_j) lu_neighbour<NV, FWD>(du1, rj.U, rj.DU, rj.Lc, rj.Lv, rj.S, own, 1, wlu, gamma, visc);
-            if (rem_k) lu_neighbour<NV, FWD>(du2, rk.U, rk.DU, rk.Lc, rk.Lv, rk.S, own, 2, wlu, gamma, visc);
-            if (!has_j) {
+            if (has_j) {
+                if (rem_j) lu_neighbour<NV, FWD>(du1, rj.U, rj.DU, rj.Lc, rj.Lv, rj.S, own, 1, wlu, gamma, visc);
+                else lu_neighbour<NV, FWD>(du1, oj.U, oj.DU, oj.Lc[0], oj.Lv[0], oj.S[0], own, 1, wlu, gamma, visc);
+            } else {
 #pragma unroll
                 for (int m = 0; m < NV; m++) du1[m] = 0.0;
             }
-            if (!has_k) {
+            if (has_k) {
+                if (rem_k) lu_neighbour<NV, FWD>(du2, rk.U, rk.DU, rk.Lc, rk.Lv, rk.S, own, 2, wlu, gamma, visc);
+                else lu_neighbour<NV, FWD>(du2, ok.U, ok.DU, ok.Lc[1], ok.Lv[1], ok.S[1], own, 2, wlu, gamma, visc);
+            } else {
 #pragma unroll
                 for (int m = 0; m < NV; m++) du2[m] = 0.0;
             }
@@ -389,44 +484,54 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_pipe(const Blk* __restrict__ bl
                 double acc = FWD ? own.R[m] : 0.0;
                 acc = ((acc + du0[m]) + du1[m]) + du2[m];
                 oDU[m] = FWD ? acc / alfa[m] : own.R[m] + acc / alfa[m];
-                if (pub_lane) st_coherent(b.DU[m] + s, oDU[m]);
-                else b.DU[m][s] = oDU[m];
+            }
+        }
+        // every memory operation issued before this point is complete: the previous step's DU
+        // stores, and this step's loads.  Publish the previous step, then store this one.
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (lane == 0 && q >= 1) __hip_atomic_store(prog + w, q - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (act) {
+            const size_t e = own_e(q);
+#pragma unroll
+            for (int m = 0; m < NV; m++) {
+                double* dst = pk + (size_t)(F::DU + m) * P + e;
+                if (pub_lane) st_coherent(dst, oDU[m]);
+                else *dst = oDU[m];
             }
             LuSlot<NV>& me = slot[buf][lane];
 #pragma unroll
             for (int m = 0; m < NV; m++) { me.U[m] = own.U[m]; me.DU[m] = oDU[m]; }
             me.Lc[0] = own.Lc[1]; me.Lc[1] = own.Lc[2]; me.Lv[0] = own.Lv[1]; me.Lv[1] = own.Lv[2];
             me.S[0] = own.S[1]; me.S[1] = own.S[2];
-            // this cell is the i neighbour of the lane's next cell
 #pragma unroll
             for (int m = 0; m < NV; m++) { pU[m] = own.U[m]; pDU[m] = oDU[m]; }
             pLc = own.Lc[0]; pLv = own.Lv[0]; pS = own.S[0];
         }
-        // publish: the edge stores of this wave complete before the counter moves
-        if ((q + 1) % PUB == 0 || q + 1 == nsteps) {
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            __builtin_amdgcn_s_waitcnt(0);
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            if (lane == 0) __hip_atomic_store(prog + w, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (q + 1 < nsteps && deps && lane == 0) wait_for(q + 1);
+        if (look) { known_j = max(known_j, got_j); known_k = max(known_k, got_k); }
+        if (deps && lane == 0 && q + 2 < nsteps) wait_for(q + 2);
         __syncthreads();
         if (bail) return;
-        if (act1) {
-            const size_t s1 = IX(b, i1, j, k);
-            if (rem_j) lu_load_remote<NV, FWD>(b, FWD ? s1 - sj : s1 + sj, 1, rj);
-            if (rem_k) lu_load_remote<NV, FWD>(b, FWD ? s1 - sk : s1 + sk, 2, rk);
-            own = nxt;
-        }
+        own = nxt;
+        rj = rjn;
+        rk = rkn;
     }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (lane == 0) __hip_atomic_store(prog + w, nsteps - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------------------------
 // host side
 
 // Tiles of every block for one sweep direction, numbered so that upstream tiles come first.
-std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward) {
+// The sweep-ordered storage of a tile does not depend on the direction: tiles are laid out block
+// after block in forward order, each taking (nsteps x 64) elements.
+std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, long long* total) {
     std::vector<LuTask> out;
+    long long pbase = 0;
     for (size_t bi = 0; bi < blks.size(); bi++) {
         const Blk& b = blks[bi];
         int nj = b.ny - 1, nk = b.nz - 1;
@@ -434,41 +539,62 @@ std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward) {
         int base = (int)out.size();
         auto size_j = [&](int J) { return std::min(LU_TJ, nj - J * LU_TJ); };
         auto size_k = [&](int K) { return std::min(LU_TK, nk - K * LU_TK); };
+        // storage offsets in forward tile order
+        std::vector<long long> pofs((size_t)NJt * NKt);
+        for (int K = 0; K < NKt; K++) for (int J = 0; J < NJt; J++) {
+            pofs[J + NJt * K] = pbase;
+            pbase += (long long)((b.nx - 1) + size_j(J) + size_k(K) - 2) * LU_NT;
+        }
+        auto P = [&](int J, int K) { return (int)pofs[J + NJt * K]; };
         // position of tile (J,K) in this block's list
         auto idx = [&](int J, int K) { return forward ? J + NJt * K : (NJt - 1 - J) + NJt * (NKt - 1 - K); };
         std::vector<LuTask> blk_tasks((size_t)NJt * NKt);
         for (int K = 0; K < NKt; K++) for (int J = 0; J < NJt; J++) {
             LuTask t{};
             t.blk = (int)bi; t.j0 = 1 + J * LU_TJ; t.k0 = 1 + K * LU_TK; t.tj = size_j(J); t.tk = size_k(K);
+            t.pofs = P(J, K);
             int Ju = forward ? J - 1 : J + 1, Ku = forward ? K - 1 : K + 1;
             t.dep_j = t.dep_k = -1;
             if (Ju >= 0 && Ju < NJt) {
                 t.dep_j = base + idx(Ju, K);
                 t.off_j = size_j(Ju) - 1;
                 t.last_j = (b.nx - 1) + size_j(Ju) + t.tk - 3;
+                t.pofs_j = P(Ju, K);
+                t.tj_up = size_j(Ju);
             }
             if (Ku >= 0 && Ku < NKt) {
                 t.dep_k = base + idx(J, Ku);
                 t.off_k = size_k(Ku) - 1;
                 t.last_k = (b.nx - 1) + t.tj + size_k(Ku) - 3;
+                t.pofs_k = P(J, Ku);
+                t.tk_up = size_k(Ku);
             }
             blk_tasks[idx(J, K)] = t;
         }
         out.insert(out.end(), blk_tasks.begin(), blk_tasks.end());
     }
+    if (total) *total = pbase;
     return out;
 }
 
 template <int NV>
 static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st) {
     if (!fenced) {
-        constexpr int PUB = 1;
-        if (forward)
-            hipLaunchKernelGGL((k_lusgs_pipe<NV, true, PUB>), dim3((unsigned)L.n_fwd), dim3(LU_NT), 0, st,
-                               L.blks, L.fwd, L.prog_fwd, L.prog_bwd, L.n_bwd, L.err, p);
-        else
-            hipLaunchKernelGGL((k_lusgs_pipe<NV, false, PUB>), dim3((unsigned)L.n_bwd), dim3(LU_NT), 0, st,
-                               L.blks, L.bwd, L.prog_bwd, L.prog_fwd, L.n_fwd, L.err, p);
+        using F = Pk<NV>;
+        dim3 tgrid((unsigned)L.n_fwd, (unsigned)L.nwin);
+        if (forward) {
+            // this iteration's inputs into sweep order: U, Res, Lc, Lv, dt
+            hipLaunchKernelGGL((k_lu_transpose<NV, true>), tgrid, dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P,
+                               F::U, F::DU, F::LC, F::VOL);
+            hipLaunchKernelGGL((k_lusgs_pipe<NV, true>), dim3((unsigned)L.n_fwd), dim3(LU_NT), 0, st,
+                               L.blks, L.fwd, L.pk, L.P, L.prog_fwd, L.prog_bwd, L.n_bwd, L.err, p);
+        } else {
+            hipLaunchKernelGGL((k_lusgs_pipe<NV, false>), dim3((unsigned)L.n_bwd), dim3(LU_NT), 0, st,
+                               L.blks, L.bwd, L.pk, L.P, L.prog_bwd, L.prog_fwd, L.n_fwd, L.err, p);
+            // DU back into the block layout
+            hipLaunchKernelGGL((k_lu_transpose<NV, false>), tgrid, dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P,
+                               F::DU, F::LC, F::LC, F::LC);
+        }
         return;
     }
     if (forward)
@@ -483,6 +609,23 @@ void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fe
     if (p.NV == 5) sweep_nv<5>(L, p, forward, fenced, st);
     else if (p.NV == 6) sweep_nv<6>(L, p, forward, fenced, st);
     else sweep_nv<7>(L, p, forward, fenced, st);
+}
+
+int lusgs_fields(int NV) { return 3 * NV + 20; }
+int lusgs_windows(int max_cells_i) { return (max_cells_i + PK_W - 1) / PK_W; }
+
+// the geometry the sweeps read (vol, S, n), into sweep order once per plan
+template <int NV>
+static void pack_geometry_nv(const LuPlan& L, hipStream_t st) {
+    using F = Pk<NV>;
+    hipLaunchKernelGGL((k_lu_transpose<NV, true>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin), dim3(PK_T), 0, st,
+                       L.blks, L.fwd, L.pk, L.P, F::VOL, F::NF, F::NF, F::NF);
+}
+
+void launch_lusgs_pack_geometry(const LuPlan& L, int NV, hipStream_t st) {
+    if (NV == 5) pack_geometry_nv<5>(L, st);
+    else if (NV == 6) pack_geometry_nv<6>(L, st);
+    else pack_geometry_nv<7>(L, st);
 }
 
 }  // namespace ocfd

@@ -307,7 +307,8 @@ int build_lu_plan(ocfd_ctx* c) {
     if (c->lu.fwd) return OCFD_OK;
     std::vector<Blk> blks;
     for (HostBlock& hb : c->blocks) blks.push_back(hb.d);
-    std::vector<LuTask> f = lusgs_tasks(blks, true), b = lusgs_tasks(blks, false);
+    long long P = 0;
+    std::vector<LuTask> f = lusgs_tasks(blks, true, &P), b = lusgs_tasks(blks, false, nullptr);
     auto put = [&](const void* src, size_t bytes, void** out) -> int {
         void* q = nullptr;
         HIPCHK(hipMalloc(&q, bytes));
@@ -326,6 +327,20 @@ int build_lu_plan(ocfd_ctx* c) {
     c->lu.blks = (const Blk*)pb; c->lu.fwd = (const LuTask*)pf; c->lu.bwd = (const LuTask*)pw;
     c->lu.n_fwd = (int)f.size(); c->lu.n_bwd = (int)b.size();
     c->lu.prog_fwd = (int*)qf; c->lu.prog_bwd = (int*)qb; c->lu.err = (int*)pe;
+    // sweep-ordered storage and the geometry in it
+    void* pk = nullptr;
+    size_t bytes = (size_t)lusgs_fields(c->p.NV) * (size_t)P * sizeof(double);
+    HIPCHK(hipMalloc(&pk, bytes));
+    c->lu_allocs.push_back(pk);
+    HIPCHK(hipMemset(pk, 0, bytes));
+    c->lu.pk = (double*)pk;
+    c->lu.P = P;
+    int maxnx = 0;
+    for (const Blk& q : blks) maxnx = std::max(maxnx, q.nx - 1);
+    c->lu.nwin = lusgs_windows(maxnx);
+    launch_lusgs_pack_geometry(c->lu, c->p.NV, c->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
     return OCFD_OK;
 }
 
@@ -348,6 +363,13 @@ int ready(ocfd_ctx* c) {
     if (c->blocks.empty()) return fail(OCFD_EARG, "no blocks added");
     for (size_t q = 0; q < c->blocks.size(); q++)
         if (c->blocks[q].no != (int)q + 1) return fail(OCFD_EARG, "block " + std::to_string(q + 1) + " missing");
+    return OCFD_OK;
+}
+
+// ready() plus the tables the iteration's launches read (built outside any graph capture)
+int prepare(ocfd_ctx* c) {
+    int r = ready(c);
+    if (r) return r;
     if (c->lu_schedule != OCFD_LU_HYPERPLANE) return build_lu_plan(c);
     return OCFD_OK;
 }
@@ -620,6 +642,8 @@ int ocfd_setup_geometry(ocfd_ctx* c) {
     }
     for (HostBlock& hb : c->blocks) launch_mesh_quality(hb.d, c->st);
     for (HostBlock& hb : c->blocks) for (int q = 0; q < 3; q++) hb.nodes[q].clear();
+    drop_graphs(c);
+    drop_lu_plan(c);         // the sweep-ordered geometry is taken from the new arrays
     return finish(c);
 }
 
@@ -701,7 +725,7 @@ int ocfd_init_flow(ocfd_ctx* c) {
 }
 
 int ocfd_step_lusgs(ocfd_ctx* c, int32_t niter) {
-    int r = ready(c);
+    int r = prepare(c);
     if (r) return r;
     for (HostBlock& hb : c->blocks) if ((r = check_subs(c, hb))) return r;
     for (int it = 0; it < niter; it++) {
@@ -713,7 +737,7 @@ int ocfd_step_lusgs(ocfd_ctx* c, int32_t niter) {
 }
 
 int ocfd_step_timed(ocfd_ctx* c, int32_t niter, double* stage_ms) {
-    int r = ready(c);
+    int r = prepare(c);
     if (r) return r;
     if (!stage_ms || niter < 1) return fail(OCFD_EARG, "null stage_ms or niter < 1");
     for (HostBlock& hb : c->blocks) if ((r = check_subs(c, hb))) return r;
@@ -746,7 +770,7 @@ int ocfd_step_timed(ocfd_ctx* c, int32_t niter, double* stage_ms) {
     return OCFD_OK;
 }
 
-int ocfd_residual(ocfd_ctx* c) { int r = ready(c); if (r) return r; enqueue_residual(c); return finish(c); }
+int ocfd_residual(ocfd_ctx* c) { int r = prepare(c); if (r) return r; enqueue_residual(c); return finish(c); }
 int ocfd_update_limit(ocfd_ctx* c) { int r = ready(c); if (r) return r; enqueue_stage(c, OCFD_STAGE_UPDATE); return finish(c); }
 int ocfd_bc(ocfd_ctx* c) {
     int r = ready(c); if (r) return r;

@@ -49,6 +49,86 @@ struct Sub {
     int L1, L2, L3;
 };
 
+// ---------------------------------------------------------------------------------------------
+// Block sets: every block of a context (of every case it holds) goes through the same launch.
+// A launch over an iteration space covers the tiles of all blocks; tile t belongs to block
+// tile_blk[t] and is that block's (t - tile0[block])-th tile of 64 points along i.
+
+enum Space { SP_BOX = 0, SP_CELL, SP_FI, SP_FJ, SP_FK, SP_N0, SP_COUNT };
+
+// extent and origin of an iteration space of one block
+__device__ __host__ __forceinline__ void space_dims(int sp, const Blk& b, int& n0, int& n1, int& n2, int& o0, int& o1, int& o2) {
+    switch (sp) {
+    case SP_BOX: n0 = b.NI; n1 = b.NJ; n2 = b.NK; o0 = o1 = o2 = -3; break;
+    case SP_CELL: n0 = b.nx - 1; n1 = b.ny - 1; n2 = b.nz - 1; o0 = o1 = o2 = 1; break;
+    case SP_FI: n0 = b.nx; n1 = b.ny - 1; n2 = b.nz - 1; o0 = o1 = o2 = 1; break;
+    case SP_FJ: n0 = b.nx - 1; n1 = b.ny; n2 = b.nz - 1; o0 = o1 = o2 = 1; break;
+    case SP_FK: n0 = b.nx - 1; n1 = b.ny - 1; n2 = b.nz; o0 = o1 = o2 = 1; break;
+    default: n0 = b.nx + 1; n1 = b.ny + 1; n2 = b.nz + 1; o0 = o1 = o2 = 0; break;
+    }
+}
+
+struct BSet {
+    const Blk* blk;
+    const Par* par;
+    const int* tile_blk;     // for the launch's space
+    const int* tile0;
+};
+
+template <int SP>
+__device__ __forceinline__ bool locate(const BSet& S, int& bi, int& i, int& j, int& k) {
+    const int t = (int)blockIdx.x;
+    bi = S.tile_blk[t];
+    int n0, n1, n2, o0, o1, o2;
+    space_dims(SP, S.blk[bi], n0, n1, n2, o0, o1, o2);
+    const int lt = t - S.tile0[bi], T0 = (n0 + 63) / 64;
+    const int a = (lt % T0) * 64 + (int)threadIdx.x, r = lt / T0;
+    i = o0 + a;
+    j = o1 + r % n1;
+    k = o2 + r / n1;
+    (void)n2;
+    return a < n0;
+}
+
+#define OCFD_LOCATE(SP)                                  \
+    int bi_, i, j, k;                                    \
+    if (!locate<(SP)>(S, bi_, i, j, k)) return;          \
+    const Blk& b = S.blk[bi_];                           \
+    const Par& p = S.par[bi_];
+
+// one subface of one block in a subface launch: n0 x n1 points, tiles of 64 along n0
+enum SubKind { SUB_WALL = 0, SUB_FAR, SUB_SYM, SUB_EXT, SUB_AMUT, SUB_KBC };
+struct SubTask {
+    int blk, kind, arg, n0, n1, tile0;
+    Sub B;
+};
+struct SubSet {
+    const Blk* blk;
+    const Par* par;
+    const SubTask* t;
+    const int* tile_task;
+};
+
+#define OCFD_SUB_LOCATE                                                      \
+    const SubTask& T_ = S.t[S.tile_task[blockIdx.x]];                        \
+    const Blk& b = S.blk[T_.blk];                                            \
+    const Par& p = S.par[T_.blk];                                            \
+    const Sub& B = T_.B;                                                     \
+    const int lt_ = (int)blockIdx.x - T_.tile0, T0_ = (T_.n0 + 63) / 64;    \
+    const int a = (lt_ % T0_) * 64 + (int)threadIdx.x, c = lt_ / T0_;
+
+// block-to-block ghost copy of one inner subface (Umessage_send_mpi's index map)
+struct HaloMap { int kb[3], kb1[3], ke1[3], L[3], P[3], ks[3]; };
+struct HaloTask {
+    int src, dst, n, tile0;      // n points, tiles of 256
+    HaloMap h;
+};
+struct HaloSet {
+    const Blk* blk;
+    const HaloTask* t;
+    const int* tile_task;
+};
+
 // one tile of the pipelined LU-SGS sweep: cells j0..j0+tj-1, k0..k0+tk-1 of block `blk`, all i
 struct LuTask {
     int blk, j0, k0, tj, tk;

@@ -11,10 +11,9 @@ namespace ocfd {
 
 // ---------------------------------------------------------------------------------------------
 // comput_duvtpc (sub_Residual.f90:1417-1456): primitives on the whole LAP box
-__global__ void k_prims(Blk b, Par p) {
-    size_t n = (size_t)b.NI * b.NJ * b.NK;
-    size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+__global__ void __launch_bounds__(64) k_prims(BSet S) {
+    OCFD_LOCATE(SP_BOX)
+    size_t s = IX(b, i, j, k);
     double p00 = 1.0 / (p.gamma * p.Ma * p.Ma);
     double d = b.U[0][s];
     double u = b.U[1][s] / d, v = b.U[2][s] / d, w = b.U[3][s] / d;
@@ -25,9 +24,8 @@ __global__ void k_prims(Blk b, Par p) {
 
 // get_viscous (sub_Residual.f90:1355-1383); muf keeps the value left after the turbulence
 // model's mu*Re ... mu/Re round trip (sub_turbulence_SA.f90:41,262)
-__global__ void k_visc(Blk b, Par p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > b.nx - 1) return;
+__global__ void __launch_bounds__(64) k_visc(BSet S) {
+    OCFD_LOCATE(SP_CELL)
     size_t s = IX(b, i, j, k);
     double Tsb = 110.4 / p.T_inf, T = b.prim[5][s];
     double mu = ((1.0 / p.Re) * (1.0 + Tsb)) * sqrt((T * T) * T) / (Tsb + T);
@@ -35,8 +33,9 @@ __global__ void k_visc(Blk b, Par p) {
     b.muf[s] = (p.turb == 2 || p.turb == 3) ? (mu * p.Re) / p.Re : mu;
 }
 
-__global__ void k_visc_ghost(Blk b) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x + 1, c = blockIdx.y + 1, f = blockIdx.z;
+__global__ void k_visc_ghost(BSet S) {
+    const Blk& b = S.blk[blockIdx.z / 6];
+    int a = blockIdx.x * blockDim.x + threadIdx.x + 1, c = blockIdx.y + 1, f = blockIdx.z % 6;
     int nx = b.nx, ny = b.ny, nz = b.nz;
     size_t d, s;
     if (f < 2) { if (a > ny - 1 || c > nz - 1) return; d = IX(b, f == 0 ? 0 : nx, a, c); s = IX(b, f == 0 ? 1 : nx - 1, a, c); }
@@ -54,9 +53,8 @@ __device__ __forceinline__ double pw3(double x) { return (x * x) * x; }
 __device__ __forceinline__ double pw6(double x) { return ((((x * x) * x) * x) * x) * x; }   // flang expansion order
 
 // mu_t over 0..n with both limit_mut passes (sub_Residual.f90:1387-1408) folded in
-__global__ void k_sa_mut(Blk b, Par p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-    if (i > b.nx) return;
+__global__ void __launch_bounds__(64) k_sa_mut(BSet S) {
+    OCFD_LOCATE(SP_N0)
     size_t s = IX(b, i, j, k);
     const double Cv1_3 = pw3(7.1);
     double mu_sa = b.mu[s] * p.Re;
@@ -168,8 +166,7 @@ __device__ __forceinline__ double sa_residual(const Blk& b, const Par& p, int i,
 }
 
 // Amut_boundary (sub_turbulence_SA.f90:282-368) on one wall subface
-__global__ void k_amut(Blk b, Sub B) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+__device__ __forceinline__ void d_amut(const Blk& b, const Sub& B, int a, int c) {
     switch (B.face) {
     case 1: case 4: { int j = B.jb + a, k = B.kb + c; if (j > B.je - 1 || k > B.ke - 1) return;
         int ig = B.face == 1 ? 0 : B.ie, ii = B.face == 1 ? 1 : B.ie - 1; b.mut[IX(b, ig, j, k)] = -b.mut[IX(b, ii, j, k)]; break; }
@@ -187,9 +184,8 @@ __device__ __forceinline__ double kt_of(const Blk& b, size_t s) { double v = b.U
 __device__ __forceinline__ double wt_of(const Blk& b, size_t s) { double v = b.U[6][s]; v = v < 0.0 ? 0.0 : v; return v / b.U[0][s]; }
 
 // cell pass: mu_t, f1, sources into Res(6:7)
-__global__ void k_sst_cell(Blk b, Par p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > b.nx - 1) return;
+__global__ void __launch_bounds__(64) k_sst_cell(BSet S) {
+    OCFD_LOCATE(SP_CELL)
     const double sigma_w2 = 0.856, beta1 = 0.075, Cw1s = 0.533, beta2 = 0.0828, Cw2s = 0.440;
     const double a1 = 0.31, betas = 0.09, c009 = (double)0.09f;
     const double Re = p.Re;
@@ -237,7 +233,8 @@ __global__ void k_sst_cell(Blk b, Par p) {
 
 // mu_t ghost copies over the whole -1:n+1 extent (sub_turbulence_SST.f90:170-175): face f in
 // order i-, i+, j-, j+, k-, k+ launched one after the other
-__global__ void k_sst_ghost(Blk b, int f) {
+__global__ void k_sst_ghost(BSet S, int f) {
+    const Blk& b = S.blk[blockIdx.z];
     int a = blockIdx.x * blockDim.x + threadIdx.x - 1, c = blockIdx.y - 1;
     int nx = b.nx, ny = b.ny, nz = b.nz;
     if (f < 2) { if (a > ny + 1 || c > nz + 1) return; b.mut[IX(b, f == 0 ? 0 : nx, a, c)] = b.mut[IX(b, f == 0 ? 1 : nx - 1, a, c)]; }
@@ -246,9 +243,8 @@ __global__ void k_sst_ghost(Blk b, int f) {
 }
 
 // k and omega face fluxes, accumulated into Res(6:7) per direction in i, j, k order
-__global__ void k_sst_res(Blk b, Par p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > b.nx - 1) return;
+__global__ void __launch_bounds__(64) k_sst_res(BSet S) {
+    OCFD_LOCATE(SP_CELL)
     const double sigma_k1 = 0.85, sigma_w1 = 0.5, sigma_k2 = 1.0, sigma_w2 = 0.856;
     const double Re = p.Re;
     const double *uu = b.prim[1], *vv = b.prim[2], *ww = b.prim[3];
@@ -310,9 +306,8 @@ __global__ void k_sst_res(Blk b, Par p) {
 }
 
 // mu_t/Re and the limit_mut pass of Residual (sub_turbulence_SST.f90:333-340; sub_Residual.f90:1387-1408)
-__global__ void k_sst_finish(Blk b, Par p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-    if (i > b.nx) return;
+__global__ void __launch_bounds__(64) k_sst_finish(BSet S) {
+    OCFD_LOCATE(SP_N0)
     size_t s = IX(b, i, j, k);
     double m2 = b.mut[s] / p.Re;
     double muf = (b.mu[s] * p.Re) / p.Re;
@@ -324,11 +319,8 @@ __global__ void k_sst_finish(Blk b, Par p) {
 // ---------------------------------------------------------------------------------------------
 // flux_inviscous_i/j/k (sub_Residual.f90:182-705): one thread per face, face flux stored
 template <int DIR>
-__global__ void k_inv_flux(Blk b, Par p, int overlimit_host) {
-    constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
-    int ni = b.nx - 1 + di, nj = b.ny - 1 + dj;
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > ni || j > nj) return;
+__global__ void __launch_bounds__(64) k_inv_flux(BSet S) {
+    OCFD_LOCATE(SP_FI + DIR)
     size_t s = IX(b, i, j, k);
     int pos = DIR == 0 ? i : DIR == 1 ? j : k;
     int nd = DIR == 0 ? b.nx : DIR == 1 ? b.ny : b.nz;
@@ -337,7 +329,6 @@ __global__ void k_inv_flux(Blk b, Par p, int overlimit_host) {
     if (fl & 1) { if (pos == 1) scheme = S_CD2; else if (pos < LAP) scheme = p.bscheme; }
     if (fl & 2) { if (pos == nd) scheme = S_CD2; else if (pos > nd - LAP + 1) scheme = p.bscheme; }
     if (b.flags[0] == 1) scheme = S_UD1;
-    (void)overlimit_host;
     double s1x = b.nv[3 * DIR][s], s1y = b.nv[3 * DIR + 1][s], s1z = b.nv[3 * DIR + 2][s];
     double t1x, t1y, t1z;
     if (DIR == 0) { size_t a = IX(b, i, j + 1, k), c = IX(b, i, j, k + 1); t1x = b.x[a] - b.x[c]; t1y = b.y[a] - b.y[c]; t1z = b.z[a] - b.z[c]; }
@@ -425,8 +416,9 @@ __global__ void k_inv_flux(Blk b, Par p, int overlimit_host) {
 }
 
 // explicit average-state flux on a physical k subface (sub_Residual.f90:667-688)
-__global__ void k_inv_kbc(Blk b, Par p, Sub B) {
-    int i = B.ib + blockIdx.x * blockDim.x + threadIdx.x, j = B.jb + blockIdx.y, k = B.kb;
+__global__ void __launch_bounds__(64) k_inv_kbc(SubSet S) {
+    OCFD_SUB_LOCATE
+    int i = B.ib + a, j = B.jb + c, k = B.kb;
     if (i > B.ie - 1 || j > B.je - 1) return;
     size_t s = IX(b, i, j, k), t = IX(b, i, j, k - 1);
     double g = p.gamma;
@@ -445,11 +437,9 @@ __global__ void k_inv_kbc(Blk b, Par p, Sub B) {
 
 // flux_viscous_i/j/k (sub_Residual.f90:712-1247)
 template <int DIR>
-__global__ void k_visc_flux(Blk b, Par p) {
+__global__ void __launch_bounds__(64) k_visc_flux(BSet S) {
     constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
-    int ni = b.nx - 1 + di, nj = b.ny - 1 + dj;
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > ni || j > nj) return;
+    OCFD_LOCATE(SP_FI + DIR)
     const double tmp1 = 4.0 / 3.0, tmp2 = 2.0 / 3.0;
     size_t s = IX(b, i, j, k);
     if (b.flags[0] == 1) {      // over-limit block: no viscous flux, zero wall forces (:734-750)
@@ -527,9 +517,8 @@ __global__ void k_visc_flux(Blk b, Par p) {
 // Res(1:5): inviscid i, j, k then viscous i, j, k face differences, in the reference order
 // (sub_Residual.f90:346-354,507-516,692-701,864-872,1039-1048,1219-1229)
 // followed by Res(6) of SA, then comput_Lijk + comput_dt (sub_time_acceleraction.f90:7-112)
-__global__ void k_cell(Blk b, Par p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > b.nx - 1) return;
+__global__ void __launch_bounds__(64) k_cell(BSet S) {
+    OCFD_LOCATE(SP_CELL)
     size_t s = IX(b, i, j, k);
     size_t sp[3] = {s + 1, s + (size_t)b.NI, s + (size_t)b.NI * b.NJ};
     bool visc = p.visc == 1 && b.flags[0] != 1;
@@ -650,9 +639,8 @@ __global__ void k_lusgs_plane(Blk b, Par p, int plane) {
 // ---------------------------------------------------------------------------------------------
 // U = Un + DU (sub_time_advance.f90:92-101) and the limit_flow_U snapshot/detection
 // (sub_limitflow.f90:46-94).  The snapshot (rho,u,v,w,p) goes to prim[0..4].
-__global__ void k_update_detect(Blk b, Par p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > b.nx - 1) return;
+__global__ void __launch_bounds__(64) k_update_detect(BSet S) {
+    OCFD_LOCATE(SP_CELL)
     size_t s = IX(b, i, j, k);
     double U[MAXV];
     for (int m = 0; m < p.NV; m++) { U[m] = b.U[m][s] + b.DU[m][s]; b.U[m][s] = U[m]; }
@@ -671,10 +659,9 @@ __device__ __forceinline__ bool lim_bad(const Blk& b, const Par& p, size_t s) {
 
 // limit_flow_U replacement step (sub_limitflow.f90:84-150): bad cells take the mean of the valid
 // 3x3x3 neighbours of the snapshot, or their own clamped values; sets the IF_OverLimit latch
-__global__ void k_limit_apply(Blk b, Par p) {
+__global__ void __launch_bounds__(64) k_limit_apply(BSet S) {
+    OCFD_LOCATE(SP_CELL)
     if (b.flags[1] == 0) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > b.nx - 1) return;
     if (i == 1 && j == 1 && k == 1) b.flags[0] = 1;
     size_t s = IX(b, i, j, k);
     if (!lim_bad(b, p, s)) return;
@@ -709,9 +696,8 @@ __global__ void k_limit_apply(Blk b, Par p) {
 // residual); when some cell exceeds LSAmax, one work-group replays the reference loop over the
 // flagged cells in index order: an earlier neighbour contributes its final value, a later one
 // its original value, exactly as in the sequential reference.
-__global__ void k_limit_sa_detect(Blk b, Par p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > b.nx - 1) return;
+__global__ void __launch_bounds__(64) k_limit_sa_detect(BSet S) {
+    OCFD_LOCATE(SP_CELL)
     size_t s = IX(b, i, j, k);
     double v = b.U[5][s];
     b.prim[5][s] = v;
@@ -719,7 +705,9 @@ __global__ void k_limit_sa_detect(Blk b, Par p) {
     else if (v < 1.0e-8) b.U[5][s] = 1.0e-8;
 }
 
-__global__ void k_limit_sa_serial(Blk b, Par p) {
+__global__ void k_limit_sa_serial(BSet S) {
+    const Blk& b = S.blk[blockIdx.x];
+    const Par& p = S.par[blockIdx.x];
     if (b.flags[2] == 0) return;
     const double SAmin = 1.0e-8;
     double* U6 = b.U[5];
@@ -758,22 +746,21 @@ __global__ void k_limit_sa_serial(Blk b, Par p) {
     }
 }
 
-__global__ void k_reset_flags(Blk b) {
+__global__ void k_reset_flags(BSet S) {
+    const Blk& b = S.blk[blockIdx.x];
     if (threadIdx.x == 0) { b.flags[1] = 0; b.flags[2] = 0; }
 }
 
 // force_vt_kw (sub_time_advance.f90:27-61)
-__global__ void k_force_vt_kw(Blk b, Par p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > b.nx - 1) return;
+__global__ void __launch_bounds__(64) k_force_vt_kw(BSet S) {
+    OCFD_LOCATE(SP_CELL)
     size_t s = IX(b, i, j, k);
     for (int m = 5; m < p.NV; m++) if (b.U[m][s] < 0) b.U[m][s] = 1.0e-10;
 }
 
 // ---------------------------------------------------------------------------------------------
 // Boundary conditions on one subface (sub_boundary.f90:6-504)
-__global__ void k_bc_wall(Blk b, Par p, Sub B) {   // boundary_wall + wall_bound (:87-147, :405-454)
-    int a = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+__device__ __forceinline__ void d_bc_wall(const Blk& b, const Par& p, const Sub& B, int a, int c) {   // boundary_wall + wall_bound (:87-147, :405-454)
     size_t s1, sg;
     switch (B.face) {
     case 1: case 4: { int j = B.jb + a, k = B.kb + c; if (j > B.je - 1 || k > B.ke - 1) return;
@@ -811,8 +798,7 @@ __global__ void k_bc_wall(Blk b, Par p, Sub B) {   // boundary_wall + wall_bound
 
 // boundary_Farfield (:152-283); flag 0 far field, 1 forced outlet, 2 forced inlet.  The node
 // range of the subface is inclusive, as in the reference loop.
-__global__ void k_bc_far(Blk b, Par p, Sub B, int flag) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+__device__ __forceinline__ void d_bc_far(const Blk& b, const Par& p, const Sub& B, int flag, int a, int c) {
     int i, j, k;
     switch (B.face) {
     case 1: case 4: j = B.jb + a; k = B.kb + c; i = B.ib; if (j > B.je || k > B.ke) return; break;
@@ -868,8 +854,7 @@ __global__ void k_bc_far(Blk b, Par p, Sub B, int flag) {
     if (p.NV == 7) b.U[6][s2] = nv7;
 }
 
-__global__ void k_bc_sym(Blk b, Par p, Sub B) {   // boundary_Symmetry_or_SlideWall (:291-373)
-    int a = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+__device__ __forceinline__ void d_bc_sym(const Blk& b, const Par& p, const Sub& B, int a, int c) {   // boundary_Symmetry_or_SlideWall (:291-373)
     size_t s1, s2, sn;
     int nb;
     switch (B.face) {
@@ -891,8 +876,7 @@ __global__ void k_bc_sym(Blk b, Par p, Sub B) {   // boundary_Symmetry_or_SlideW
     b.U[3][s2] = b.U[3][s2] - (2.0 * Vn) * n3;
 }
 
-__global__ void k_bc_ext(Blk b, Par p, Sub B) {   // boundary_Extrapolate (:458-504)
-    int a = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+__device__ __forceinline__ void d_bc_ext(const Blk& b, const Par& p, const Sub& B, int a, int c) {   // boundary_Extrapolate (:458-504)
     int i, j, k;
     switch (B.face) {
     case 1: case 4: j = B.jb + a; k = B.kb + c; i = B.ib; if (j > B.je || k > B.ke) return; break;
@@ -907,13 +891,29 @@ __global__ void k_bc_ext(Blk b, Par p, Sub B) {   // boundary_Extrapolate (:458-
     for (int m = 0; m < p.NV; m++) b.U[m][IX(b, i2, j2, k2)] = b.U[m][IX(b, i1, j1, k1)];
 }
 
+// every physical subface of the set in one launch; the ghost cells written by different
+// subfaces are disjoint and only interior cells are read
+__global__ void __launch_bounds__(64) k_bc(SubSet S) {
+    OCFD_SUB_LOCATE
+    switch (T_.kind) {
+    case SUB_WALL: d_bc_wall(b, p, B, a, c); break;
+    case SUB_FAR: d_bc_far(b, p, B, T_.arg, a, c); break;
+    case SUB_SYM: d_bc_sym(b, p, B, a, c); break;
+    default: d_bc_ext(b, p, B, a, c); break;
+    }
+}
+
+// Amut_boundary on every wall subface of the set
+__global__ void __launch_bounds__(64) k_amut(SubSet S) {
+    OCFD_SUB_LOCATE
+    (void)p;
+    d_amut(b, B, a, c);
+}
+
 // Umessage_send_mpi local copy (sub_update_buffer_mpi.f90:42-141): LAP ghost layers of the
 // neighbour block from this block's interior, with the L1..L3 axis permutation and order.
-struct HaloMap { int kb[3], kb1[3], ke1[3], L[3], P[3], ks[3]; };
-
-__global__ void k_halo(Blk src, Blk dst, HaloMap h, int NV) {
+__device__ __forceinline__ void d_halo(const Blk& src, const Blk& dst, const HaloMap& h, int NV, int t) {
     int n0 = h.ke1[0] - h.kb1[0] + 1, n1 = h.ke1[1] - h.kb1[1] + 1, n2 = h.ke1[2] - h.kb1[2] + 1;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n0 * n1 * n2) return;
     int ka[3] = {t % n0, (t / n0) % n1, t / (n0 * n1)};
     int i = h.kb1[0] + ka[0], j = h.kb1[1] + ka[1], k = h.kb1[2] + ka[2];
@@ -922,9 +922,19 @@ __global__ void k_halo(Blk src, Blk dst, HaloMap h, int NV) {
     for (int m = 0; m < NV; m++) dst.U[m][d] = src.U[m][s];
 }
 
+// single-pair launch, used while the geometry is set up
+__global__ void k_halo(Blk src, Blk dst, HaloMap h, int NV) {
+    d_halo(src, dst, h, NV, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// every inner subface of the set in one launch (the ghost regions written are disjoint)
+__global__ void __launch_bounds__(256) k_halo_set(HaloSet S, int NV) {
+    const HaloTask& T = S.t[S.tile_task[blockIdx.x]];
+    d_halo(S.blk[T.src], S.blk[T.dst], T.h, NV, (blockIdx.x - T.tile0) * 256 + threadIdx.x);
+}
+
 // get_U_conner (sub_update_buffer_mpi.f90:515-549): 12 edges, then 8 vertices
-__global__ void k_corner_edges(Blk b, int NV) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x, e = blockIdx.y, m = blockIdx.z;
+__device__ __forceinline__ void d_corner_edges(const Blk& b, int NV, int t, int e, int m) {
     int nx = b.nx, ny = b.ny, nz = b.nz;
     double* U = b.U[m];
     auto V = [&](int i, int j, int k) -> double& { return U[IX(b, i, j, k)]; };
@@ -946,8 +956,7 @@ __global__ void k_corner_edges(Blk b, int NV) {
     (void)NV;
 }
 
-__global__ void k_corner_vertices(Blk b) {
-    int c = threadIdx.x, m = blockIdx.x;
+__device__ __forceinline__ void d_corner_vertices(const Blk& b, int c, int m) {
     if (c >= 8) return;
     int nx = b.nx, ny = b.ny, nz = b.nz;
     double* U = b.U[m];
@@ -958,10 +967,24 @@ __global__ void k_corner_vertices(Blk b) {
     U[IX(b, I, J, K)] = r;
 }
 
+// edges then vertices of every block of the set (blockIdx.z: block x variable)
+__global__ void k_corner_edges(BSet S, int NV) {
+    d_corner_edges(S.blk[blockIdx.z / NV], NV, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z % NV);
+}
+__global__ void k_corner_vertices(BSet S, int NV) {
+    d_corner_vertices(S.blk[blockIdx.x / NV], threadIdx.x, blockIdx.x % NV);
+}
+// single-block forms, used while the geometry is set up
+__global__ void k_corner_edges1(Blk b, int NV) {
+    d_corner_edges(b, NV, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z);
+}
+__global__ void k_corner_vertices1(Blk b) { d_corner_vertices(b, threadIdx.x, blockIdx.x); }
+
 // ---------------------------------------------------------------------------------------------
 // comput_max_Res_onemesh (sub_time_advance.f90:330-380): per-block partial max and sum of
 // squares in a fixed tree order (deterministic, independent of co-scheduled work)
-__global__ void k_norms(Blk b, int NV, double* part) {
+__global__ void k_norms(BSet S, int NV, double* part) {
+    const Blk& b = S.blk[blockIdx.z];
     __shared__ double smax[256], ssum[256];
     __shared__ int snan[256];
     int m = blockIdx.y;
@@ -985,7 +1008,7 @@ __global__ void k_norms(Blk b, int NV, double* part) {
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        double* o = part + 3 * ((size_t)m * gridDim.x + blockIdx.x);
+        double* o = part + 3 * (((size_t)blockIdx.z * NV + m) * gridDim.x + blockIdx.x);
         o[0] = smax[0]; o[1] = ssum[0]; o[2] = snan[0];
     }
     (void)NV;
@@ -1027,76 +1050,90 @@ __global__ void k_force_sub(Blk b, Par p, Sub B, double* acc9) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// host launchers
+// host launchers.  Every launch covers all blocks of the set; the per-block conditions of the
+// reference (turbulence model, limiter, NVAR) are uniform over a set.
 static inline dim3 g3(int n0, int n1, int n2, int bs) { return dim3((unsigned)((n0 + bs - 1) / bs), (unsigned)n1, (unsigned)n2); }
+static inline BSet bset(const SetPlan& P, int sp) { return BSet{P.blk, P.par, P.tile_blk[sp], P.tile0[sp]}; }
 
-void launch_prims(const Blk& b, const Par& p, hipStream_t st) {
-    size_t n = (size_t)b.NI * b.NJ * b.NK;
-    hipLaunchKernelGGL(k_prims, GRID1(n, 256), 0, st, b, p);
+template <int SP>
+static void launch_space(void (*kern)(BSet), const SetPlan& P, hipStream_t st) {
+    if (P.ntiles[SP] > 0) hipLaunchKernelGGL(kern, dim3((unsigned)P.ntiles[SP]), dim3(64), 0, st, bset(P, SP));
 }
 
-void launch_viscosity(const Blk& b, const Par& p, hipStream_t st) {
-    hipLaunchKernelGGL(k_visc, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
-    int mx = std::max(b.nx, std::max(b.ny, b.nz));
-    hipLaunchKernelGGL(k_visc_ghost, dim3((unsigned)((mx + 63) / 64), (unsigned)mx, 6), dim3(64), 0, st, b);
+static void launch_subs(void (*kern)(SubSet), const SetPlan& P, const SubList& L, hipStream_t st) {
+    if (L.ntiles > 0) hipLaunchKernelGGL(kern, dim3((unsigned)L.ntiles), dim3(64), 0, st, SubSet{P.blk, P.par, L.t, L.tile_task});
 }
 
-void launch_sa(const Blk& b, const Par& p, const Sub* subs, int nsub, hipStream_t st) {
-    hipLaunchKernelGGL(k_sa_mut, g3(b.nx + 1, b.ny + 1, b.nz + 1, 64), dim3(64), 0, st, b, p);
-    for (int q = 0; q < nsub; q++) {
-        if (subs[q].bc != 2) continue;
-        int mx = std::max(b.nx, std::max(b.ny, b.nz));
-        hipLaunchKernelGGL(k_amut, dim3((unsigned)((mx + 63) / 64), (unsigned)mx), dim3(64), 0, st, b, subs[q]);
+void set_pre(const SetPlan& P, hipStream_t st) {
+    const Par& p = P.p0;
+    hipLaunchKernelGGL(k_reset_flags, dim3((unsigned)P.nb), dim3(64), 0, st, bset(P, SP_CELL));
+    launch_space<SP_BOX>(k_prims, P, st);
+    if (p.visc != 1) return;
+    launch_space<SP_CELL>(k_visc, P, st);
+    hipLaunchKernelGGL(k_visc_ghost, dim3((unsigned)((P.mx + 63) / 64), (unsigned)P.mx, (unsigned)(6 * P.nb)), dim3(64), 0, st, bset(P, SP_CELL));
+    if (p.turb == 2) {
+        launch_space<SP_N0>(k_sa_mut, P, st);
+        launch_subs(k_amut, P, P.amut, st);
+    } else if (p.turb == 3) {
+        launch_space<SP_CELL>(k_sst_cell, P, st);
+        int m3 = P.mx + 3;
+        for (int f = 0; f < 6; f++)
+            hipLaunchKernelGGL(k_sst_ghost, dim3((unsigned)((m3 + 63) / 64), (unsigned)m3, (unsigned)P.nb), dim3(64), 0, st, bset(P, SP_CELL), f);
+        launch_subs(k_amut, P, P.amut, st);
+        launch_space<SP_CELL>(k_sst_res, P, st);
+        launch_space<SP_N0>(k_sst_finish, P, st);
+        launch_subs(k_amut, P, P.amut, st);
+    } else {
+        launch_subs(k_amut, P, P.amut, st);
     }
 }
 
-void launch_sst(const Blk& b, const Par& p, const Sub* subs, int nsub, hipStream_t st) {
-    hipLaunchKernelGGL(k_sst_cell, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
-    int mx = std::max(b.nx, std::max(b.ny, b.nz)) + 3;
-    for (int f = 0; f < 6; f++)
-        hipLaunchKernelGGL(k_sst_ghost, dim3((unsigned)((mx + 63) / 64), (unsigned)mx), dim3(64), 0, st, b, f);
-    for (int q = 0; q < nsub; q++) {
-        if (subs[q].bc != 2) continue;
-        int m2 = std::max(b.nx, std::max(b.ny, b.nz));
-        hipLaunchKernelGGL(k_amut, dim3((unsigned)((m2 + 63) / 64), (unsigned)m2), dim3(64), 0, st, b, subs[q]);
+void set_inviscid(const SetPlan& P, hipStream_t st) {
+    launch_space<SP_FI>(k_inv_flux<0>, P, st);
+    launch_space<SP_FJ>(k_inv_flux<1>, P, st);
+    launch_space<SP_FK>(k_inv_flux<2>, P, st);
+    launch_subs(k_inv_kbc, P, P.kbc, st);
+}
+
+void set_viscous_dt(const SetPlan& P, hipStream_t st) {
+    if (P.p0.visc == 1) {
+        launch_space<SP_FI>(k_visc_flux<0>, P, st);
+        launch_space<SP_FJ>(k_visc_flux<1>, P, st);
+        launch_space<SP_FK>(k_visc_flux<2>, P, st);
     }
-    hipLaunchKernelGGL(k_sst_res, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
-    hipLaunchKernelGGL(k_sst_finish, g3(b.nx + 1, b.ny + 1, b.nz + 1, 64), dim3(64), 0, st, b, p);
-    for (int q = 0; q < nsub; q++) {
-        if (subs[q].bc != 2) continue;
-        int m2 = std::max(b.nx, std::max(b.ny, b.nz));
-        hipLaunchKernelGGL(k_amut, dim3((unsigned)((m2 + 63) / 64), (unsigned)m2), dim3(64), 0, st, b, subs[q]);
+    launch_space<SP_CELL>(k_cell, P, st);
+}
+
+void set_update_limit(const SetPlan& P, hipStream_t st) {
+    launch_space<SP_CELL>(k_update_detect, P, st);
+    if (P.p0.limit_flow != 1) return;
+    launch_space<SP_CELL>(k_limit_apply, P, st);
+    if (P.p0.NV == 6) {
+        launch_space<SP_CELL>(k_limit_sa_detect, P, st);
+        hipLaunchKernelGGL(k_limit_sa_serial, dim3((unsigned)P.nb), dim3(64), 0, st, bset(P, SP_CELL));
     }
 }
 
-void launch_laminar_amut(const Blk& b, const Sub* subs, int nsub, hipStream_t st) {
-    for (int q = 0; q < nsub; q++) {
-        if (subs[q].bc != 2) continue;
-        int mx = std::max(b.nx, std::max(b.ny, b.nz));
-        hipLaunchKernelGGL(k_amut, dim3((unsigned)((mx + 63) / 64), (unsigned)mx), dim3(64), 0, st, b, subs[q]);
-    }
+void set_bc(const SetPlan& P, hipStream_t st) { launch_subs(k_bc, P, P.bc, st); }
+
+void set_halo(const SetPlan& P, hipStream_t st) {
+    if (P.halo.ntiles > 0)
+        hipLaunchKernelGGL(k_halo_set, dim3((unsigned)P.halo.ntiles), dim3(256), 0, st, HaloSet{P.blk, P.halo.t, P.halo.tile_task}, P.p0.NV);
+    hipLaunchKernelGGL(k_corner_edges, dim3((unsigned)((P.mx + 63) / 64), 12, (unsigned)(P.p0.NV * P.nb)), dim3(64), 0, st, bset(P, SP_CELL), P.p0.NV);
+    hipLaunchKernelGGL(k_corner_vertices, dim3((unsigned)(P.p0.NV * P.nb)), dim3(64), 0, st, bset(P, SP_CELL), P.p0.NV);
 }
 
-void launch_inv_fluxes(const Blk& b, const Par& p, const Sub* subs, int nsub, hipStream_t st) {
-    hipLaunchKernelGGL(k_inv_flux<0>, g3(b.nx, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p, 0);
-    hipLaunchKernelGGL(k_inv_flux<1>, g3(b.nx - 1, b.ny, b.nz - 1, 64), dim3(64), 0, st, b, p, 0);
-    hipLaunchKernelGGL(k_inv_flux<2>, g3(b.nx - 1, b.ny - 1, b.nz, 64), dim3(64), 0, st, b, p, 0);
-    for (int q = 0; q < nsub; q++) {
-        const Sub& B = subs[q];
-        if (B.bc > 0 && (B.face == 3 || B.face == 6))
-            hipLaunchKernelGGL(k_inv_kbc, dim3((unsigned)((B.ie - B.ib + 63) / 64), (unsigned)std::max(1, B.je - B.jb)), dim3(64), 0, st, b, p, B);
-    }
+void set_force_vt_kw(const SetPlan& P, hipStream_t st) {
+    if (P.p0.NV >= 6) launch_space<SP_CELL>(k_force_vt_kw, P, st);
 }
 
-void launch_visc_assemble_dt(const Blk& b, const Par& p, hipStream_t st) {
-    if (p.visc == 1) {
-        hipLaunchKernelGGL(k_visc_flux<0>, g3(b.nx, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
-        hipLaunchKernelGGL(k_visc_flux<1>, g3(b.nx - 1, b.ny, b.nz - 1, 64), dim3(64), 0, st, b, p);
-        hipLaunchKernelGGL(k_visc_flux<2>, g3(b.nx - 1, b.ny - 1, b.nz, 64), dim3(64), 0, st, b, p);
-    }
-    hipLaunchKernelGGL(k_cell, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
+int norms_parts() { return 64; }
+
+void set_norms(const SetPlan& P, double* part, hipStream_t st) {
+    hipLaunchKernelGGL(k_norms, dim3((unsigned)norms_parts(), (unsigned)P.p0.NV, (unsigned)P.nb), dim3(256), 0, st, bset(P, SP_CELL), P.p0.NV, part);
 }
 
+// ---- per-block launches: the hyperplane LU-SGS schedule, the setup-time halo, forces ----
 template <int NV>
 static void lusgs_nv(const Blk& b, const Par& p, bool fwd, hipStream_t st) {
     int np = b.nx + b.ny + b.nz - 3;
@@ -1113,37 +1150,7 @@ void launch_lusgs_sweep(const Blk& b, const Par& p, bool forward, hipStream_t st
     else lusgs_nv<7>(b, p, forward, st);
 }
 
-void launch_reset_flags(const Blk& b, hipStream_t st) {
-    hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(64), 0, st, b);
-}
-
-void launch_update_limit(const Blk& b, const Par& p, hipStream_t st) {
-    hipLaunchKernelGGL(k_update_detect, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
-    if (p.limit_flow != 1) return;
-    hipLaunchKernelGGL(k_limit_apply, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
-    if (p.NV == 6) {
-        hipLaunchKernelGGL(k_limit_sa_detect, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
-        hipLaunchKernelGGL(k_limit_sa_serial, dim3(1), dim3(64), 0, st, b, p);
-    }
-}
-
-void launch_bc(const Blk& b, const Par& p, const Sub& B, hipStream_t st) {
-    int n0, n1;
-    switch (B.face) {
-    case 1: case 4: n0 = B.je - B.jb + 1; n1 = B.ke - B.kb + 1; break;
-    case 2: case 5: n0 = B.ie - B.ib + 1; n1 = B.ke - B.kb + 1; break;
-    default: n0 = B.ie - B.ib + 1; n1 = B.je - B.jb + 1; break;
-    }
-    dim3 grid((unsigned)((n0 + 63) / 64), (unsigned)n1), blk(64);
-    if (B.bc == 2 && p.visc == 1) hipLaunchKernelGGL(k_bc_wall, grid, blk, 0, st, b, p, B);
-    else if (B.bc == 4) hipLaunchKernelGGL(k_bc_far, grid, blk, 0, st, b, p, B, 0);
-    else if (B.bc == 5) hipLaunchKernelGGL(k_bc_far, grid, blk, 0, st, b, p, B, 2);
-    else if (B.bc == 6) hipLaunchKernelGGL(k_bc_far, grid, blk, 0, st, b, p, B, 1);
-    else if (B.bc == 3 || (B.bc == 2 && p.visc == 0)) hipLaunchKernelGGL(k_bc_sym, grid, blk, 0, st, b, p, B);
-    else if (B.bc == 401) hipLaunchKernelGGL(k_bc_ext, grid, blk, 0, st, b, p, B);
-}
-
-void launch_halo(const Blk& src, const Blk& dst, const Sub& B, int NV, hipStream_t st) {
+HaloMap halo_map(const Sub& B) {
     HaloMap h;
     int kb[3] = {B.ib, B.jb, B.kb}, ke[3] = {B.ie - 1, B.je - 1, B.ke - 1};
     int k = (B.face - 1) % 3;
@@ -1159,25 +1166,17 @@ void launch_halo(const Blk& src, const Blk& dst, const Sub& B, int NV, hipStream
         h.L[q] = std::abs(Ls[q]); h.P[q] = Ls[q] >= 0 ? 1 : -1;
         h.ks[q] = h.P[q] > 0 ? kb[q] : ke[q];
     }
-    int n = (ke1[0] - kb1[0] + 1) * (ke1[1] - kb1[1] + 1) * (ke1[2] - kb1[2] + 1);
+    return h;
+}
+
+int halo_points(const HaloMap& h) {
+    return (h.ke1[0] - h.kb1[0] + 1) * (h.ke1[1] - h.kb1[1] + 1) * (h.ke1[2] - h.kb1[2] + 1);
+}
+
+void launch_halo(const Blk& src, const Blk& dst, const Sub& B, int NV, hipStream_t st) {
+    HaloMap h = halo_map(B);
+    int n = halo_points(h);
     hipLaunchKernelGGL(k_halo, GRID1(n, 256), 0, st, src, dst, h, NV);
-}
-
-void launch_corners(const Blk& b, int NV, hipStream_t st) {
-    int mx = std::max(b.nx, std::max(b.ny, b.nz));
-    hipLaunchKernelGGL(k_corner_edges, dim3((unsigned)((mx + 63) / 64), 12, (unsigned)NV), dim3(64), 0, st, b, NV);
-    hipLaunchKernelGGL(k_corner_vertices, dim3((unsigned)NV), dim3(64), 0, st, b);
-}
-
-void launch_force_vt_kw(const Blk& b, const Par& p, hipStream_t st) {
-    if (p.NV < 6) return;
-    hipLaunchKernelGGL(k_force_vt_kw, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b, p);
-}
-
-int norms_parts() { return 64; }
-
-void launch_norms(const Blk& b, int NV, double* part, hipStream_t st) {
-    hipLaunchKernelGGL(k_norms, dim3((unsigned)norms_parts(), (unsigned)NV), dim3(256), 0, st, b, NV, part);
 }
 
 void launch_force_sub(const Blk& b, const Par& p, const Sub& B, double* acc12, hipStream_t st) {

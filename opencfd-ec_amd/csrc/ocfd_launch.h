@@ -5,28 +5,53 @@
 #include "ocfd_dev.h"
 
 namespace ocfd {
-void launch_reset_flags(const Blk& b, hipStream_t st);
-void launch_prims(const Blk& b, const Par& p, hipStream_t st);
-void launch_viscosity(const Blk& b, const Par& p, hipStream_t st);
-void launch_sa(const Blk& b, const Par& p, const Sub* subs, int nsub, hipStream_t st);
-void launch_sst(const Blk& b, const Par& p, const Sub* subs, int nsub, hipStream_t st);
-void launch_laminar_amut(const Blk& b, const Sub* subs, int nsub, hipStream_t st);
-void launch_inv_fluxes(const Blk& b, const Par& p, const Sub* subs, int nsub, hipStream_t st);
-void launch_visc_assemble_dt(const Blk& b, const Par& p, hipStream_t st);
+
+// task list of a subface launch: entries and, per tile, the entry it belongs to
+struct SubList {
+    SubTask* t = nullptr;
+    int* tile_task = nullptr;
+    int ntiles = 0;
+};
+struct HaloList {
+    HaloTask* t = nullptr;
+    int* tile_task = nullptr;
+    int ntiles = 0;
+};
+
+// device tables of one block set (built by the runtime from its blocks)
+struct SetPlan {
+    Blk* blk = nullptr;
+    Par* par = nullptr;
+    int nb = 0;
+    Par p0{};                        // the set-wide switches (turbulence model, NVAR, limiter)
+    int* tile_blk[SP_COUNT] = {};
+    int* tile0[SP_COUNT] = {};
+    int ntiles[SP_COUNT] = {};
+    SubList bc, amut, kbc;
+    HaloList halo;
+    int mx = 0;                      // max over blocks of max(nx, ny, nz)
+};
+
+void set_pre(const SetPlan& P, hipStream_t st);
+void set_inviscid(const SetPlan& P, hipStream_t st);
+void set_viscous_dt(const SetPlan& P, hipStream_t st);
+void set_update_limit(const SetPlan& P, hipStream_t st);
+void set_bc(const SetPlan& P, hipStream_t st);
+void set_halo(const SetPlan& P, hipStream_t st);
+void set_force_vt_kw(const SetPlan& P, hipStream_t st);
+int norms_parts();
+void set_norms(const SetPlan& P, double* part, hipStream_t st);
+
+HaloMap halo_map(const Sub& B);
+int halo_points(const HaloMap& h);
+void launch_halo(const Blk& src, const Blk& dst, const Sub& B, int NV, hipStream_t st);
 void launch_lusgs_sweep(const Blk& b, const Par& p, bool forward, hipStream_t st);
+void launch_force_sub(const Blk& b, const Par& p, const Sub& B, double* acc12, hipStream_t st);
 std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, long long* total_elements);
 int lusgs_fields(int NV);
 int lusgs_windows(int max_cells_i);
 void launch_lusgs_pack_geometry(const LuPlan& L, int NV, hipStream_t st);
 void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st);
-void launch_update_limit(const Blk& b, const Par& p, hipStream_t st);
-void launch_bc(const Blk& b, const Par& p, const Sub& B, hipStream_t st);
-void launch_halo(const Blk& src, const Blk& dst, const Sub& B, int NV, hipStream_t st);
-void launch_corners(const Blk& b, int NV, hipStream_t st);
-void launch_force_vt_kw(const Blk& b, const Par& p, hipStream_t st);
-int norms_parts();
-void launch_norms(const Blk& b, int NV, double* part, hipStream_t st);
-void launch_force_sub(const Blk& b, const Par& p, const Sub& B, double* acc12, hipStream_t st);
 // device geometry precompute (ocfd_geometry.hip)
 void launch_geometry(const Blk& b, const Sub* subs, int nsub, hipStream_t st);
 void launch_mesh_quality(const Blk& b, hipStream_t st);

@@ -56,6 +56,8 @@ struct ocfd_ctx {
     int lu_schedule = OCFD_LU_PIPELINED;     // LU-SGS sweep schedule
     LuPlan lu{};                              // pipelined-sweep tables (built on first use)
     std::vector<void*> lu_allocs;
+    SetPlan set{};                            // block-set launch tables (built on first use)
+    std::vector<void*> set_allocs;
 };
 
 namespace {
@@ -207,29 +209,21 @@ void read_subs(HostBlock& hb, int nsub, const int32_t* a) {
 }
 
 // ---- iteration -------------------------------------------------------------------------------
-// The iteration is enqueued stage by stage over all blocks; blocks are independent inside the
-// residual and update (the halo was filled at the end of the previous iteration), so the
-// stage-major order gives the results of the reference's block-major loop.
+// The iteration is enqueued stage by stage; every launch covers all blocks of the context.
+// Blocks are independent inside the residual and update (the halo was filled at the end of the
+// previous iteration), so the stage-major order gives the results of the reference's
+// block-major loop.
 void enqueue_stage(ocfd_ctx* c, int stage) {
+    const SetPlan& P = c->set;
     switch (stage) {
     case OCFD_STAGE_PRE:      // Set_Un is implicit (U is not modified before the update)
-        for (HostBlock& hb : c->blocks) {
-            const Blk& b = hb.d;
-            launch_reset_flags(b, c->st);
-            launch_prims(b, c->p, c->st);
-            if (c->p.visc == 1) {
-                launch_viscosity(b, c->p, c->st);
-                if (c->p.turb == 2) launch_sa(b, c->p, hb.subs.data(), (int)hb.subs.size(), c->st);
-                else if (c->p.turb == 3) launch_sst(b, c->p, hb.subs.data(), (int)hb.subs.size(), c->st);
-                else launch_laminar_amut(b, hb.subs.data(), (int)hb.subs.size(), c->st);
-            }
-        }
+        set_pre(P, c->st);
         break;
     case OCFD_STAGE_INVISCID:
-        for (HostBlock& hb : c->blocks) launch_inv_fluxes(hb.d, c->p, hb.subs.data(), (int)hb.subs.size(), c->st);
+        set_inviscid(P, c->st);
         break;
     case OCFD_STAGE_VISCOUS_DT:
-        for (HostBlock& hb : c->blocks) launch_visc_assemble_dt(hb.d, c->p, c->st);
+        set_viscous_dt(P, c->st);
         break;
     case OCFD_STAGE_LU_FORWARD:
     case OCFD_STAGE_LU_BACKWARD:
@@ -239,17 +233,12 @@ void enqueue_stage(ocfd_ctx* c, int stage) {
             for (HostBlock& hb : c->blocks) launch_lusgs_sweep(hb.d, c->p, stage == OCFD_STAGE_LU_FORWARD, c->st);
         break;
     case OCFD_STAGE_UPDATE:
-        for (HostBlock& hb : c->blocks) launch_update_limit(hb.d, c->p, c->st);
+        set_update_limit(P, c->st);
         break;
     case OCFD_STAGE_BC_HALO:
-        for (HostBlock& hb : c->blocks)
-            for (const Sub& B : hb.subs)
-                if (B.bc > 0) launch_bc(hb.d, c->p, B, c->st);
-        for (HostBlock& hb : c->blocks)
-            for (const Sub& B : hb.subs)
-                if (B.bc < 0) launch_halo(hb.d, c->blocks[B.nb1 - 1].d, B, c->p.NV, c->st);
-        for (HostBlock& hb : c->blocks) launch_corners(hb.d, c->p.NV, c->st);
-        for (HostBlock& hb : c->blocks) launch_force_vt_kw(hb.d, c->p, c->st);
+        set_bc(P, c->st);
+        set_halo(P, c->st);
+        set_force_vt_kw(P, c->st);
         break;
     }
 }
@@ -257,23 +246,9 @@ void enqueue_stage(ocfd_ctx* c, int stage) {
 void enqueue_residual(ocfd_ctx* c) {
     for (int s = OCFD_STAGE_PRE; s <= OCFD_STAGE_LU_BACKWARD; s++) enqueue_stage(c, s);
 }
-
-void enqueue_bc(ocfd_ctx* c) {
-    for (HostBlock& hb : c->blocks)
-        for (const Sub& B : hb.subs)
-            if (B.bc > 0) launch_bc(hb.d, c->p, B, c->st);
-}
-
-void enqueue_halo(ocfd_ctx* c) {
-    for (HostBlock& hb : c->blocks)
-        for (const Sub& B : hb.subs)
-            if (B.bc < 0) launch_halo(hb.d, c->blocks[B.nb1 - 1].d, B, c->p.NV, c->st);
-    for (HostBlock& hb : c->blocks) launch_corners(hb.d, c->p.NV, c->st);
-}
-
-void enqueue_force_vt_kw(ocfd_ctx* c) {
-    for (HostBlock& hb : c->blocks) launch_force_vt_kw(hb.d, c->p, c->st);
-}
+void enqueue_bc(ocfd_ctx* c) { set_bc(c->set, c->st); }
+void enqueue_halo(ocfd_ctx* c) { set_halo(c->set, c->st); }
+void enqueue_force_vt_kw(ocfd_ctx* c) { set_force_vt_kw(c->set, c->st); }
 
 void drop_graphs(ocfd_ctx* c) {
     for (int s = 0; s < OCFD_NSTAGES; s++) {
@@ -358,12 +333,115 @@ int check_lu_error(ocfd_ctx* c) {
     return fail(OCFD_EHIP, "LU-SGS sweep: a tile waited beyond its poll limit; the iteration is invalid");
 }
 
+void drop_set_plan(ocfd_ctx* c) {
+    for (void* q : c->set_allocs) (void)hipFree(q);
+    c->set_allocs.clear();
+    c->set = SetPlan{};
+}
+
+template <typename T>
+int upload_table(ocfd_ctx* c, const std::vector<T>& v, T** out) {
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    HIPCHK(hipMalloc(&q, bytes));
+    c->set_allocs.push_back(q);
+    if (!v.empty()) HIPCHK(hipMemcpy(q, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = (T*)q;
+    return OCFD_OK;
+}
+
+// extent of a subface launch (the kernels check the exact inclusive/exclusive node ranges)
+void sub_extent(const Sub& B, int& n0, int& n1) {
+    switch (B.face) {
+    case 1: case 4: n0 = B.je - B.jb + 1; n1 = B.ke - B.kb + 1; break;
+    case 2: case 5: n0 = B.ie - B.ib + 1; n1 = B.ke - B.kb + 1; break;
+    default: n0 = B.ie - B.ib + 1; n1 = B.je - B.jb + 1; break;
+    }
+}
+
+int build_sub_list(ocfd_ctx* c, std::vector<SubTask>& tasks, SubList& L) {
+    std::vector<int> tile_task;
+    for (size_t q = 0; q < tasks.size(); q++) {
+        SubTask& t = tasks[q];
+        t.tile0 = (int)tile_task.size();
+        int nt = ((t.n0 + 63) / 64) * std::max(t.n1, 0);
+        for (int a = 0; a < nt; a++) tile_task.push_back((int)q);
+    }
+    L.ntiles = (int)tile_task.size();
+    int r;
+    if ((r = upload_table(c, tasks, &L.t)) || (r = upload_table(c, tile_task, &L.tile_task))) return r;
+    return OCFD_OK;
+}
+
+// Device tables of the block set: blocks, parameters, the tile maps of every iteration space
+// and the task lists of the subface launches.  Built once per block layout.
+int build_set_plan(ocfd_ctx* c) {
+    if (c->set.blk) return OCFD_OK;
+    SetPlan& P = c->set;
+    std::vector<Blk> blks;
+    std::vector<Par> pars;
+    for (HostBlock& hb : c->blocks) { blks.push_back(hb.d); pars.push_back(c->p); }
+    P.nb = (int)blks.size();
+    P.p0 = c->p;
+    int r;
+    if ((r = upload_table(c, blks, &P.blk)) || (r = upload_table(c, pars, &P.par))) return r;
+    for (int sp = 0; sp < SP_COUNT; sp++) {
+        std::vector<int> tb, t0;
+        for (int bi = 0; bi < P.nb; bi++) {
+            int n0, n1, n2, o0, o1, o2;
+            space_dims(sp, blks[bi], n0, n1, n2, o0, o1, o2);
+            t0.push_back((int)tb.size());
+            long nt = (long)((n0 + 63) / 64) * n1 * n2;
+            for (long a = 0; a < nt; a++) tb.push_back(bi);
+        }
+        P.ntiles[sp] = (int)tb.size();
+        if ((r = upload_table(c, tb, &P.tile_blk[sp])) || (r = upload_table(c, t0, &P.tile0[sp]))) return r;
+    }
+    std::vector<SubTask> bc, amut, kbc;
+    std::vector<HaloTask> halo;
+    std::vector<int> halo_tile;
+    for (int bi = 0; bi < P.nb; bi++) {
+        HostBlock& hb = c->blocks[bi];
+        P.mx = std::max(P.mx, std::max(hb.nx, std::max(hb.ny, hb.nz)));
+        for (const Sub& B : hb.subs) {
+            SubTask t{};
+            t.blk = bi; t.B = B;
+            sub_extent(B, t.n0, t.n1);
+            if (B.bc > 0) {
+                if (B.bc == 2 && c->p.visc == 1) t.kind = SUB_WALL;
+                else if (B.bc == 4) { t.kind = SUB_FAR; t.arg = 0; }
+                else if (B.bc == 5) { t.kind = SUB_FAR; t.arg = 2; }
+                else if (B.bc == 6) { t.kind = SUB_FAR; t.arg = 1; }
+                else if (B.bc == 3 || B.bc == 2) t.kind = SUB_SYM;
+                else t.kind = SUB_EXT;
+                bc.push_back(t);
+                if (B.bc == 2) { SubTask a = t; a.kind = SUB_AMUT; amut.push_back(a); }
+                if (B.face == 3 || B.face == 6) {
+                    SubTask k = t;
+                    k.kind = SUB_KBC; k.n0 = B.ie - B.ib; k.n1 = std::max(1, B.je - B.jb);
+                    kbc.push_back(k);
+                }
+            } else if (B.bc < 0) {
+                HaloTask h{};
+                h.src = bi; h.dst = B.nb1 - 1; h.h = halo_map(B); h.n = halo_points(h.h);
+                h.tile0 = (int)halo_tile.size();
+                for (int a = 0; a < (h.n + 255) / 256; a++) halo_tile.push_back((int)halo.size());
+                halo.push_back(h);
+            }
+        }
+    }
+    if ((r = build_sub_list(c, bc, P.bc)) || (r = build_sub_list(c, amut, P.amut)) || (r = build_sub_list(c, kbc, P.kbc))) return r;
+    P.halo.ntiles = (int)halo_tile.size();
+    if ((r = upload_table(c, halo, &P.halo.t)) || (r = upload_table(c, halo_tile, &P.halo.tile_task))) return r;
+    return OCFD_OK;
+}
+
 int ready(ocfd_ctx* c) {
     if (!c) return fail(OCFD_EARG, "null context");
     if (c->blocks.empty()) return fail(OCFD_EARG, "no blocks added");
     for (size_t q = 0; q < c->blocks.size(); q++)
         if (c->blocks[q].no != (int)q + 1) return fail(OCFD_EARG, "block " + std::to_string(q + 1) + " missing");
-    return OCFD_OK;
+    return build_set_plan(c);
 }
 
 // ready() plus the tables the iteration's launches read (built outside any graph capture)
@@ -412,6 +490,7 @@ int ocfd_destroy(ocfd_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     drop_graphs(c);
     drop_lu_plan(c);
+    drop_set_plan(c);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     for (HostBlock& hb : c->blocks) for (void* q : hb.allocs) (void)hipFree(q);
     if (c->norm_part) (void)hipFree(c->norm_part);
@@ -432,6 +511,7 @@ static int add_block_common(ocfd_ctx* c, int32_t no, int32_t nx, int32_t ny, int
     if (r) return r;
     drop_graphs(c);
     drop_lu_plan(c);
+    drop_set_plan(c);
     *out = &hb;
     return OCFD_OK;
 }
@@ -791,7 +871,7 @@ int ocfd_res_norms(ocfd_ctx* c, double* res_max, double* res_rms) {
         HIPCHK(hipMalloc(&c->norm_part, need * sizeof(double)));
         c->norm_part_n = need;
     }
-    for (size_t m = 0; m < c->blocks.size(); m++) launch_norms(c->blocks[m].d, NV, c->norm_part + m * NV * P * 3, c->st);
+    set_norms(c->set, c->norm_part, c->st);
     std::vector<double> h(need);
     HIPCHK(hipMemcpyAsync(h.data(), c->norm_part, need * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));

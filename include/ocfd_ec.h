@@ -169,6 +169,23 @@ int ocfd_set_graph(ocfd_ctx *ctx, int32_t enable);
 #define OCFD_LU_PIPELINED_FENCED 2
 int ocfd_set_lu_schedule(ocfd_ctx *ctx, int32_t schedule);
 
+/* Batches.  No counterpart in the reference: several contexts (solves of different cases with
+ * the same NVAR, schemes, flux, turbulence model, limiter, gamma and w_LU) advance together,
+ * every kernel of the iteration covering the blocks of all of them in one launch; each context
+ * keeps its own parameters (Mach, AoA, Re, ...), state, counters, norms and forces.  A context is
+ * added after its geometry and state are set; while it is in a batch it is stepped through the
+ * batch only.  Results are those of the context stepped alone, bit for bit. */
+typedef struct ocfd_batch ocfd_batch;
+int ocfd_batch_create(ocfd_batch **out);
+int ocfd_batch_destroy(ocfd_batch *batch);
+int ocfd_batch_add(ocfd_batch *batch, ocfd_ctx *ctx);
+int ocfd_batch_remove(ocfd_batch *batch, ocfd_ctx *ctx);
+int ocfd_batch_step(ocfd_batch *batch, int32_t niter);
+int ocfd_batch_set_graph(ocfd_batch *batch, int32_t enable);
+
+/* Device memory held by the context's block arrays, in bytes (admission control). */
+int ocfd_mem_bytes(ocfd_ctx *ctx, int64_t *bytes);
+
 /* Wait for the context's stream. */
 int ocfd_sync(ocfd_ctx *ctx);
 

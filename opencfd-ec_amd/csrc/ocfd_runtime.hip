@@ -34,36 +34,73 @@ struct HostBlock {
     std::vector<void*> allocs;
     double* force_acc = nullptr;     // 12 doubles: 9 accumulators + centroid
     std::vector<double> nodes[3];    // node coordinates awaiting ocfd_setup_geometry
+    size_t bytes = 0;                // device memory of the block's arrays
     size_t n() const { return (size_t)d.NI * d.NJ * d.NK; }
 };
 
 }  // namespace
 
-struct ocfd_ctx {
-    ocfd_params hp{};
-    Par p{};
-    std::vector<HostBlock> blocks;   // index block_no - 1
+// Launch machinery of one block list: its stream, the per-stage graphs and the device tables
+// of the batched launches and of the pipelined LU-SGS sweeps.  A context owns one for its own
+// blocks; a batch owns one for the blocks of all its member contexts.
+struct Engine {
     hipStream_t st = nullptr;
-    double* norm_part = nullptr;
-    size_t norm_part_n = 0;
-    int kstep = 0;
-    double tt = 0.0;
     bool use_graph = false;
     hipGraphExec_t exec[OCFD_NSTAGES] = {};   // one captured graph per stage of the iteration
     hipGraph_t graph[OCFD_NSTAGES] = {};
     std::vector<hipEvent_t> events;           // stage boundaries of ocfd_step_timed
-    int device = 0;
     int lu_schedule = OCFD_LU_PIPELINED;     // LU-SGS sweep schedule
     LuPlan lu{};                              // pipelined-sweep tables (built on first use)
     std::vector<void*> lu_allocs;
-    SetPlan set{};                            // block-set launch tables (built on first use)
+    SetPlan set{};                            // batched-launch tables (built on first use)
     std::vector<void*> set_allocs;
 };
+
+// one block of an engine's list, the parameters of its case and the list position of its
+// case's block 1 (block numbers inside a case's boundary table are relative to it)
+struct BlockRef {
+    HostBlock* hb;
+    const Par* p;
+    int base;
+};
+
+struct ocfd_ctx {
+    ocfd_params hp{};
+    Par p{};
+    std::vector<HostBlock> blocks;   // index block_no - 1
+    double* norm_part = nullptr;
+    size_t norm_part_n = 0;
+    int kstep = 0;
+    double tt = 0.0;
+    int device = 0;
+    size_t bytes = 0;                // device memory of the block arrays
+    Engine e;
+    std::vector<BlockRef> refs() {
+        std::vector<BlockRef> r;
+        for (HostBlock& hb : blocks) r.push_back(BlockRef{&hb, &p, 0});
+        return r;
+    }
+};
+
+struct ocfd_batch {
+    Engine e;
+    std::vector<ocfd_ctx*> members;
+    std::vector<BlockRef> refs() {
+        std::vector<BlockRef> r;
+        for (ocfd_ctx* c : members) {
+            int base = (int)r.size();
+            for (HostBlock& hb : c->blocks) r.push_back(BlockRef{&hb, &c->p, base});
+        }
+        return r;
+    }
+};
+
 
 namespace {
 
 int dalloc(HostBlock& hb, double** p, size_t n) {
     void* q = nullptr;
+    hb.bytes += n * sizeof(double);
     HIPCHK(hipMalloc(&q, n * sizeof(double)));
     HIPCHK(hipMemset(q, 0, n * sizeof(double)));
     hb.allocs.push_back(q);
@@ -209,145 +246,138 @@ void read_subs(HostBlock& hb, int nsub, const int32_t* a) {
 }
 
 // ---- iteration -------------------------------------------------------------------------------
-// The iteration is enqueued stage by stage; every launch covers all blocks of the context.
+// The iteration is enqueued stage by stage; every launch covers all blocks of the engine's list.
 // Blocks are independent inside the residual and update (the halo was filled at the end of the
 // previous iteration), so the stage-major order gives the results of the reference's
 // block-major loop.
-void enqueue_stage(ocfd_ctx* c, int stage) {
-    const SetPlan& P = c->set;
+void enqueue_stage(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, int stage) {
+    const SetPlan& P = e.set;
     switch (stage) {
     case OCFD_STAGE_PRE:      // Set_Un is implicit (U is not modified before the update)
-        set_pre(P, c->st);
+        set_pre(P, e.st);
         break;
     case OCFD_STAGE_INVISCID:
-        set_inviscid(P, c->st);
+        set_inviscid(P, e.st);
         break;
     case OCFD_STAGE_VISCOUS_DT:
-        set_viscous_dt(P, c->st);
+        set_viscous_dt(P, e.st);
         break;
     case OCFD_STAGE_LU_FORWARD:
     case OCFD_STAGE_LU_BACKWARD:
-        if (c->lu_schedule != OCFD_LU_HYPERPLANE)
-            launch_lusgs_pipelined(c->lu, c->p, stage == OCFD_STAGE_LU_FORWARD, c->lu_schedule == OCFD_LU_PIPELINED_FENCED, c->st);
+        if (e.lu_schedule != OCFD_LU_HYPERPLANE)
+            launch_lusgs_pipelined(e.lu, p0, stage == OCFD_STAGE_LU_FORWARD, e.lu_schedule == OCFD_LU_PIPELINED_FENCED, e.st);
         else
-            for (HostBlock& hb : c->blocks) launch_lusgs_sweep(hb.d, c->p, stage == OCFD_STAGE_LU_FORWARD, c->st);
+            for (const BlockRef& r : refs) launch_lusgs_sweep(r.hb->d, *r.p, stage == OCFD_STAGE_LU_FORWARD, e.st);
         break;
     case OCFD_STAGE_UPDATE:
-        set_update_limit(P, c->st);
+        set_update_limit(P, e.st);
         break;
     case OCFD_STAGE_BC_HALO:
-        set_bc(P, c->st);
-        set_halo(P, c->st);
-        set_force_vt_kw(P, c->st);
+        set_bc(P, e.st);
+        set_halo(P, e.st);
+        set_force_vt_kw(P, e.st);
         break;
     }
 }
 
-void enqueue_residual(ocfd_ctx* c) {
-    for (int s = OCFD_STAGE_PRE; s <= OCFD_STAGE_LU_BACKWARD; s++) enqueue_stage(c, s);
-}
-void enqueue_bc(ocfd_ctx* c) { set_bc(c->set, c->st); }
-void enqueue_halo(ocfd_ctx* c) { set_halo(c->set, c->st); }
-void enqueue_force_vt_kw(ocfd_ctx* c) { set_force_vt_kw(c->set, c->st); }
-
-void drop_graphs(ocfd_ctx* c) {
+void drop_graphs(Engine& e) {
     for (int s = 0; s < OCFD_NSTAGES; s++) {
-        if (c->exec[s]) { (void)hipGraphExecDestroy(c->exec[s]); c->exec[s] = nullptr; }
-        if (c->graph[s]) { (void)hipGraphDestroy(c->graph[s]); c->graph[s] = nullptr; }
+        if (e.exec[s]) { (void)hipGraphExecDestroy(e.exec[s]); e.exec[s] = nullptr; }
+        if (e.graph[s]) { (void)hipGraphDestroy(e.graph[s]); e.graph[s] = nullptr; }
     }
 }
 
 // One stage of the iteration: eager launches, or the stage's graph (captured on first use).
-int run_stage(ocfd_ctx* c, int stage) {
-    if (!c->use_graph) { enqueue_stage(c, stage); return OCFD_OK; }
-    if (!c->exec[stage]) {
-        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        enqueue_stage(c, stage);
-        HIPCHK(hipStreamEndCapture(c->st, &c->graph[stage]));
-        HIPCHK(hipGraphInstantiate(&c->exec[stage], c->graph[stage], nullptr, nullptr, 0));
+int run_stage(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, int stage) {
+    if (!e.use_graph) { enqueue_stage(e, refs, p0, stage); return OCFD_OK; }
+    if (!e.exec[stage]) {
+        HIPCHK(hipStreamBeginCapture(e.st, hipStreamCaptureModeThreadLocal));
+        enqueue_stage(e, refs, p0, stage);
+        HIPCHK(hipStreamEndCapture(e.st, &e.graph[stage]));
+        HIPCHK(hipGraphInstantiate(&e.exec[stage], e.graph[stage], nullptr, nullptr, 0));
     }
-    HIPCHK(hipGraphLaunch(c->exec[stage], c->st));
+    HIPCHK(hipGraphLaunch(e.exec[stage], e.st));
     return OCFD_OK;
 }
 
-void drop_lu_plan(ocfd_ctx* c) {
-    for (void* q : c->lu_allocs) (void)hipFree(q);
-    c->lu_allocs.clear();
-    c->lu = LuPlan{};
+void drop_lu_plan(Engine& e) {
+    for (void* q : e.lu_allocs) (void)hipFree(q);
+    e.lu_allocs.clear();
+    e.lu = LuPlan{};
 }
 
-// Device tables of the pipelined LU-SGS sweeps: the block table, the tile lists of both sweep
-// directions, one progress counter per tile and direction (-1 = nothing done) and the error flag.
-int build_lu_plan(ocfd_ctx* c) {
-    if (c->lu.fwd) return OCFD_OK;
+void drop_set_plan(Engine& e) {
+    for (void* q : e.set_allocs) (void)hipFree(q);
+    e.set_allocs.clear();
+    e.set = SetPlan{};
+}
+
+// every table of the engine, e.g. when its block list changes
+void drop_plans(Engine& e) {
+    drop_graphs(e);
+    drop_lu_plan(e);
+    drop_set_plan(e);
+}
+
+template <typename T>
+int upload_table(std::vector<void*>& allocs, const std::vector<T>& v, T** out) {
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    HIPCHK(hipMalloc(&q, bytes));
+    allocs.push_back(q);
+    if (!v.empty()) HIPCHK(hipMemcpy(q, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = (T*)q;
+    return OCFD_OK;
+}
+
+// Device tables of the pipelined sweeps: the block table, the tile lists of both sweep
+// directions, one progress counter per tile and direction (-1 = nothing done), the error flag
+// and the sweep-ordered storage with the geometry packed into it.
+int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, int NV) {
+    if (e.lu.fwd) return OCFD_OK;
     std::vector<Blk> blks;
-    for (HostBlock& hb : c->blocks) blks.push_back(hb.d);
+    for (const BlockRef& r : refs) blks.push_back(r.hb->d);
     long long P = 0;
     std::vector<LuTask> f = lusgs_tasks(blks, true, &P), b = lusgs_tasks(blks, false, nullptr);
-    auto put = [&](const void* src, size_t bytes, void** out) -> int {
-        void* q = nullptr;
-        HIPCHK(hipMalloc(&q, bytes));
-        c->lu_allocs.push_back(q);
-        HIPCHK(hipMemcpy(q, src, bytes, hipMemcpyHostToDevice));
-        *out = q;
-        return OCFD_OK;
-    };
     std::vector<int> minus(std::max(f.size(), b.size()), -1), zero(4, 0);
-    void *pb, *pf, *pw, *qf, *qb, *pe;
+    std::vector<int> mf(minus.begin(), minus.begin() + f.size()), mb(minus.begin(), minus.begin() + b.size());
+    Blk* pb; LuTask *pf, *pw; int *qf, *qb, *pe;
     int r;
-    if ((r = put(blks.data(), blks.size() * sizeof(Blk), &pb)) || (r = put(f.data(), f.size() * sizeof(LuTask), &pf)) ||
-        (r = put(b.data(), b.size() * sizeof(LuTask), &pw)) || (r = put(minus.data(), f.size() * sizeof(int), &qf)) ||
-        (r = put(minus.data(), b.size() * sizeof(int), &qb)) || (r = put(zero.data(), zero.size() * sizeof(int), &pe)))
+    if ((r = upload_table(e.lu_allocs, blks, &pb)) || (r = upload_table(e.lu_allocs, f, &pf)) ||
+        (r = upload_table(e.lu_allocs, b, &pw)) || (r = upload_table(e.lu_allocs, mf, &qf)) ||
+        (r = upload_table(e.lu_allocs, mb, &qb)) || (r = upload_table(e.lu_allocs, zero, &pe)))
         return r;
-    c->lu.blks = (const Blk*)pb; c->lu.fwd = (const LuTask*)pf; c->lu.bwd = (const LuTask*)pw;
-    c->lu.n_fwd = (int)f.size(); c->lu.n_bwd = (int)b.size();
-    c->lu.prog_fwd = (int*)qf; c->lu.prog_bwd = (int*)qb; c->lu.err = (int*)pe;
-    // sweep-ordered storage and the geometry in it
+    e.lu.blks = pb; e.lu.fwd = pf; e.lu.bwd = pw;
+    e.lu.n_fwd = (int)f.size(); e.lu.n_bwd = (int)b.size();
+    e.lu.prog_fwd = qf; e.lu.prog_bwd = qb; e.lu.err = pe;
     void* pk = nullptr;
-    size_t bytes = (size_t)lusgs_fields(c->p.NV) * (size_t)P * sizeof(double);
+    size_t bytes = (size_t)lusgs_fields(NV) * (size_t)P * sizeof(double);
     HIPCHK(hipMalloc(&pk, bytes));
-    c->lu_allocs.push_back(pk);
+    e.lu_allocs.push_back(pk);
     HIPCHK(hipMemset(pk, 0, bytes));
-    c->lu.pk = (double*)pk;
-    c->lu.P = P;
+    e.lu.pk = (double*)pk;
+    e.lu.P = P;
     int maxnx = 0;
     for (const Blk& q : blks) maxnx = std::max(maxnx, q.nx - 1);
-    c->lu.nwin = lusgs_windows(maxnx);
-    launch_lusgs_pack_geometry(c->lu, c->p.NV, c->st);
+    e.lu.nwin = lusgs_windows(maxnx);
+    launch_lusgs_pack_geometry(e.lu, NV, e.st);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(e.st));
     return OCFD_OK;
 }
 
 // After a sweep gave up waiting: clear the flag and re-arm both counter sets.
-int check_lu_error(ocfd_ctx* c) {
-    if (!c->lu.err) return OCFD_OK;
-    int e = 0;
-    HIPCHK(hipMemcpy(&e, c->lu.err, sizeof(int), hipMemcpyDeviceToHost));
-    if (e == 0) return OCFD_OK;
-    std::vector<int> minus(std::max(c->lu.n_fwd, c->lu.n_bwd), -1);
-    HIPCHK(hipMemcpy(c->lu.prog_fwd, minus.data(), c->lu.n_fwd * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->lu.prog_bwd, minus.data(), c->lu.n_bwd * sizeof(int), hipMemcpyHostToDevice));
+int check_lu_error(Engine& e) {
+    if (!e.lu.err) return OCFD_OK;
+    int v = 0;
+    HIPCHK(hipMemcpy(&v, e.lu.err, sizeof(int), hipMemcpyDeviceToHost));
+    if (v == 0) return OCFD_OK;
+    std::vector<int> minus(std::max(e.lu.n_fwd, e.lu.n_bwd), -1);
+    HIPCHK(hipMemcpy(e.lu.prog_fwd, minus.data(), e.lu.n_fwd * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e.lu.prog_bwd, minus.data(), e.lu.n_bwd * sizeof(int), hipMemcpyHostToDevice));
     int z = 0;
-    HIPCHK(hipMemcpy(c->lu.err, &z, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e.lu.err, &z, sizeof(int), hipMemcpyHostToDevice));
     return fail(OCFD_EHIP, "LU-SGS sweep: a tile waited beyond its poll limit; the iteration is invalid");
-}
-
-void drop_set_plan(ocfd_ctx* c) {
-    for (void* q : c->set_allocs) (void)hipFree(q);
-    c->set_allocs.clear();
-    c->set = SetPlan{};
-}
-
-template <typename T>
-int upload_table(ocfd_ctx* c, const std::vector<T>& v, T** out) {
-    void* q = nullptr;
-    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
-    HIPCHK(hipMalloc(&q, bytes));
-    c->set_allocs.push_back(q);
-    if (!v.empty()) HIPCHK(hipMemcpy(q, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    *out = (T*)q;
-    return OCFD_OK;
 }
 
 // extent of a subface launch (the kernels check the exact inclusive/exclusive node ranges)
@@ -359,7 +389,7 @@ void sub_extent(const Sub& B, int& n0, int& n1) {
     }
 }
 
-int build_sub_list(ocfd_ctx* c, std::vector<SubTask>& tasks, SubList& L) {
+int build_sub_list(std::vector<void*>& allocs, std::vector<SubTask>& tasks, SubList& L) {
     std::vector<int> tile_task;
     for (size_t q = 0; q < tasks.size(); q++) {
         SubTask& t = tasks[q];
@@ -369,22 +399,22 @@ int build_sub_list(ocfd_ctx* c, std::vector<SubTask>& tasks, SubList& L) {
     }
     L.ntiles = (int)tile_task.size();
     int r;
-    if ((r = upload_table(c, tasks, &L.t)) || (r = upload_table(c, tile_task, &L.tile_task))) return r;
+    if ((r = upload_table(allocs, tasks, &L.t)) || (r = upload_table(allocs, tile_task, &L.tile_task))) return r;
     return OCFD_OK;
 }
 
-// Device tables of the block set: blocks, parameters, the tile maps of every iteration space
-// and the task lists of the subface launches.  Built once per block layout.
-int build_set_plan(ocfd_ctx* c) {
-    if (c->set.blk) return OCFD_OK;
-    SetPlan& P = c->set;
+// Device tables of the batched launches: blocks, their parameters, the tile maps of every
+// iteration space and the task lists of the subface launches.  Built once per block list.
+int build_set_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) {
+    if (e.set.blk) return OCFD_OK;
+    SetPlan& P = e.set;
     std::vector<Blk> blks;
     std::vector<Par> pars;
-    for (HostBlock& hb : c->blocks) { blks.push_back(hb.d); pars.push_back(c->p); }
+    for (const BlockRef& r : refs) { blks.push_back(r.hb->d); pars.push_back(*r.p); }
     P.nb = (int)blks.size();
-    P.p0 = c->p;
+    P.p0 = p0;
     int r;
-    if ((r = upload_table(c, blks, &P.blk)) || (r = upload_table(c, pars, &P.par))) return r;
+    if ((r = upload_table(e.set_allocs, blks, &P.blk)) || (r = upload_table(e.set_allocs, pars, &P.par))) return r;
     for (int sp = 0; sp < SP_COUNT; sp++) {
         std::vector<int> tb, t0;
         for (int bi = 0; bi < P.nb; bi++) {
@@ -395,20 +425,21 @@ int build_set_plan(ocfd_ctx* c) {
             for (long a = 0; a < nt; a++) tb.push_back(bi);
         }
         P.ntiles[sp] = (int)tb.size();
-        if ((r = upload_table(c, tb, &P.tile_blk[sp])) || (r = upload_table(c, t0, &P.tile0[sp]))) return r;
+        if ((r = upload_table(e.set_allocs, tb, &P.tile_blk[sp])) || (r = upload_table(e.set_allocs, t0, &P.tile0[sp]))) return r;
     }
     std::vector<SubTask> bc, amut, kbc;
     std::vector<HaloTask> halo;
     std::vector<int> halo_tile;
     for (int bi = 0; bi < P.nb; bi++) {
-        HostBlock& hb = c->blocks[bi];
+        const HostBlock& hb = *refs[bi].hb;
+        const Par& p = *refs[bi].p;
         P.mx = std::max(P.mx, std::max(hb.nx, std::max(hb.ny, hb.nz)));
         for (const Sub& B : hb.subs) {
             SubTask t{};
             t.blk = bi; t.B = B;
             sub_extent(B, t.n0, t.n1);
             if (B.bc > 0) {
-                if (B.bc == 2 && c->p.visc == 1) t.kind = SUB_WALL;
+                if (B.bc == 2 && p.visc == 1) t.kind = SUB_WALL;
                 else if (B.bc == 4) { t.kind = SUB_FAR; t.arg = 0; }
                 else if (B.bc == 5) { t.kind = SUB_FAR; t.arg = 2; }
                 else if (B.bc == 6) { t.kind = SUB_FAR; t.arg = 1; }
@@ -423,17 +454,77 @@ int build_set_plan(ocfd_ctx* c) {
                 }
             } else if (B.bc < 0) {
                 HaloTask h{};
-                h.src = bi; h.dst = B.nb1 - 1; h.h = halo_map(B); h.n = halo_points(h.h);
+                h.src = bi; h.dst = refs[bi].base + B.nb1 - 1; h.h = halo_map(B); h.n = halo_points(h.h);
                 h.tile0 = (int)halo_tile.size();
                 for (int a = 0; a < (h.n + 255) / 256; a++) halo_tile.push_back((int)halo.size());
                 halo.push_back(h);
             }
         }
     }
-    if ((r = build_sub_list(c, bc, P.bc)) || (r = build_sub_list(c, amut, P.amut)) || (r = build_sub_list(c, kbc, P.kbc))) return r;
+    if ((r = build_sub_list(e.set_allocs, bc, P.bc)) || (r = build_sub_list(e.set_allocs, amut, P.amut)) ||
+        (r = build_sub_list(e.set_allocs, kbc, P.kbc)))
+        return r;
     P.halo.ntiles = (int)halo_tile.size();
-    if ((r = upload_table(c, halo, &P.halo.t)) || (r = upload_table(c, halo_tile, &P.halo.tile_task))) return r;
+    if ((r = upload_table(e.set_allocs, halo, &P.halo.t)) || (r = upload_table(e.set_allocs, halo_tile, &P.halo.tile_task))) return r;
     return OCFD_OK;
+}
+
+// the tables an iteration's launches read (built outside any graph capture)
+int engine_prepare(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, bool lu) {
+    int r = build_set_plan(e, refs, p0);
+    if (r) return r;
+    if (lu && e.lu_schedule != OCFD_LU_HYPERPLANE) return build_lu_plan(e, refs, p0.NV);
+    return OCFD_OK;
+}
+
+int engine_finish(Engine& e) {
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e.st));
+    return check_lu_error(e);
+}
+
+// niter iterations; with stage_ms, HIP events at the stage boundaries (device ms per stage,
+// summed over the iterations)
+int engine_step(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, int niter, double* stage_ms) {
+    int r;
+    if (stage_ms) {
+        size_t need = (size_t)niter * (OCFD_NSTAGES + 1);
+        while (e.events.size() < need) {
+            hipEvent_t ev;
+            HIPCHK(hipEventCreate(&ev));
+            e.events.push_back(ev);
+        }
+    }
+    for (int it = 0; it < niter; it++) {
+        hipEvent_t* ev = stage_ms ? e.events.data() + (size_t)it * (OCFD_NSTAGES + 1) : nullptr;
+        if (ev) HIPCHK(hipEventRecord(ev[0], e.st));
+        for (int s = 0; s < OCFD_NSTAGES; s++) {
+            if ((r = run_stage(e, refs, p0, s))) return r;
+            if (ev) HIPCHK(hipEventRecord(ev[s + 1], e.st));
+        }
+    }
+    if ((r = engine_finish(e))) return r;
+    if (stage_ms) {
+        for (int s = 0; s < OCFD_NSTAGES; s++) stage_ms[s] = 0.0;
+        for (int it = 0; it < niter; it++) {
+            hipEvent_t* ev = e.events.data() + (size_t)it * (OCFD_NSTAGES + 1);
+            for (int s = 0; s < OCFD_NSTAGES; s++) {
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
+                stage_ms[s] += (double)ms;
+            }
+        }
+    }
+    return OCFD_OK;
+}
+
+void engine_destroy(Engine& e) {
+    if (e.st) (void)hipStreamSynchronize(e.st);
+    drop_plans(e);
+    for (hipEvent_t ev : e.events) (void)hipEventDestroy(ev);
+    e.events.clear();
+    if (e.st) (void)hipStreamDestroy(e.st);
+    e.st = nullptr;
 }
 
 int ready(ocfd_ctx* c) {
@@ -441,22 +532,25 @@ int ready(ocfd_ctx* c) {
     if (c->blocks.empty()) return fail(OCFD_EARG, "no blocks added");
     for (size_t q = 0; q < c->blocks.size(); q++)
         if (c->blocks[q].no != (int)q + 1) return fail(OCFD_EARG, "block " + std::to_string(q + 1) + " missing");
-    return build_set_plan(c);
+    return build_set_plan(c->e, c->refs(), c->p);
 }
 
-// ready() plus the tables the iteration's launches read (built outside any graph capture)
+// ready() plus the LU-SGS tables
 int prepare(ocfd_ctx* c) {
     int r = ready(c);
     if (r) return r;
-    if (c->lu_schedule != OCFD_LU_HYPERPLANE) return build_lu_plan(c);
-    return OCFD_OK;
+    return engine_prepare(c->e, c->refs(), c->p, true);
 }
 
-int finish(ocfd_ctx* c) {
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->st));
-    return check_lu_error(c);
+int finish(ocfd_ctx* c) { return engine_finish(c->e); }
+
+void enqueue_stage(ocfd_ctx* c, int stage) { enqueue_stage(c->e, c->refs(), c->p, stage); }
+void enqueue_residual(ocfd_ctx* c) {
+    for (int s = OCFD_STAGE_PRE; s <= OCFD_STAGE_LU_BACKWARD; s++) enqueue_stage(c, s);
 }
+void enqueue_bc(ocfd_ctx* c) { set_bc(c->e.set, c->e.st); }
+void enqueue_halo(ocfd_ctx* c) { set_halo(c->e.set, c->e.st); }
+void enqueue_force_vt_kw(ocfd_ctx* c) { set_force_vt_kw(c->e.set, c->e.st); }
 
 }  // namespace
 
@@ -480,21 +574,16 @@ int ocfd_create(const ocfd_params* p, ocfd_ctx** out) {
     c->hp = *p;
     c->p = to_par(*p);
     HIPCHK(hipGetDevice(&c->device));
-    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&c->e.st, hipStreamNonBlocking));
     *out = c;
     return OCFD_OK;
 }
 
 int ocfd_destroy(ocfd_ctx* c) {
     if (!c) return OCFD_OK;
-    if (c->st) (void)hipStreamSynchronize(c->st);
-    drop_graphs(c);
-    drop_lu_plan(c);
-    drop_set_plan(c);
-    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    engine_destroy(c->e);
     for (HostBlock& hb : c->blocks) for (void* q : hb.allocs) (void)hipFree(q);
     if (c->norm_part) (void)hipFree(c->norm_part);
-    if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return OCFD_OK;
 }
@@ -509,9 +598,8 @@ static int add_block_common(ocfd_ctx* c, int32_t no, int32_t nx, int32_t ny, int
     read_subs(hb, nsub, bc21);
     int r = alloc_block(c, hb);
     if (r) return r;
-    drop_graphs(c);
-    drop_lu_plan(c);
-    drop_set_plan(c);
+    c->bytes += hb.bytes;
+    drop_plans(c->e);
     *out = &hb;
     return OCFD_OK;
 }
@@ -671,7 +759,7 @@ int ocfd_setup_geometry(ocfd_ctx* c) {
         HIPCHK(hipMemcpy(hb.d.x, X[0][m].data(), hb.n() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(hb.d.y, X[1][m].data(), hb.n() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(hb.d.z, X[2][m].data(), hb.n() * sizeof(double), hipMemcpyHostToDevice));
-        launch_geometry(hb.d, hb.subs.data(), (int)hb.subs.size(), c->st);
+        launch_geometry(hb.d, hb.subs.data(), (int)hb.subs.size(), c->e.st);
     }
     // update_Mesh_Center: cell-centre ghosts of inner faces, then the periodic shift
     // (sub_udate_Meshlink_mpi.f90:12-249), through the U-halo kernel on (xc, yc, zc)
@@ -681,9 +769,9 @@ int ocfd_setup_geometry(ocfd_ctx* c) {
             Blk s = hb.d, t = c->blocks[B.nb1 - 1].d;
             s.U[0] = hb.d.xc; s.U[1] = hb.d.yc; s.U[2] = hb.d.zc;
             t.U[0] = t.xc; t.U[1] = t.yc; t.U[2] = t.zc;
-            launch_halo(s, t, B, 3, c->st);
+            launch_halo(s, t, B, 3, c->e.st);
         }
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->e.st));
     for (HostBlock& hb : c->blocks) {
         bool any = false;
         for (const Sub& B : hb.subs) any = any || B.bc == -2 || B.bc == -3;
@@ -716,14 +804,13 @@ int ocfd_setup_geometry(ocfd_ctx* c) {
         double* dW = nullptr;
         HIPCHK(hipMalloc(&dW, std::max<size_t>(W.size(), 3) * sizeof(double)));
         if (!W.empty()) HIPCHK(hipMemcpy(dW, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice));
-        for (HostBlock& hb : c->blocks) launch_wall_distance(hb.d, dW, (long)(W.size() / 3), c->st);
-        HIPCHK(hipStreamSynchronize(c->st));
+        for (HostBlock& hb : c->blocks) launch_wall_distance(hb.d, dW, (long)(W.size() / 3), c->e.st);
+        HIPCHK(hipStreamSynchronize(c->e.st));
         HIPCHK(hipFree(dW));
     }
-    for (HostBlock& hb : c->blocks) launch_mesh_quality(hb.d, c->st);
+    for (HostBlock& hb : c->blocks) launch_mesh_quality(hb.d, c->e.st);
     for (HostBlock& hb : c->blocks) for (int q = 0; q < 3; q++) hb.nodes[q].clear();
-    drop_graphs(c);
-    drop_lu_plan(c);         // the sweep-ordered geometry is taken from the new arrays
+    drop_plans(c->e);        // the sweep-ordered geometry is taken from the new arrays
     return finish(c);
 }
 
@@ -745,7 +832,7 @@ int ocfd_upload_U(ocfd_ctx* c, int32_t no, const double* U) {
 int ocfd_download_U(ocfd_ctx* c, int32_t no, double* U) {
     HostBlock* hb = find_block(c, no);
     if (!hb || !U) return fail(OCFD_EARG, "unknown block or null U");
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->e.st));
     int NV = c->p.NV, r;
     std::vector<double> h;
     for (int m = 0; m < NV; m++) {
@@ -758,7 +845,7 @@ int ocfd_download_U(ocfd_ctx* c, int32_t no, double* U) {
 int ocfd_download_field(ocfd_ctx* c, int32_t no, int32_t field, double* out) {
     HostBlock* hb = find_block(c, no);
     if (!hb || !out) return fail(OCFD_EARG, "unknown block or null output");
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->e.st));
     const Blk& b = hb->d;
     int nv = (field == OCFD_F_RES || field == OCFD_F_DU) ? c->p.NV : 1;
     long n0 = b.nx + 3, n1 = b.ny + 3, n2 = b.nz + 3;
@@ -808,12 +895,9 @@ int ocfd_step_lusgs(ocfd_ctx* c, int32_t niter) {
     int r = prepare(c);
     if (r) return r;
     for (HostBlock& hb : c->blocks) if ((r = check_subs(c, hb))) return r;
-    for (int it = 0; it < niter; it++) {
-        for (int s = 0; s < OCFD_NSTAGES; s++) if ((r = run_stage(c, s))) return r;
-        c->tt = c->tt + c->p.dt_global;
-        c->kstep = c->kstep + 1;
-    }
-    return finish(c);
+    if ((r = engine_step(c->e, c->refs(), c->p, niter, nullptr))) return r;
+    for (int it = 0; it < niter; it++) { c->tt = c->tt + c->p.dt_global; c->kstep = c->kstep + 1; }
+    return OCFD_OK;
 }
 
 int ocfd_step_timed(ocfd_ctx* c, int32_t niter, double* stage_ms) {
@@ -821,32 +905,8 @@ int ocfd_step_timed(ocfd_ctx* c, int32_t niter, double* stage_ms) {
     if (r) return r;
     if (!stage_ms || niter < 1) return fail(OCFD_EARG, "null stage_ms or niter < 1");
     for (HostBlock& hb : c->blocks) if ((r = check_subs(c, hb))) return r;
-    size_t need = (size_t)niter * (OCFD_NSTAGES + 1);
-    while (c->events.size() < need) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        c->events.push_back(e);
-    }
-    for (int it = 0; it < niter; it++) {
-        hipEvent_t* ev = c->events.data() + (size_t)it * (OCFD_NSTAGES + 1);
-        HIPCHK(hipEventRecord(ev[0], c->st));
-        for (int s = 0; s < OCFD_NSTAGES; s++) {
-            if ((r = run_stage(c, s))) return r;
-            HIPCHK(hipEventRecord(ev[s + 1], c->st));
-        }
-        c->tt = c->tt + c->p.dt_global;
-        c->kstep = c->kstep + 1;
-    }
-    if ((r = finish(c))) return r;
-    for (int s = 0; s < OCFD_NSTAGES; s++) stage_ms[s] = 0.0;
-    for (int it = 0; it < niter; it++) {
-        hipEvent_t* ev = c->events.data() + (size_t)it * (OCFD_NSTAGES + 1);
-        for (int s = 0; s < OCFD_NSTAGES; s++) {
-            float ms = 0.0f;
-            HIPCHK(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
-            stage_ms[s] += (double)ms;
-        }
-    }
+    if ((r = engine_step(c->e, c->refs(), c->p, niter, stage_ms))) return r;
+    for (int it = 0; it < niter; it++) { c->tt = c->tt + c->p.dt_global; c->kstep = c->kstep + 1; }
     return OCFD_OK;
 }
 
@@ -871,10 +931,10 @@ int ocfd_res_norms(ocfd_ctx* c, double* res_max, double* res_rms) {
         HIPCHK(hipMalloc(&c->norm_part, need * sizeof(double)));
         c->norm_part_n = need;
     }
-    set_norms(c->set, c->norm_part, c->st);
+    set_norms(c->e.set, c->norm_part, c->e.st);
     std::vector<double> h(need);
-    HIPCHK(hipMemcpyAsync(h.data(), c->norm_part, need * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpyAsync(h.data(), c->norm_part, need * sizeof(double), hipMemcpyDeviceToHost, c->e.st));
+    HIPCHK(hipStreamSynchronize(c->e.st));
     double mx[MAXV] = {0}, sm[MAXV] = {0};
     bool nan = false;
     long ncell = 0;
@@ -898,11 +958,11 @@ int ocfd_forces(ocfd_ctx* c, double* out) {
     std::vector<double> tot(9, 0.0);
     for (HostBlock& hb : c->blocks) {
         double init[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, c->hp.Centroid[0], c->hp.Centroid[1], c->hp.Centroid[2]};
-        HIPCHK(hipMemcpyAsync(hb.force_acc, init, sizeof(init), hipMemcpyHostToDevice, c->st));
-        for (const Sub& B : hb.subs) if (B.bc == 2) launch_force_sub(hb.d, c->p, B, hb.force_acc, c->st);
+        HIPCHK(hipMemcpyAsync(hb.force_acc, init, sizeof(init), hipMemcpyHostToDevice, c->e.st));
+        for (const Sub& B : hb.subs) if (B.bc == 2) launch_force_sub(hb.d, c->p, B, hb.force_acc, c->e.st);
         double acc[12];
-        HIPCHK(hipMemcpyAsync(acc, hb.force_acc, sizeof(acc), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipStreamSynchronize(c->st));
+        HIPCHK(hipMemcpyAsync(acc, hb.force_acc, sizeof(acc), hipMemcpyDeviceToHost, c->e.st));
+        HIPCHK(hipStreamSynchronize(c->e.st));
         for (int q = 0; q < 9; q++) tot[q] = tot[q] + acc[q];
     }
     for (int q = 0; q < 6; q++) tot[q] = 2.0 * tot[q] / c->hp.Ref_S;
@@ -918,7 +978,7 @@ int ocfd_block_status(ocfd_ctx* c, int32_t no, int32_t* overlimit, int32_t* nlim
     HostBlock* hb = find_block(c, no);
     if (!hb || !overlimit || !nlimited) return fail(OCFD_EARG, "unknown block or null output");
     int f[16];
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->e.st));
     HIPCHK(hipMemcpy(f, hb->d.flags, sizeof(f), hipMemcpyDeviceToHost));
     *overlimit = f[0]; *nlimited = f[1];
     return OCFD_OK;
@@ -932,7 +992,8 @@ int ocfd_get_counters(ocfd_ctx* c, int32_t* kstep, double* tt) {
 
 int ocfd_set_graph(ocfd_ctx* c, int32_t enable) {
     if (!c) return fail(OCFD_EARG, "null context");
-    c->use_graph = enable != 0;
+    if ((enable != 0) != c->e.use_graph) drop_graphs(c->e);
+    c->e.use_graph = enable != 0;
     return OCFD_OK;
 }
 
@@ -940,13 +1001,89 @@ int ocfd_set_lu_schedule(ocfd_ctx* c, int32_t schedule) {
     if (!c) return fail(OCFD_EARG, "null context");
     if (schedule != OCFD_LU_PIPELINED && schedule != OCFD_LU_HYPERPLANE && schedule != OCFD_LU_PIPELINED_FENCED)
         return fail(OCFD_EARG, "unknown LU-SGS schedule");
-    if (schedule != c->lu_schedule) { drop_graphs(c); c->lu_schedule = schedule; }
+    if (schedule != c->e.lu_schedule) { drop_graphs(c->e); c->e.lu_schedule = schedule; }
     return OCFD_OK;
 }
 
 int ocfd_sync(ocfd_ctx* c) {
     if (!c) return fail(OCFD_EARG, "null context");
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->e.st));
+    return OCFD_OK;
+}
+
+// ---- batches -------------------------------------------------------------------------------
+
+static bool same_switches(const Par& a, const Par& b) {
+    return a.NV == b.NV && a.visc == b.visc && a.turb == b.turb && a.scheme == b.scheme && a.flux == b.flux &&
+           a.recon == b.recon && a.bscheme == b.bscheme && a.local_dt == b.local_dt && a.dtime_mesh == b.dtime_mesh &&
+           a.positivity == b.positivity && a.limit_flow == b.limit_flow && a.gamma == b.gamma && a.w_LU == b.w_LU;
+}
+
+int ocfd_batch_create(ocfd_batch** out) {
+    if (!out) return fail(OCFD_EARG, "null argument");
+    ocfd_batch* b = new ocfd_batch();
+    hipError_t e = hipStreamCreateWithFlags(&b->e.st, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete b; return fail(OCFD_EHIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e)); }
+    *out = b;
+    return OCFD_OK;
+}
+
+int ocfd_batch_destroy(ocfd_batch* b) {
+    if (!b) return OCFD_OK;
+    engine_destroy(b->e);
+    delete b;
+    return OCFD_OK;
+}
+
+int ocfd_batch_add(ocfd_batch* b, ocfd_ctx* c) {
+    if (!b || !c) return fail(OCFD_EARG, "null argument");
+    int r = ready(c);
+    if (r) return r;
+    for (HostBlock& hb : c->blocks) if ((r = check_subs(c, hb))) return r;
+    for (ocfd_ctx* m : b->members) if (m == c) return fail(OCFD_EARG, "context already in the batch");
+    if (!b->members.empty() && !same_switches(b->members[0]->p, c->p))
+        return fail(OCFD_EARG, "a batch needs the same NVAR, schemes, flux, turbulence model, limiter, gamma and w_LU in every member");
+    HIPCHK(hipStreamSynchronize(c->e.st));
+    b->members.push_back(c);
+    drop_plans(b->e);
+    return OCFD_OK;
+}
+
+int ocfd_batch_remove(ocfd_batch* b, ocfd_ctx* c) {
+    if (!b || !c) return fail(OCFD_EARG, "null argument");
+    for (size_t q = 0; q < b->members.size(); q++) {
+        if (b->members[q] != c) continue;
+        HIPCHK(hipStreamSynchronize(b->e.st));
+        b->members.erase(b->members.begin() + q);
+        drop_plans(b->e);
+        return OCFD_OK;
+    }
+    return fail(OCFD_EARG, "context not in the batch");
+}
+
+int ocfd_batch_step(ocfd_batch* b, int32_t niter) {
+    if (!b) return fail(OCFD_EARG, "null batch");
+    if (b->members.empty()) return OCFD_OK;
+    const Par& p0 = b->members[0]->p;
+    std::vector<BlockRef> refs = b->refs();
+    int r = engine_prepare(b->e, refs, p0, true);
+    if (r) return r;
+    if ((r = engine_step(b->e, refs, p0, niter, nullptr))) return r;
+    for (ocfd_ctx* c : b->members)
+        for (int it = 0; it < niter; it++) { c->tt = c->tt + c->p.dt_global; c->kstep = c->kstep + 1; }
+    return OCFD_OK;
+}
+
+int ocfd_batch_set_graph(ocfd_batch* b, int32_t enable) {
+    if (!b) return fail(OCFD_EARG, "null batch");
+    if ((enable != 0) != b->e.use_graph) drop_graphs(b->e);
+    b->e.use_graph = enable != 0;
+    return OCFD_OK;
+}
+
+int ocfd_mem_bytes(ocfd_ctx* c, int64_t* bytes) {
+    if (!c || !bytes) return fail(OCFD_EARG, "null argument");
+    *bytes = (int64_t)c->bytes;
     return OCFD_OK;
 }
 

@@ -57,6 +57,8 @@ EXPORTS = [
     "ocfd_step_lusgs", "ocfd_residual", "ocfd_update_limit", "ocfd_bc", "ocfd_halo", "ocfd_force_vt_kw",
     "ocfd_res_norms", "ocfd_forces", "ocfd_block_status", "ocfd_get_counters", "ocfd_set_graph", "ocfd_sync",
     "ocfd_step_timed", "ocfd_set_lu_schedule",
+    "ocfd_batch_create", "ocfd_batch_destroy", "ocfd_batch_add", "ocfd_batch_remove", "ocfd_batch_step",
+    "ocfd_batch_set_graph", "ocfd_mem_bytes",
 ]
 LU_PIPELINED, LU_HYPERPLANE, LU_PIPELINED_FENCED = 0, 1, 2
 
@@ -92,6 +94,13 @@ def lib():
         L.ocfd_set_graph.argtypes = [vp, i32]
         L.ocfd_step_timed.argtypes = [vp, i32, dp]
         L.ocfd_set_lu_schedule.argtypes = [vp, i32]
+        L.ocfd_batch_create.argtypes = [C.POINTER(vp)]
+        L.ocfd_batch_destroy.argtypes = [vp]
+        L.ocfd_batch_add.argtypes = [vp, vp]
+        L.ocfd_batch_remove.argtypes = [vp, vp]
+        L.ocfd_batch_step.argtypes = [vp, i32]
+        L.ocfd_batch_set_graph.argtypes = [vp, i32]
+        L.ocfd_mem_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -324,10 +333,57 @@ class Solver:
         _check(lib().ocfd_forces(self._h, _dp(out)))
         return out
 
+    def counters(self):
+        """(Kstep, tt) of the context."""
+        k = C.c_int32(); t = C.c_double()
+        _check(lib().ocfd_get_counters(self._h, C.byref(k), C.byref(t)))
+        return int(k.value), float(t.value)
+
+    def mem_bytes(self) -> int:
+        v = C.c_int64()
+        _check(lib().ocfd_mem_bytes(self._h, C.byref(v)))
+        return int(v.value)
+
     def status(self, no: int):
         a = C.c_int32(); b = C.c_int32()
         _check(lib().ocfd_block_status(self._h, no, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+
+class Batch:
+    """Several solvers advancing together, every kernel covering all their blocks in one launch
+    (ocfd_batch_*).  Members keep their own parameters, state, counters, norms and forces."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().ocfd_batch_create(C.byref(h)))
+        self._h = h
+        self.members: List["Solver"] = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ocfd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, s: "Solver"):
+        _check(lib().ocfd_batch_add(self._h, s._h))
+        self.members.append(s)
+
+    def remove(self, s: "Solver"):
+        _check(lib().ocfd_batch_remove(self._h, s._h))
+        self.members.remove(s)
+
+    def step(self, n: int = 1):
+        _check(lib().ocfd_batch_step(self._h, n))
+
+    def set_graph(self, on: bool):
+        _check(lib().ocfd_batch_set_graph(self._h, 1 if on else 0))
 
 
 def solver_from_case(case, precision: int = 0, states: Optional[Sequence[np.ndarray]] = None) -> Solver:

@@ -69,6 +69,42 @@ def cpu_sample_case(name: str, flux: int):
     return build_workload(name, flux), "the benchmark workload"
 
 
+def batch_member(name: str, flux: int, q: int):
+    """Member q of a batched run: the workload's grid and schemes at its own freestream (AoA raised
+    by 0.5 deg per member), as different requests on the same geometry would be."""
+    case = build_workload(name, flux)
+    if q:
+        case = C.Case(case.name, case.blocks, dict(case.params, AoA=float(case.params["AoA"]) + 0.5 * q))
+    return case
+
+
+def variant_rate(name: str, flux: int, precision: int, batch: int, steps: int, warmup: int, graph: bool):
+    """Mcell-iter/s of `batch` copies of the workload stepped together (one launch per kernel over
+    all their blocks), in the given precision mode; inputs resident before the timed region."""
+    members = [A.solver_from_case(batch_member(name, flux, q), precision=precision) for q in range(batch)]
+    cells = sum(ncells(batch_member(name, flux, 0)) for _ in range(batch))
+    if batch == 1:
+        run = members[0]
+        run.set_graph(graph)
+    else:
+        run = A.Batch()
+        run.set_graph(graph)
+        for m in members:
+            run.add(m)
+    run.step(warmup)
+    t0 = time.perf_counter()
+    run.step(steps)               # returns after the device has finished
+    wall = time.perf_counter() - t0
+    rms = [float(m.norms()[1][0]) for m in members]
+    if batch > 1:
+        run.close()
+    for m in members:
+        m.close()
+    return dict(precision="fp64" if precision == 0 else "mixed", batch=batch, cells=cells,
+                value=round(cells * steps / wall / 1e6, 3), ms_per_step=round(wall * 1e3 / steps, 4),
+                res_rms1_max=max(rms))
+
+
 def ncells(case) -> int:
     return sum((b.dims[0] - 1) * (b.dims[1] - 1) * (b.dims[2] - 1) for b in case.blocks)
 
@@ -109,6 +145,53 @@ def cpu_baseline(case, what: str, iters: int):
                 sample=sample + " (oracle C restatement, serial loops)")
 
 
+def serve_main(args) -> int:
+    """The serving sweep: requests arrive (Poisson), are admitted against the HBM budget and
+    advance together through shared launches until their Res_rms(1) has dropped by --drop; the
+    baseline runs the plain solver on one request at a time to the same criterion."""
+    from ocfd_amd import serve as SV
+    reqs = SV.sweep_requests(args.serve, lam=args.rate, seed=5)
+    srv = SV.Server(mem_budget=args.serve_budget * 1e9, round_iters=args.round_iters, drop=args.drop,
+                    max_iters=args.max_iters, graph=bool(args.graph))
+    log = (lambda m: print("# " + m, file=sys.stderr, flush=True)) if args.verbose else None
+    t0 = time.perf_counter()
+    res = srv.run(reqs, log=log)
+    wall = time.perf_counter() - t0
+    served = SV.summarize(res, wall)
+    k = min(args.seq_sample, len(reqs))
+    sub = sorted(reqs, key=lambda r: r.arrival)[:k]
+    # the baseline starts each request as soon as the previous one is done (no idle gaps): its
+    # solves/hour is an upper bound for the sequential solver at this arrival rate
+    sub = [SV.Request(r.rid, 0.0, r.case, r.cells) for r in sub]
+    t0 = time.perf_counter()
+    seq = srv.run_sequential(sub)
+    swall = time.perf_counter() - t0
+    sequential = SV.summarize(seq, swall)
+    same = {r.rid: r for r in res}
+    dev = [max(abs(q.CL - same[q.rid].CL) / max(abs(q.CL), 1e-12), abs(q.CD - same[q.rid].CD) / max(abs(q.CD), 1e-12))
+           for q in seq if q.rid in same]
+    out = {
+        "metric": "sweep solves/hour with p50/p99 latency",
+        "value": round(served["solves_per_hour"], 2),
+        "unit": "solves/h",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic requests generated in-process (NACA 4-digit C-grids, random Mach/AoA/Re)",
+        "config": {"workload": f"naca_sweep_{args.serve}", "requests": args.serve, "arrival_rate_per_s": args.rate,
+                   "dims": [list(d) for d in SV.DIMS], "drop": args.drop, "round_iters": args.round_iters,
+                   "max_iters": args.max_iters, "mem_budget_GB": args.serve_budget},
+        "served": {k2: (round(v, 4) if isinstance(v, float) else v) for k2, v in served.items()},
+        "sequential": dict({k2: (round(v, 4) if isinstance(v, float) else v) for k2, v in sequential.items()},
+                           sample=f"the first {k} requests by arrival, back to back"),
+        "speedup_solves_per_hour": round(served["solves_per_hour"] / max(sequential["solves_per_hour"], 1e-9), 3),
+        "served_vs_sequential_max_rel_force_diff": max(dev) if dev else None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +205,20 @@ def main() -> int:
     ap.add_argument("--lu", choices=("pipelined", "hyperplane", "fenced"), default="pipelined", help="LU-SGS sweep schedule")
     ap.add_argument("--cpu-iters", type=int, default=20, help="iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="precision of the headline run (fp64: the drop-in; mixed: FP32 reconstruction, "
+                         "fluxes and SA terms)")
+    ap.add_argument("--variants", default=None,
+                    help="extra runs reported beside the headline, precision:batch pairs ('' for none; "
+                         "default mixed:1,fp64:4,mixed:4 on the single-block C-grid, none elsewhere)")
+    ap.add_argument("--serve", type=int, default=0, help="run the serving sweep with this many requests instead")
+    ap.add_argument("--rate", type=float, default=2.0, help="serving: request arrivals per second")
+    ap.add_argument("--drop", type=float, default=1e-5, help="serving: Res_rms(1) drop that ends a solve")
+    ap.add_argument("--round-iters", type=int, default=20, help="serving: iterations between admission/convergence checks")
+    ap.add_argument("--max-iters", type=int, default=20000, help="serving: iteration cap per request")
+    ap.add_argument("--serve-budget", type=float, default=200.0, help="serving: HBM budget of admitted requests, GB")
+    ap.add_argument("--seq-sample", type=int, default=32, help="serving: requests the sequential baseline runs")
+    ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -130,9 +227,12 @@ def main() -> int:
               "multi-GPU sharding by block is a design note, not an implemented path.", file=sys.stderr)
         return 2
 
+    if args.serve:
+        return serve_main(args)
     case = build_workload(args.workload, args.flux)
     cells = ncells(case)
-    s = A.solver_from_case(case)          # geometry, wall distance, freestream: all on the device
+    precision = 0 if args.precision == "fp64" else 1
+    s = A.solver_from_case(case, precision=precision)   # geometry, wall distance, freestream: on the device
     nv = s.nv
     s.set_graph(bool(args.graph))
     s.set_lu_schedule({"pipelined": A.LU_PIPELINED, "hyperplane": A.LU_HYPERPLANE, "fenced": A.LU_PIPELINED_FENCED}[args.lu])
@@ -174,7 +274,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if precision == 0 else "f32+f64",
         "data": "synthetic grid and freestream start (generated in-process)",
         "config": {
             "workload": args.workload,
@@ -185,6 +285,7 @@ def main() -> int:
             "flux": FLUX_NAMES[int(case.params["Iflag_Flux"])],
             "graph": bool(args.graph),
             "lu_schedule": args.lu,
+            "precision": args.precision,
         },
         "roofline": roofline,
         "iteration_hbm": {"alg_bytes_per_cell": sum(words) * 8, "achieved": round(iter_bytes / (ms_per_step * 1e-3) / 1e9, 2),
@@ -193,6 +294,16 @@ def main() -> int:
         "passes": {k: {"ms": round(v["ms"], 4), "GB/s": round(v["gbs"], 1)} for k, v in passes.items()},
         "state": {"res_rms1": float(rms[0]), "overlimit": int(overlimit)},
     }
+    s.close()
+    if args.variants is None:
+        args.variants = "mixed:1,fp64:4,mixed:4" if args.workload == "naca0012_cgrid_257x97x33" else ""
+    if args.variants:
+        out["variants"] = []
+        for v in args.variants.split(","):
+            pr, bt = v.split(":")
+            r = variant_rate(args.workload, args.flux, 0 if pr == "fp64" else 1, int(bt), args.steps, args.warmup, bool(args.graph))
+            r["vs_headline"] = round(r["value"] / value, 3)
+            out["variants"].append(r)
     if not args.no_cpu:
         ccase, what = cpu_sample_case(args.workload, args.flux)
         out["cpu_baseline"] = cpu_baseline(ccase, what, args.cpu_iters)

@@ -44,7 +44,10 @@ typedef struct ocfd_params {
     int32_t NVAR, If_viscous, Iflag_turbulence_model, Iflag_Scheme, Iflag_Flux;
     int32_t IFlag_Reconstruction, Bound_Scheme, Iflag_local_dt, If_dtime_mesh;
     int32_t IF_Scheme_Positivity, IFLAG_LIMIT_FLOW, Cood_Y_UP;
-    int32_t precision;       /* 0: all FP64 (the drop-in) */
+    int32_t precision;       /* 0: all FP64 (the drop-in).  1: reconstruction, face fluxes (inviscid,
+                                viscous) and the SA face and source terms in FP32; metrics, LU-SGS
+                                (Jacobians, diagonal), update, limiter and norms in FP64.  Laminar
+                                and SA only (SST is refused with OCFD_EUNSUPPORTED). */
 } ocfd_params;
 
 /* Geometry of one block as Comput_Goemetric_var leaves it (sub_geometry.f90:6-346). */

@@ -17,6 +17,7 @@ struct Par {                 // control parameters after set_const_para (sub_rea
     double CFL, dtmax, dtmin, dt_global, w_LU, MUT_MAX, Kt_inf, Wt_inf;
     double Ldmin, Ldmax, Lpmin, Lpmax, Lumax, LSAmax;
     int NV, visc, turb, scheme, flux, recon, bscheme, local_dt, dtime_mesh, positivity, limit_flow;
+    int mixed;                   // 1: reconstruction, face fluxes and the SA terms in FP32
 };
 
 struct Blk {
@@ -29,6 +30,9 @@ struct Blk {
     double *F[3][5];                     // inviscid face flux, m = 1..5
     double *Fv[3][4];                    // viscous face flux, m = 2..5
     double *Fsa[3][2];                   // SA face terms: convection+diffusion flux, grad(vt).n S
+    // mixed precision: FP32 primitives (rho u v w p T) and FP32 face terms; the face arrays share
+    // the memory of their FP64 counterparts
+    float *primf[6], *Ff[3][5], *Fvf[3][4], *Fsaf[3][2];
     double *aux;                         // turbulence scratch (SST f1)
     uint8_t *bcf[3];                     // face-row flags: bit0 lower face physical, bit1 upper face physical
     double *surf[6][3];                  // Surf1..6 (wall viscous force of the i-/j-/k-/i+/j+/k+ faces)

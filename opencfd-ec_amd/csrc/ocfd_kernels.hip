@@ -20,6 +20,10 @@ __global__ void __launch_bounds__(64) k_prims(BSet S) {
     double T = (b.U[4][s] - (0.5 * d) * ((u * u + v * v) + w * w)) / (p.Cv * d);
     b.prim[0][s] = d; b.prim[1][s] = u; b.prim[2][s] = v; b.prim[3][s] = w;
     b.prim[4][s] = (p00 * d) * T; b.prim[5][s] = T; b.prim[6][s] = sqrt(T) / p.Ma;
+    if (p.mixed) {
+        b.primf[0][s] = (float)d; b.primf[1][s] = (float)u; b.primf[2][s] = (float)v; b.primf[3][s] = (float)w;
+        b.primf[4][s] = (float)((p00 * d) * T); b.primf[5][s] = (float)T;
+    }
 }
 
 // get_viscous (sub_Residual.f90:1355-1383); muf keeps the value left after the turbulence
@@ -52,6 +56,27 @@ __device__ __forceinline__ double vt_of(const Blk& b, size_t s) { double v = b.U
 __device__ __forceinline__ double pw3(double x) { return (x * x) * x; }
 __device__ __forceinline__ double pw6(double x) { return ((((x * x) * x) * x) * x) * x; }   // flang expansion order
 
+// Storage of the compute type T of the residual: FP64 (the drop-in) or FP32 (the mixed mode's
+// reconstruction, fluxes and turbulence source, whose face fluxes the FP64 assembly reads).
+template <typename T> __device__ __forceinline__ const T* prim_of(const Blk& b, int m);
+template <> __device__ __forceinline__ const double* prim_of<double>(const Blk& b, int m) { return b.prim[m]; }
+template <> __device__ __forceinline__ const float* prim_of<float>(const Blk& b, int m) { return b.primf[m]; }
+__device__ __forceinline__ void put_F(const Blk& b, int d, int m, size_t s, double v) { b.F[d][m][s] = v; }
+__device__ __forceinline__ void put_F(const Blk& b, int d, int m, size_t s, float v) { b.Ff[d][m][s] = v; }
+__device__ __forceinline__ void put_Fv(const Blk& b, int d, int m, size_t s, double v) { b.Fv[d][m][s] = v; }
+__device__ __forceinline__ void put_Fv(const Blk& b, int d, int m, size_t s, float v) { b.Fvf[d][m][s] = v; }
+__device__ __forceinline__ void put_Fsa(const Blk& b, int d, int m, size_t s, double v) { b.Fsa[d][m][s] = v; }
+__device__ __forceinline__ void put_Fsa(const Blk& b, int d, int m, size_t s, float v) { b.Fsaf[d][m][s] = v; }
+template <typename T> __device__ __forceinline__ double get_F(const Blk& b, int d, int m, size_t s) {
+    if constexpr (sizeof(T) == 8) return b.F[d][m][s]; else return (double)b.Ff[d][m][s];
+}
+template <typename T> __device__ __forceinline__ double get_Fv(const Blk& b, int d, int m, size_t s) {
+    if constexpr (sizeof(T) == 8) return b.Fv[d][m][s]; else return (double)b.Fvf[d][m][s];
+}
+template <typename T> __device__ __forceinline__ double get_Fsa(const Blk& b, int d, int m, size_t s) {
+    if constexpr (sizeof(T) == 8) return b.Fsa[d][m][s]; else return (double)b.Fsaf[d][m][s];
+}
+
 // mu_t over 0..n with both limit_mut passes (sub_Residual.f90:1387-1408) folded in
 __global__ void __launch_bounds__(64) k_sa_mut(BSet S) {
     OCFD_LOCATE(SP_N0)
@@ -80,89 +105,92 @@ struct SAc {
 // nu-tilde face flux of the SA equation on face (fi,fj,fk) of direction DIR (I-1/2 convention):
 // fv = (-upwind convection + (1+cb2)/sigma (vt+mu/rho)/Re grad(vt).n) S and f2 = grad(vt).n S
 // (sub_turbulence_SA.f90:60-174)
-template <int DIR>
-__device__ __forceinline__ void sa_face(const Blk& b, const Par& p, int fi, int fj, int fk, double& fv, double& f2) {
+template <int DIR, typename T>
+__device__ __forceinline__ void sa_face(const Blk& b, const Par& p, int fi, int fj, int fk, T& fv, T& f2) {
     constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
-    const double c_v0 = (0.5 * (1.0 + SAc::Cb2)) / SAc::sigma;
-    const double Re = p.Re;
-    const double* d = b.prim[0];
-    const double *uu = b.prim[1], *vv = b.prim[2], *ww = b.prim[3];
+    const T c_v0 = T((0.5 * (1.0 + SAc::Cb2)) / SAc::sigma);
+    const T Re = T(p.Re);
+    const T* d = prim_of<T>(b, 0);
+    const T *uu = prim_of<T>(b, 1), *vv = prim_of<T>(b, 2), *ww = prim_of<T>(b, 3);
     size_t f = IX(b, fi, fj, fk), fm = IX(b, fi - di, fj - dj, fk - dk);
-    double s1x = b.nv[3 * DIR][f], s1y = b.nv[3 * DIR + 1][f], s1z = b.nv[3 * DIR + 2][f];
-    double gi, gj, gk;
-    auto VT = [&](int a, int bb, int cc) { return vt_of(b, IX(b, a, bb, cc)); };
+    T s1x = b.nv[3 * DIR][f], s1y = b.nv[3 * DIR + 1][f], s1z = b.nv[3 * DIR + 2][f];
+    T gi, gj, gk;
+    auto VT = [&](int a, int bb, int cc) { return T(vt_of(b, IX(b, a, bb, cc))); };
     if (DIR == 0) {
         gi = VT(fi, fj, fk) - VT(fi - 1, fj, fk);
-        gj = 0.25 * (((VT(fi, fj + 1, fk) - VT(fi, fj - 1, fk)) + VT(fi - 1, fj + 1, fk)) - VT(fi - 1, fj - 1, fk));
-        gk = 0.25 * (((VT(fi, fj, fk + 1) - VT(fi, fj, fk - 1)) + VT(fi - 1, fj, fk + 1)) - VT(fi - 1, fj, fk - 1));
+        gj = T(0.25) * (((VT(fi, fj + 1, fk) - VT(fi, fj - 1, fk)) + VT(fi - 1, fj + 1, fk)) - VT(fi - 1, fj - 1, fk));
+        gk = T(0.25) * (((VT(fi, fj, fk + 1) - VT(fi, fj, fk - 1)) + VT(fi - 1, fj, fk + 1)) - VT(fi - 1, fj, fk - 1));
     } else if (DIR == 1) {
-        gi = 0.25 * (((VT(fi + 1, fj, fk) - VT(fi - 1, fj, fk)) + VT(fi + 1, fj - 1, fk)) - VT(fi - 1, fj - 1, fk));
+        gi = T(0.25) * (((VT(fi + 1, fj, fk) - VT(fi - 1, fj, fk)) + VT(fi + 1, fj - 1, fk)) - VT(fi - 1, fj - 1, fk));
         gj = VT(fi, fj, fk) - VT(fi, fj - 1, fk);
-        gk = 0.25 * (((VT(fi, fj, fk + 1) - VT(fi, fj, fk - 1)) + VT(fi, fj - 1, fk + 1)) - VT(fi, fj - 1, fk - 1));
+        gk = T(0.25) * (((VT(fi, fj, fk + 1) - VT(fi, fj, fk - 1)) + VT(fi, fj - 1, fk + 1)) - VT(fi, fj - 1, fk - 1));
     } else {
-        gi = 0.25 * (((VT(fi + 1, fj, fk) - VT(fi - 1, fj, fk)) + VT(fi + 1, fj, fk - 1)) - VT(fi - 1, fj, fk - 1));
-        gj = 0.25 * (((VT(fi, fj + 1, fk) - VT(fi, fj - 1, fk)) + VT(fi, fj + 1, fk - 1)) - VT(fi, fj - 1, fk - 1));
+        gi = T(0.25) * (((VT(fi + 1, fj, fk) - VT(fi - 1, fj, fk)) + VT(fi + 1, fj, fk - 1)) - VT(fi - 1, fj, fk - 1));
+        gj = T(0.25) * (((VT(fi, fj + 1, fk) - VT(fi, fj - 1, fk)) + VT(fi, fj + 1, fk - 1)) - VT(fi, fj - 1, fk - 1));
         gk = VT(fi, fj, fk) - VT(fi, fj, fk - 1);
     }
     double* const* mt = b.met[DIR + 1];
-    double vtx = (gi * mt[0][f] + gj * mt[3][f]) + gk * mt[6][f];
-    double vty = (gi * mt[1][f] + gj * mt[4][f]) + gk * mt[7][f];
-    double vtz = (gi * mt[2][f] + gj * mt[5][f]) + gk * mt[8][f];
-    double v0 = c_v0 * (((vt_of(b, f) + (b.mu[f] * Re) / d[f]) + vt_of(b, fm)) + (b.mu[fm] * Re) / d[fm]);
-    double vn1 = (uu[fm] * s1x + vv[fm] * s1y) + ww[fm] * s1z;
-    double vn2 = (uu[f] * s1x + vv[f] * s1y) + ww[f] * s1z;
-    double vfi = 0.5 * ((vn1 + fabs(vn1)) * vt_of(b, fm) + (vn2 - fabs(vn2)) * vt_of(b, f));
-    double gn = (vtx * s1x + vty * s1y) + vtz * s1z;
-    double Sf = b.S[DIR][f];
+    auto M = [&](int q) { return T(mt[q][f]); };
+    T vtx = (gi * M(0) + gj * M(3)) + gk * M(6);
+    T vty = (gi * M(1) + gj * M(4)) + gk * M(7);
+    T vtz = (gi * M(2) + gj * M(5)) + gk * M(8);
+    T v0 = c_v0 * (((T(vt_of(b, f)) + (T(b.mu[f]) * Re) / d[f]) + T(vt_of(b, fm))) + (T(b.mu[fm]) * Re) / d[fm]);
+    T vn1 = (uu[fm] * s1x + vv[fm] * s1y) + ww[fm] * s1z;
+    T vn2 = (uu[f] * s1x + vv[f] * s1y) + ww[f] * s1z;
+    T vfi = T(0.5) * ((vn1 + fabs(vn1)) * T(vt_of(b, fm)) + (vn2 - fabs(vn2)) * T(vt_of(b, f)));
+    T gn = (vtx * s1x + vty * s1y) + vtz * s1z;
+    T Sf = T(b.S[DIR][f]);
     fv = (-vfi + (v0 / Re) * gn) * Sf;
     f2 = gn * Sf;
 }
 
 // Res(6) of cell s from its six stored face fluxes (i, j, k in that order) and the source in the
 // CFL3D form (sub_turbulence_SA.f90:103,138,174,181-249)
+template <typename T>
 __device__ __forceinline__ double sa_residual(const Blk& b, const Par& p, int i, int j, int k, size_t s) {
     const double Cw1 = SAc::Cb1 / (SAc::k * SAc::k) + (1.0 + SAc::Cb2) / SAc::sigma;
     const double Cv1_3 = pw3(7.1), Cw3_6 = pw6(SAc::Cw3);
     const double Re = p.Re;
     const double* d = b.prim[0];
-    const double *uu = b.prim[1], *vv = b.prim[2], *ww = b.prim[3];
+    const T* dt_ = prim_of<T>(b, 0);
+    const T *ut = prim_of<T>(b, 1), *vtt = prim_of<T>(b, 2), *wt = prim_of<T>(b, 3);
     double vt_c = vt_of(b, s), mu_c = b.mu[s] * Re;
     double v0c = ((vt_c + mu_c / d[s]) * SAc::Cb2) / SAc::sigma;
     const size_t sp[3] = {s + 1, s + (size_t)b.NI, s + (size_t)b.NI * b.NJ};
     double res = 0.0;
 #pragma unroll
     for (int dir = 0; dir < 3; dir++) {
-        double fv0 = b.Fsa[dir][0][s], fv1 = b.Fsa[dir][0][sp[dir]];
-        double f20 = b.Fsa[dir][1][s], f21 = b.Fsa[dir][1][sp[dir]];
+        double fv0 = get_Fsa<T>(b, dir, 0, s), fv1 = get_Fsa<T>(b, dir, 0, sp[dir]);
+        double f20 = get_Fsa<T>(b, dir, 1, s), f21 = get_Fsa<T>(b, dir, 1, sp[dir]);
         if (dir == 0) res = (fv1 - fv0) - (v0c / Re) * (f21 - f20);
         else res = ((res + fv1) - fv0) - (v0c / Re) * (f21 - f20);
     }
-    auto UU = [&](int a, int bb, int cc) { return uu[IX(b, a, bb, cc)]; };
-    auto VV = [&](int a, int bb, int cc) { return vv[IX(b, a, bb, cc)]; };
-    auto WW = [&](int a, int bb, int cc) { return ww[IX(b, a, bb, cc)]; };
-    double ui = UU(i + 1, j, k) - UU(i - 1, j, k), vi = VV(i + 1, j, k) - VV(i - 1, j, k), wi = WW(i + 1, j, k) - WW(i - 1, j, k);
-    double uj = UU(i, j + 1, k) - UU(i, j - 1, k), vj = VV(i, j + 1, k) - VV(i, j - 1, k), wj = WW(i, j + 1, k) - WW(i, j - 1, k);
-    double uk = UU(i, j, k + 1) - UU(i, j, k - 1), vk = VV(i, j, k + 1) - VV(i, j, k - 1), wk = WW(i, j, k + 1) - WW(i, j, k - 1);
+    auto UU = [&](int a, int bb, int cc) { return ut[IX(b, a, bb, cc)]; };
+    auto VV = [&](int a, int bb, int cc) { return vtt[IX(b, a, bb, cc)]; };
+    auto WW = [&](int a, int bb, int cc) { return wt[IX(b, a, bb, cc)]; };
+    T ui = UU(i + 1, j, k) - UU(i - 1, j, k), vi = VV(i + 1, j, k) - VV(i - 1, j, k), wi = WW(i + 1, j, k) - WW(i - 1, j, k);
+    T uj = UU(i, j + 1, k) - UU(i, j - 1, k), vj = VV(i, j + 1, k) - VV(i, j - 1, k), wj = WW(i, j + 1, k) - WW(i, j - 1, k);
+    T uk = UU(i, j, k + 1) - UU(i, j, k - 1), vk = VV(i, j, k + 1) - VV(i, j, k - 1), wk = WW(i, j, k + 1) - WW(i, j, k - 1);
     double* const* m0 = b.met[0];
-    double ix = m0[0][s], iy = m0[1][s], iz = m0[2][s], jx = m0[3][s], jy = m0[4][s], jz = m0[5][s], kx = m0[6][s], ky = m0[7][s], kz = m0[8][s];
-    double vx = (vi * ix + vj * jx) + vk * kx, wx = (wi * ix + wj * jx) + wk * kx;
-    double uy = (ui * iy + uj * jy) + uk * ky, wy = (wi * iy + wj * jy) + wk * ky;
-    double uz = (ui * iz + uj * jz) + uk * kz, vz = (vi * iz + vj * jz) + vk * kz;
-    double S = sqrt((sq(wy - vz) + sq(uz - wx)) + sq(vx - uy));
-    double dw = b.dw[s];
-    double X = d[s] * vt_c / mu_c;
-    double fv1 = pw3(X) / (pw3(X) + Cv1_3);
-    double fv2 = 1.0 - X / (1.0 + X * fv1);
-    double ft2 = SAc::Ct3 * exp(-((SAc::Ct4 * X) * X));
-    double S1 = S + (fv2 * vt_c) / (Re * sq(SAc::k * dw));
-    S1 = S1 > 0.0 ? S1 : 0.0;
-    double r = vt_c / (((((Re * S1) * SAc::k) * SAc::k) * dw) * dw);
-    if (r > 10.0) r = 10.0;
-    double g = r + SAc::Cw2 * (pw6(r) - r);
-    double fw = g * pow((1.0 + Cw3_6) / (pw6(g) + Cw3_6), 1.0 / 6.0);
-    double Q3 = (SAc::Cb1 * ((1.0 - ft2) * fv2 + ft2)) / (SAc::k * SAc::k) - Cw1 * fw;
-    double Q_SA = ((SAc::Cb1 * (1.0 - ft2)) * S1) * vt_c + (Q3 * sq(vt_c / dw)) / Re;
-    return res + Q_SA * b.vol[s];
+    T ix = T(m0[0][s]), iy = T(m0[1][s]), iz = T(m0[2][s]), jx = T(m0[3][s]), jy = T(m0[4][s]), jz = T(m0[5][s]), kx = T(m0[6][s]), ky = T(m0[7][s]), kz = T(m0[8][s]);
+    T vx = (vi * ix + vj * jx) + vk * kx, wx = (wi * ix + wj * jx) + wk * kx;
+    T uy = (ui * iy + uj * jy) + uk * ky, wy = (wi * iy + wj * jy) + wk * ky;
+    T uz = (ui * iz + uj * jz) + uk * kz, vz = (vi * iz + vj * jz) + vk * kz;
+    T S = sqrt((sq(wy - vz) + sq(uz - wx)) + sq(vx - uy));
+    T dw = T(b.dw[s]);
+    T X = dt_[s] * T(vt_c) / T(mu_c);
+    T fv1 = (X * X * X) / ((X * X * X) + T(Cv1_3));
+    T fv2 = T(1) - X / (T(1) + X * fv1);
+    T ft2 = T(SAc::Ct3) * exp(-((T(SAc::Ct4) * X) * X));
+    T S1 = S + (fv2 * T(vt_c)) / (T(Re) * sq(T(SAc::k) * dw));
+    S1 = S1 > T(0) ? S1 : T(0);
+    T r = T(vt_c) / (((((T(Re) * S1) * T(SAc::k)) * T(SAc::k)) * dw) * dw);
+    if (r > T(10)) r = T(10);
+    T g = r + T(SAc::Cw2) * ((((((r * r) * r) * r) * r) * r) - r);
+    T fw = g * pow((T(1) + T(Cw3_6)) / ((((((g * g) * g) * g) * g) * g) + T(Cw3_6)), T(1.0 / 6.0));
+    T Q3 = (T(SAc::Cb1) * ((T(1) - ft2) * fv2 + ft2)) / T(SAc::k * SAc::k) - T(Cw1) * fw;
+    T Q_SA = ((T(SAc::Cb1) * (T(1) - ft2)) * S1) * T(vt_c) + (Q3 * sq(T(vt_c) / dw)) / T(Re);
+    return res + (double)(Q_SA * T(b.vol[s]));
 }
 
 // Amut_boundary (sub_turbulence_SA.f90:282-368) on one wall subface
@@ -318,7 +346,7 @@ __global__ void __launch_bounds__(64) k_sst_finish(BSet S) {
 
 // ---------------------------------------------------------------------------------------------
 // flux_inviscous_i/j/k (sub_Residual.f90:182-705): one thread per face, face flux stored
-template <int DIR>
+template <int DIR, typename T>
 __global__ void __launch_bounds__(64) k_inv_flux(BSet S) {
     OCFD_LOCATE(SP_FI + DIR)
     size_t s = IX(b, i, j, k);
@@ -329,93 +357,95 @@ __global__ void __launch_bounds__(64) k_inv_flux(BSet S) {
     if (fl & 1) { if (pos == 1) scheme = S_CD2; else if (pos < LAP) scheme = p.bscheme; }
     if (fl & 2) { if (pos == nd) scheme = S_CD2; else if (pos > nd - LAP + 1) scheme = p.bscheme; }
     if (b.flags[0] == 1) scheme = S_UD1;
-    double s1x = b.nv[3 * DIR][s], s1y = b.nv[3 * DIR + 1][s], s1z = b.nv[3 * DIR + 2][s];
-    double t1x, t1y, t1z;
-    if (DIR == 0) { size_t a = IX(b, i, j + 1, k), c = IX(b, i, j, k + 1); t1x = b.x[a] - b.x[c]; t1y = b.y[a] - b.y[c]; t1z = b.z[a] - b.z[c]; }
-    else if (DIR == 1) { size_t a = IX(b, i + 1, j, k + 1); t1x = b.x[a] - b.x[s]; t1y = b.y[a] - b.y[s]; t1z = b.z[a] - b.z[s]; }
-    else { size_t a = IX(b, i + 1, j + 1, k); t1x = b.x[a] - b.x[s]; t1y = b.y[a] - b.y[s]; t1z = b.z[a] - b.z[s]; }
-    double t10 = 1.0 / sqrt((t1x * t1x + t1y * t1y) + t1z * t1z);
-    double s2x = t1x * t10, s2y = t1y * t10, s2z = t1z * t10;
-    double s3x = s1y * s2z - s1z * s2y, s3y = s1z * s2x - s1x * s2z, s3z = s1x * s2y - s1y * s2x;
+    T s1x = T(b.nv[3 * DIR][s]), s1y = T(b.nv[3 * DIR + 1][s]), s1z = T(b.nv[3 * DIR + 2][s]);
+    T t1x, t1y, t1z;
+    if (DIR == 0) { size_t a = IX(b, i, j + 1, k), c = IX(b, i, j, k + 1); t1x = T(b.x[a] - b.x[c]); t1y = T(b.y[a] - b.y[c]); t1z = T(b.z[a] - b.z[c]); }
+    else if (DIR == 1) { size_t a = IX(b, i + 1, j, k + 1); t1x = T(b.x[a] - b.x[s]); t1y = T(b.y[a] - b.y[s]); t1z = T(b.z[a] - b.z[s]); }
+    else { size_t a = IX(b, i + 1, j + 1, k); t1x = T(b.x[a] - b.x[s]); t1y = T(b.y[a] - b.y[s]); t1z = T(b.z[a] - b.z[s]); }
+    T t10 = T(1) / sqrt((t1x * t1x + t1y * t1y) + t1z * t1z);
+    T s2x = t1x * t10, s2y = t1y * t10, s2z = t1z * t10;
+    T s3x = s1y * s2z - s1z * s2y, s3y = s1z * s2x - s1x * s2z, s3z = s1x * s2y - s1y * s2x;
     const long st = DIR == 0 ? 1 : DIR == 1 ? (long)b.NI : (long)b.NI * b.NJ;
-    double UL[5], UR[5];
+    T UL[5], UR[5];
     if (p.recon == 0) {
         for (int m = 0; m < 5; m++) {
-            const double* a = (m < 4 ? b.prim[m] : b.prim[4]) + s;
-            double u[8];
+            const T* a = prim_of<T>(b, m) + s;
+            T u[8];
 #pragma unroll
             for (int q = 0; q < 8; q++) u[q] = a[(q - 4) * st];     // u[q] = cell (face - 4 + q): u(q-3)
             UL[m] = scheme_fP(u + 3, scheme);
             UR[m] = scheme_fm(u + 3, scheme);
         }
     } else {
-        double U0[5][8];
+        T U0[5][8];
         for (int m = 0; m < 5; m++) {
             const double* a = b.U[m] + s;
 #pragma unroll
-            for (int q = 0; q < 8; q++) U0[m][q] = a[(q - 4) * st];
+            for (int q = 0; q < 8; q++) U0[m][q] = T(a[(q - 4) * st]);
         }
-        double QL[5], QR[5];
+        T QL[5], QR[5];
         if (p.recon == 1) {
             for (int m = 0; m < 5; m++) { QL[m] = scheme_fP(U0[m] + 3, scheme); QR[m] = scheme_fm(U0[m] + 3, scheme); }
         } else {
             // Reconstuction_Characteristic (sub_Residual.f90:1291-1347)
-            double gamma = p.gamma, Uh[5];
-            for (int m = 0; m < 5; m++) Uh[m] = 0.5 * (U0[m][3] + U0[m][4]);
-            double d1 = Uh[0], u1 = Uh[1] / d1, v1 = Uh[2] / d1, w1 = Uh[3] / d1;
-            double p1 = (Uh[4] - ((Uh[1] * u1 + Uh[2] * v1) + Uh[3] * w1) * 0.5) * (gamma - 1.0);
-            double c1 = sqrt(gamma * p1 / d1);
-            double V2 = ((u1 * u1 + v1 * v1) + w1 * w1) * 0.5;
-            double tmp1 = (gamma - 1.0) / c1, tmp3 = (gamma - 1.0) / (c1 * c1), tmp5 = 1.0 / (2.0 * c1), tmp0 = 1.0 / tmp3;
-            double S[5][5] = {{V2 - tmp0, -u1, -v1, -w1, 1.0}, {-v1, 0.0, 1.0, 0.0, 0.0}, {-w1, 0.0, 0.0, 1.0, 0.0},
-                              {-u1 - V2 * tmp1, 1.0 + tmp1 * u1, tmp1 * v1, tmp1 * w1, -tmp1},
-                              {-u1 + V2 * tmp1, 1.0 - tmp1 * u1, -tmp1 * v1, -tmp1 * w1, tmp1}};
-            double S1[5][5] = {{-tmp3, 0.0, 0.0, -tmp5, tmp5}, {-tmp3 * u1, 0.0, 0.0, 0.5 - u1 * tmp5, 0.5 + u1 * tmp5},
-                               {-tmp3 * v1, 1.0, 0.0, -v1 * tmp5, v1 * tmp5}, {-tmp3 * w1, 0.0, 1.0, -w1 * tmp5, w1 * tmp5},
-                               {-tmp3 * V2, v1, w1, ((c1 * u1 - V2) - tmp0) * tmp5, ((c1 * u1 + V2) + tmp0) * tmp5}};
-            double VL[5], VR[5];
+            T gamma = T(p.gamma), Uh[5];
+            for (int m = 0; m < 5; m++) Uh[m] = T(0.5) * (U0[m][3] + U0[m][4]);
+            T d1 = Uh[0], u1 = Uh[1] / d1, v1 = Uh[2] / d1, w1 = Uh[3] / d1;
+            T p1 = (Uh[4] - ((Uh[1] * u1 + Uh[2] * v1) + Uh[3] * w1) * T(0.5)) * (gamma - T(1));
+            T c1 = sqrt(gamma * p1 / d1);
+            T V2 = ((u1 * u1 + v1 * v1) + w1 * w1) * T(0.5);
+            T tmp1 = (gamma - T(1)) / c1, tmp3 = (gamma - T(1)) / (c1 * c1), tmp5 = T(1) / (T(2) * c1), tmp0 = T(1) / tmp3;
+            const T o = T(1), z = T(0), h = T(0.5);
+            T S[5][5] = {{V2 - tmp0, -u1, -v1, -w1, o}, {-v1, z, o, z, z}, {-w1, z, z, o, z},
+                         {-u1 - V2 * tmp1, o + tmp1 * u1, tmp1 * v1, tmp1 * w1, -tmp1},
+                         {-u1 + V2 * tmp1, o - tmp1 * u1, -tmp1 * v1, -tmp1 * w1, tmp1}};
+            T S1[5][5] = {{-tmp3, z, z, -tmp5, tmp5}, {-tmp3 * u1, z, z, h - u1 * tmp5, h + u1 * tmp5},
+                          {-tmp3 * v1, o, z, -v1 * tmp5, v1 * tmp5}, {-tmp3 * w1, z, o, -w1 * tmp5, w1 * tmp5},
+                          {-tmp3 * V2, v1, w1, ((c1 * u1 - V2) - tmp0) * tmp5, ((c1 * u1 + V2) + tmp0) * tmp5}};
+            T VL[5], VR[5];
             for (int m = 0; m < 5; m++) {
-                double V0[8];
-                for (int q = 0; q < 8; q++) { double a = 0.0; for (int jj = 0; jj < 5; jj++) a = a + S[m][jj] * U0[jj][q]; V0[q] = a; }
+                T V0[8];
+                for (int q = 0; q < 8; q++) { T a = T(0); for (int jj = 0; jj < 5; jj++) a = a + S[m][jj] * U0[jj][q]; V0[q] = a; }
                 VL[m] = scheme_fP(V0 + 3, scheme); VR[m] = scheme_fm(V0 + 3, scheme);
             }
             for (int m = 0; m < 5; m++) {
-                double a = 0.0, c = 0.0;
+                T a = T(0), c = T(0);
                 for (int jj = 0; jj < 5; jj++) { a = a + S1[m][jj] * VL[jj]; c = c + S1[m][jj] * VR[jj]; }
                 QL[m] = a; QR[m] = c;
             }
         }
-        double g = p.gamma;
+        T g = T(p.gamma);
         UL[0] = QL[0]; UL[1] = QL[1] / UL[0]; UL[2] = QL[2] / UL[0]; UL[3] = QL[3] / UL[0];
-        UL[4] = (QL[4] - ((UL[1] * QL[1] + UL[2] * QL[2]) + UL[3] * QL[3]) * 0.5) * (g - 1.0);
+        UL[4] = (QL[4] - ((UL[1] * QL[1] + UL[2] * QL[2]) + UL[3] * QL[3]) * T(0.5)) * (g - T(1));
         UR[0] = QR[0]; UR[1] = QR[1] / UR[0]; UR[2] = QR[2] / UR[0]; UR[3] = QR[3] / UR[0];
-        UR[4] = (QR[4] - ((UR[1] * QR[1] + UR[2] * QR[2]) + UR[3] * QR[3]) * 0.5) * (g - 1.0);
+        UR[4] = (QR[4] - ((UR[1] * QR[1] + UR[2] * QR[2]) + UR[3] * QR[3]) * T(0.5)) * (g - T(1));
     }
     if (p.positivity == 1) {
-        if (UL[0] <= 1.0e-20 || UL[4] <= 1.0e-20) { size_t t = s - st; for (int m = 0; m < 5; m++) UL[m] = b.prim[m][t]; }
-        if (UR[0] <= 1.0e-20 || UR[4] <= 1.0e-20) { for (int m = 0; m < 5; m++) UR[m] = b.prim[m][s]; }
+        if (UL[0] <= T(1.0e-20) || UL[4] <= T(1.0e-20)) { size_t t = s - st; for (int m = 0; m < 5; m++) UL[m] = prim_of<T>(b, m)[t]; }
+        if (UR[0] <= T(1.0e-20) || UR[4] <= T(1.0e-20)) { for (int m = 0; m < 5; m++) UR[m] = prim_of<T>(b, m)[s]; }
     }
-    double QL[5], QR[5], F0[5];
+    T QL[5], QR[5], F0[5];
     QL[0] = UL[0]; QL[1] = (UL[1] * s1x + UL[2] * s1y) + UL[3] * s1z; QL[2] = (UL[1] * s2x + UL[2] * s2y) + UL[3] * s2z;
     QL[3] = (UL[1] * s3x + UL[2] * s3y) + UL[3] * s3z; QL[4] = UL[4];
     QR[0] = UR[0]; QR[1] = (UR[1] * s1x + UR[2] * s1y) + UR[3] * s1z; QR[2] = (UR[1] * s2x + UR[2] * s2y) + UR[3] * s2z;
     QR[3] = (UR[1] * s3x + UR[2] * s3y) + UR[3] * s3z; QR[4] = UR[4];
-    riemann(QL, QR, F0, p.gamma, p.flux);
-    double Sf = b.S[DIR][s];
-    b.F[DIR][0][s] = -F0[0] * Sf;
-    b.F[DIR][1][s] = -((F0[1] * s1x + F0[2] * s2x) + F0[3] * s3x) * Sf;
-    b.F[DIR][2][s] = -((F0[1] * s1y + F0[2] * s2y) + F0[3] * s3y) * Sf;
-    b.F[DIR][3][s] = -((F0[1] * s1z + F0[2] * s2z) + F0[3] * s3z) * Sf;
-    b.F[DIR][4][s] = -F0[4] * Sf;
+    riemann(QL, QR, F0, T(p.gamma), p.flux);
+    T Sf = T(b.S[DIR][s]);
+    put_F(b, DIR, 0, s, -F0[0] * Sf);
+    put_F(b, DIR, 1, s, -((F0[1] * s1x + F0[2] * s2x) + F0[3] * s3x) * Sf);
+    put_F(b, DIR, 2, s, -((F0[1] * s1y + F0[2] * s2y) + F0[3] * s3y) * Sf);
+    put_F(b, DIR, 3, s, -((F0[1] * s1z + F0[2] * s2z) + F0[3] * s3z) * Sf);
+    put_F(b, DIR, 4, s, -F0[4] * Sf);
     if (p.turb == 2) {
-        double fv, f2;
-        sa_face<DIR>(b, p, i, j, k, fv, f2);
-        b.Fsa[DIR][0][s] = fv;
-        b.Fsa[DIR][1][s] = f2;
+        T fv, f2;
+        sa_face<DIR, T>(b, p, i, j, k, fv, f2);
+        put_Fsa(b, DIR, 0, s, fv);
+        put_Fsa(b, DIR, 1, s, f2);
     }
 }
 
 // explicit average-state flux on a physical k subface (sub_Residual.f90:667-688)
+template <typename T>
 __global__ void __launch_bounds__(64) k_inv_kbc(SubSet S) {
     OCFD_SUB_LOCATE
     int i = B.ib + a, j = B.jb + c, k = B.kb;
@@ -428,19 +458,19 @@ __global__ void __launch_bounds__(64) k_inv_kbc(SubSet S) {
     double n1 = b.nv[6][s], n2 = b.nv[7][s], n3 = b.nv[8][s];
     double un = (uu0 * n1 + v0 * n2) + w0 * n3;
     double Sk = b.S[2][s];
-    b.F[2][0][s] = -(d0 * un) * Sk;
-    b.F[2][1][s] = -((d0 * un) * uu0 + p0 * n1) * Sk;
-    b.F[2][2][s] = -((d0 * un) * v0 + p0 * n2) * Sk;
-    b.F[2][3][s] = -((d0 * un) * w0 + p0 * n3) * Sk;
-    b.F[2][4][s] = -((E0 + p0) * un) * Sk;
+    put_F(b, 2, 0, s, T(-(d0 * un) * Sk));
+    put_F(b, 2, 1, s, T(-((d0 * un) * uu0 + p0 * n1) * Sk));
+    put_F(b, 2, 2, s, T(-((d0 * un) * v0 + p0 * n2) * Sk));
+    put_F(b, 2, 3, s, T(-((d0 * un) * w0 + p0 * n3) * Sk));
+    put_F(b, 2, 4, s, T(-((E0 + p0) * un) * Sk));
 }
 
 // flux_viscous_i/j/k (sub_Residual.f90:712-1247)
-template <int DIR>
+template <int DIR, typename T>
 __global__ void __launch_bounds__(64) k_visc_flux(BSet S) {
     constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
     OCFD_LOCATE(SP_FI + DIR)
-    const double tmp1 = 4.0 / 3.0, tmp2 = 2.0 / 3.0;
+    const T tmp1 = T(4.0 / 3.0), tmp2 = T(2.0 / 3.0);
     size_t s = IX(b, i, j, k);
     if (b.flags[0] == 1) {      // over-limit block: no viscous flux, zero wall forces (:734-750)
         int pos0 = DIR == 0 ? i : DIR == 1 ? j : k, nd0 = DIR == 0 ? b.nx : DIR == 1 ? b.ny : b.nz;
@@ -453,7 +483,7 @@ __global__ void __launch_bounds__(64) k_visc_flux(BSet S) {
     }
     int im = i - di, jm = j - dj, km = k - dk;
     size_t sm = IX(b, im, jm, km);
-    const double* Q[4] = {b.prim[1], b.prim[2], b.prim[3], b.prim[5]};
+    const T* Q[4] = {prim_of<T>(b, 1), prim_of<T>(b, 2), prim_of<T>(b, 3), prim_of<T>(b, 5)};
     // tangential directions: i-faces (j,k), j-faces (i,k), k-faces (i,j)
     constexpr int ai = DIR == 0 ? 0 : 1, aj = DIR == 0 ? 1 : 0;          // first tangential unit step
     constexpr int bj = DIR == 2 ? 1 : 0, bk = DIR == 2 ? 0 : 1;          // second tangential unit step
@@ -463,49 +493,50 @@ __global__ void __launch_bounds__(64) k_visc_flux(BSet S) {
     int posB = DIR == 2 ? j : k, nB = DIR == 2 ? b.ny : b.nz;
     bool zeroA = (pos == 1 || pos == nd) && (posA == 1 || posA == nA - 1);
     bool zeroB = (pos == 1 || pos == nd) && (posB == 1 || posB == nB - 1);
-    double gi[4], gj[4], gk[4];
+    T gi[4], gj[4], gk[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const double* a = Q[q];
-        double dn = a[s] - a[sm];
-        double ta1 = 0.0, ta2 = 0.0, tb1 = 0.0, tb2 = 0.0;
+        const T* a = Q[q];
+        T dn = a[s] - a[sm];
+        T ta1 = T(0), ta2 = T(0), tb1 = T(0), tb2 = T(0);
         if (!zeroA) {
-            ta1 = 0.5 * (a[IX(b, i - ai, j - aj, k)] + a[IX(b, im - ai, jm - aj, km)]);
-            ta2 = 0.5 * (a[IX(b, i + ai, j + aj, k)] + a[IX(b, im + ai, jm + aj, km)]);
+            ta1 = T(0.5) * (a[IX(b, i - ai, j - aj, k)] + a[IX(b, im - ai, jm - aj, km)]);
+            ta2 = T(0.5) * (a[IX(b, i + ai, j + aj, k)] + a[IX(b, im + ai, jm + aj, km)]);
         }
         if (!zeroB) {
-            tb1 = 0.5 * (a[IX(b, i, j - bj, k - bk)] + a[IX(b, im, jm - bj, km - bk)]);
-            tb2 = 0.5 * (a[IX(b, i, j + bj, k + bk)] + a[IX(b, im, jm + bj, km + bk)]);
+            tb1 = T(0.5) * (a[IX(b, i, j - bj, k - bk)] + a[IX(b, im, jm - bj, km - bk)]);
+            tb2 = T(0.5) * (a[IX(b, i, j + bj, k + bk)] + a[IX(b, im, jm + bj, km + bk)]);
         }
-        double ga = 0.5 * (ta2 - ta1), gb = 0.5 * (tb2 - tb1);
+        T ga = T(0.5) * (ta2 - ta1), gb = T(0.5) * (tb2 - tb1);
         if (DIR == 0) { gi[q] = dn; gj[q] = ga; gk[q] = gb; }
         else if (DIR == 1) { gi[q] = ga; gj[q] = dn; gk[q] = gb; }
         else { gi[q] = ga; gj[q] = gb; gk[q] = dn; }
     }
     double* const* mt = b.met[DIR + 1];
-    double ix = mt[0][s], iy = mt[1][s], iz = mt[2][s], jx = mt[3][s], jy = mt[4][s], jz = mt[5][s], kx = mt[6][s], ky = mt[7][s], kz = mt[8][s];
-    double ux = (gi[0] * ix + gj[0] * jx) + gk[0] * kx, vx = (gi[1] * ix + gj[1] * jx) + gk[1] * kx;
-    double wx = (gi[2] * ix + gj[2] * jx) + gk[2] * kx, Tx = (gi[3] * ix + gj[3] * jx) + gk[3] * kx;
-    double uy = (gi[0] * iy + gj[0] * jy) + gk[0] * ky, vy = (gi[1] * iy + gj[1] * jy) + gk[1] * ky;
-    double wy = (gi[2] * iy + gj[2] * jy) + gk[2] * ky, Ty = (gi[3] * iy + gj[3] * jy) + gk[3] * ky;
-    double uz = (gi[0] * iz + gj[0] * jz) + gk[0] * kz, vz = (gi[1] * iz + gj[1] * jz) + gk[1] * kz;
-    double wz = (gi[2] * iz + gj[2] * jz) + gk[2] * kz, Tz = (gi[3] * iz + gj[3] * jz) + gk[3] * kz;
-    double mua = b.muf[s], mub = b.muf[sm], mta = b.mut[s], mtb = b.mut[sm];
-    double mu0 = 0.5 * (((mua + mta) + mub) + mtb);
-    double k0 = (0.5 * p.Cp) * (((mua / p.PrL + mta / p.PrT) + mub / p.PrL) + mtb / p.PrT);
-    double s11 = mu0 * (tmp1 * ux - tmp2 * (vy + wz)), s12 = mu0 * (uy + vx), s13 = mu0 * (uz + wx);
-    double s22 = mu0 * (tmp1 * vy - tmp2 * (ux + wz)), s23 = mu0 * (vz + wy), s33 = mu0 * (tmp1 * wz - tmp2 * (ux + vy));
-    double u1 = (Q[0][s] + Q[0][sm]) * 0.5, v1 = (Q[1][s] + Q[1][sm]) * 0.5, w1 = (Q[2][s] + Q[2][sm]) * 0.5;
-    double E1 = ((u1 * s11 + v1 * s12) + w1 * s13) + k0 * Tx;
-    double E2 = ((u1 * s12 + v1 * s22) + w1 * s23) + k0 * Ty;
-    double E3 = ((u1 * s13 + v1 * s23) + w1 * s33) + k0 * Tz;
-    double s1x = b.nv[3 * DIR][s], s1y = b.nv[3 * DIR + 1][s], s1z = b.nv[3 * DIR + 2][s];
-    double Sf = b.S[DIR][s];
-    double f1 = ((s11 * s1x + s12 * s1y) + s13 * s1z) * Sf;
-    double f2 = ((s12 * s1x + s22 * s1y) + s23 * s1z) * Sf;
-    double f3 = ((s13 * s1x + s23 * s1y) + s33 * s1z) * Sf;
-    b.Fv[DIR][0][s] = f1; b.Fv[DIR][1][s] = f2; b.Fv[DIR][2][s] = f3;
-    b.Fv[DIR][3][s] = ((E1 * s1x + E2 * s1y) + E3 * s1z) * Sf;
+    T ix = T(mt[0][s]), iy = T(mt[1][s]), iz = T(mt[2][s]), jx = T(mt[3][s]), jy = T(mt[4][s]), jz = T(mt[5][s]), kx = T(mt[6][s]), ky = T(mt[7][s]), kz = T(mt[8][s]);
+    T ux = (gi[0] * ix + gj[0] * jx) + gk[0] * kx, vx = (gi[1] * ix + gj[1] * jx) + gk[1] * kx;
+    T wx = (gi[2] * ix + gj[2] * jx) + gk[2] * kx, Tx = (gi[3] * ix + gj[3] * jx) + gk[3] * kx;
+    T uy = (gi[0] * iy + gj[0] * jy) + gk[0] * ky, vy = (gi[1] * iy + gj[1] * jy) + gk[1] * ky;
+    T wy = (gi[2] * iy + gj[2] * jy) + gk[2] * ky, Ty = (gi[3] * iy + gj[3] * jy) + gk[3] * ky;
+    T uz = (gi[0] * iz + gj[0] * jz) + gk[0] * kz, vz = (gi[1] * iz + gj[1] * jz) + gk[1] * kz;
+    T wz = (gi[2] * iz + gj[2] * jz) + gk[2] * kz, Tz = (gi[3] * iz + gj[3] * jz) + gk[3] * kz;
+    T mua = T(b.muf[s]), mub = T(b.muf[sm]), mta = T(b.mut[s]), mtb = T(b.mut[sm]);
+    T mu0 = T(0.5) * (((mua + mta) + mub) + mtb);
+    T PrL = T(p.PrL), PrT = T(p.PrT);
+    T k0 = (T(0.5) * T(p.Cp)) * (((mua / PrL + mta / PrT) + mub / PrL) + mtb / PrT);
+    T s11 = mu0 * (tmp1 * ux - tmp2 * (vy + wz)), s12 = mu0 * (uy + vx), s13 = mu0 * (uz + wx);
+    T s22 = mu0 * (tmp1 * vy - tmp2 * (ux + wz)), s23 = mu0 * (vz + wy), s33 = mu0 * (tmp1 * wz - tmp2 * (ux + vy));
+    T u1 = (Q[0][s] + Q[0][sm]) * T(0.5), v1 = (Q[1][s] + Q[1][sm]) * T(0.5), w1 = (Q[2][s] + Q[2][sm]) * T(0.5);
+    T E1 = ((u1 * s11 + v1 * s12) + w1 * s13) + k0 * Tx;
+    T E2 = ((u1 * s12 + v1 * s22) + w1 * s23) + k0 * Ty;
+    T E3 = ((u1 * s13 + v1 * s23) + w1 * s33) + k0 * Tz;
+    T s1x = T(b.nv[3 * DIR][s]), s1y = T(b.nv[3 * DIR + 1][s]), s1z = T(b.nv[3 * DIR + 2][s]);
+    T Sf = T(b.S[DIR][s]);
+    T f1 = ((s11 * s1x + s12 * s1y) + s13 * s1z) * Sf;
+    T f2 = ((s12 * s1x + s22 * s1y) + s23 * s1z) * Sf;
+    T f3 = ((s13 * s1x + s23 * s1y) + s33 * s1z) * Sf;
+    put_Fv(b, DIR, 0, s, f1); put_Fv(b, DIR, 1, s, f2); put_Fv(b, DIR, 2, s, f3);
+    put_Fv(b, DIR, 3, s, ((E1 * s1x + E2 * s1y) + E3 * s1z) * Sf);
     // Surf1..6: viscous force on the first and last face of each direction
     if (pos == 1 || pos == nd) {
         int f = DIR + (pos == 1 ? 0 : 3);
@@ -517,6 +548,7 @@ __global__ void __launch_bounds__(64) k_visc_flux(BSet S) {
 // Res(1:5): inviscid i, j, k then viscous i, j, k face differences, in the reference order
 // (sub_Residual.f90:346-354,507-516,692-701,864-872,1039-1048,1219-1229)
 // followed by Res(6) of SA, then comput_Lijk + comput_dt (sub_time_acceleraction.f90:7-112)
+template <typename T>
 __global__ void __launch_bounds__(64) k_cell(BSet S) {
     OCFD_LOCATE(SP_CELL)
     size_t s = IX(b, i, j, k);
@@ -524,17 +556,17 @@ __global__ void __launch_bounds__(64) k_cell(BSet S) {
     bool visc = p.visc == 1 && b.flags[0] != 1;
 #pragma unroll
     for (int m = 0; m < 5; m++) {
-        double r = b.F[0][m][sp[0]] - b.F[0][m][s];
-        r = (r + b.F[1][m][sp[1]]) - b.F[1][m][s];
-        r = (r + b.F[2][m][sp[2]]) - b.F[2][m][s];
+        double r = get_F<T>(b, 0, m, sp[0]) - get_F<T>(b, 0, m, s);
+        r = (r + get_F<T>(b, 1, m, sp[1])) - get_F<T>(b, 1, m, s);
+        r = (r + get_F<T>(b, 2, m, sp[2])) - get_F<T>(b, 2, m, s);
         if (visc && m > 0) {
-            r = (r + b.Fv[0][m - 1][sp[0]]) - b.Fv[0][m - 1][s];
-            r = (r + b.Fv[1][m - 1][sp[1]]) - b.Fv[1][m - 1][s];
-            r = (r + b.Fv[2][m - 1][sp[2]]) - b.Fv[2][m - 1][s];
+            r = (r + get_Fv<T>(b, 0, m - 1, sp[0])) - get_Fv<T>(b, 0, m - 1, s);
+            r = (r + get_Fv<T>(b, 1, m - 1, sp[1])) - get_Fv<T>(b, 1, m - 1, s);
+            r = (r + get_Fv<T>(b, 2, m - 1, sp[2])) - get_Fv<T>(b, 2, m - 1, s);
         }
         b.Res[m][s] = r;
     }
-    if (p.turb == 2) b.Res[5][s] = sa_residual(b, p, i, j, k, s);
+    if (p.turb == 2) b.Res[5][s] = sa_residual<T>(b, p, i, j, k, s);
     const double g43 = p.gamma > (double)(4.0f / 3.0f) ? p.gamma : (double)(4.0f / 3.0f);
     double d = b.prim[0][s], u = b.prim[1][s], v = b.prim[2][s], w = b.prim[3][s], c = b.prim[6][s];
     double D0 = p.visc == 1 ? (g43 / d) * (b.muf[s] / p.PrL + b.mut[s] / p.PrT) : 0.0;
@@ -1088,20 +1120,30 @@ void set_pre(const SetPlan& P, hipStream_t st) {
     }
 }
 
+template <typename T>
+static void inviscid_t(const SetPlan& P, hipStream_t st) {
+    launch_space<SP_FI>(k_inv_flux<0, T>, P, st);
+    launch_space<SP_FJ>(k_inv_flux<1, T>, P, st);
+    launch_space<SP_FK>(k_inv_flux<2, T>, P, st);
+    launch_subs(k_inv_kbc<T>, P, P.kbc, st);
+}
 void set_inviscid(const SetPlan& P, hipStream_t st) {
-    launch_space<SP_FI>(k_inv_flux<0>, P, st);
-    launch_space<SP_FJ>(k_inv_flux<1>, P, st);
-    launch_space<SP_FK>(k_inv_flux<2>, P, st);
-    launch_subs(k_inv_kbc, P, P.kbc, st);
+    if (P.p0.mixed) inviscid_t<float>(P, st);
+    else inviscid_t<double>(P, st);
 }
 
-void set_viscous_dt(const SetPlan& P, hipStream_t st) {
+template <typename T>
+static void viscous_dt_t(const SetPlan& P, hipStream_t st) {
     if (P.p0.visc == 1) {
-        launch_space<SP_FI>(k_visc_flux<0>, P, st);
-        launch_space<SP_FJ>(k_visc_flux<1>, P, st);
-        launch_space<SP_FK>(k_visc_flux<2>, P, st);
+        launch_space<SP_FI>(k_visc_flux<0, T>, P, st);
+        launch_space<SP_FJ>(k_visc_flux<1, T>, P, st);
+        launch_space<SP_FK>(k_visc_flux<2, T>, P, st);
     }
-    launch_space<SP_CELL>(k_cell, P, st);
+    launch_space<SP_CELL>(k_cell<T>, P, st);
+}
+void set_viscous_dt(const SetPlan& P, hipStream_t st) {
+    if (P.p0.mixed) viscous_dt_t<float>(P, st);
+    else viscous_dt_t<double>(P, st);
 }
 
 void set_update_limit(const SetPlan& P, hipStream_t st) {

@@ -141,6 +141,19 @@ int alloc_block(ocfd_ctx* c, HostBlock& hb) {
     for (int dd = 0; dd < 3; dd++) for (int m = 0; m < 5; m++) if ((r = dalloc(hb, &b.F[dd][m], n))) return r;
     if (visc) for (int dd = 0; dd < 3; dd++) for (int m = 0; m < 4; m++) if ((r = dalloc(hb, &b.Fv[dd][m], n))) return r;
     if (visc && c->p.turb == 2) for (int dd = 0; dd < 3; dd++) for (int m = 0; m < 2; m++) if ((r = dalloc(hb, &b.Fsa[dd][m], n))) return r;
+    if (c->p.mixed) {
+        // FP32 primitives in half-size slots; FP32 face terms in the first half of the FP64 arrays
+        for (int q = 0; q < 6; q++) {
+            double* a = nullptr;
+            if ((r = dalloc(hb, &a, (n + 1) / 2))) return r;
+            b.primf[q] = reinterpret_cast<float*>(a);
+        }
+        for (int dd = 0; dd < 3; dd++) {
+            for (int m = 0; m < 5; m++) b.Ff[dd][m] = reinterpret_cast<float*>(b.F[dd][m]);
+            for (int m = 0; m < 4; m++) b.Fvf[dd][m] = reinterpret_cast<float*>(b.Fv[dd][m]);
+            for (int m = 0; m < 2; m++) b.Fsaf[dd][m] = reinterpret_cast<float*>(b.Fsa[dd][m]);
+        }
+    }
     size_t jk = (size_t)b.NJ * b.NK, ik = (size_t)b.NI * b.NK, ij = (size_t)b.NI * b.NJ;
     if ((r = balloc(hb, &b.bcf[0], jk)) || (r = balloc(hb, &b.bcf[1], ik)) || (r = balloc(hb, &b.bcf[2], ij))) return r;
     size_t fsz[6] = {jk, ik, ij, jk, ik, ij};
@@ -196,6 +209,7 @@ Par to_par(const ocfd_params& h) {
     p.scheme = h.Iflag_Scheme; p.flux = h.Iflag_Flux; p.recon = h.IFlag_Reconstruction; p.bscheme = h.Bound_Scheme;
     p.local_dt = h.Iflag_local_dt; p.dtime_mesh = h.If_dtime_mesh; p.positivity = h.IF_Scheme_Positivity;
     p.limit_flow = h.IFLAG_LIMIT_FLOW;
+    p.mixed = h.precision == 1;
     return p;
 }
 
@@ -566,7 +580,9 @@ int ocfd_create(const ocfd_params* p, ocfd_ctx** out) {
         return fail(OCFD_EUNSUPPORTED, "turbulence model outside SA (2) and SST (3)");
     if (p->Iflag_Scheme == 9 || p->Bound_Scheme == 9) return fail(OCFD_EUNSUPPORTED, "WENO7 reconstruction is not on this path");
     if (p->Iflag_Flux < 1 || p->Iflag_Flux > 6) return fail(OCFD_EARG, "Iflag_Flux outside 1..6");
-    if (p->precision != 0) return fail(OCFD_EUNSUPPORTED, "precision mode other than all-FP64 is selected through ocfd_set_precision");
+    if (p->precision != 0 && p->precision != 1) return fail(OCFD_EARG, "precision must be 0 (all FP64) or 1 (FP32 reconstruction, fluxes and SA terms)");
+    if (p->precision == 1 && p->If_viscous == 1 && p->Iflag_turbulence_model == 3)
+        return fail(OCFD_EUNSUPPORTED, "precision 1 covers the laminar and SA paths; SST runs in FP64 (precision 0)");
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (ndev < 1) return fail(OCFD_EHIP, "no HIP device");
@@ -1016,7 +1032,8 @@ int ocfd_sync(ocfd_ctx* c) {
 static bool same_switches(const Par& a, const Par& b) {
     return a.NV == b.NV && a.visc == b.visc && a.turb == b.turb && a.scheme == b.scheme && a.flux == b.flux &&
            a.recon == b.recon && a.bscheme == b.bscheme && a.local_dt == b.local_dt && a.dtime_mesh == b.dtime_mesh &&
-           a.positivity == b.positivity && a.limit_flow == b.limit_flow && a.gamma == b.gamma && a.w_LU == b.w_LU;
+           a.positivity == b.positivity && a.limit_flow == b.limit_flow && a.gamma == b.gamma && a.w_LU == b.w_LU &&
+           a.mixed == b.mixed;
 }
 
 int ocfd_batch_create(ocfd_batch** out) {
@@ -1042,7 +1059,7 @@ int ocfd_batch_add(ocfd_batch* b, ocfd_ctx* c) {
     for (HostBlock& hb : c->blocks) if ((r = check_subs(c, hb))) return r;
     for (ocfd_ctx* m : b->members) if (m == c) return fail(OCFD_EARG, "context already in the batch");
     if (!b->members.empty() && !same_switches(b->members[0]->p, c->p))
-        return fail(OCFD_EARG, "a batch needs the same NVAR, schemes, flux, turbulence model, limiter, gamma and w_LU in every member");
+        return fail(OCFD_EARG, "a batch needs the same NVAR, schemes, flux, turbulence model, limiter, gamma, w_LU and precision in every member");
     HIPCHK(hipStreamSynchronize(c->e.st));
     b->members.push_back(c);
     drop_plans(b->e);

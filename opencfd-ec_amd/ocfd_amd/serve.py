@@ -1,0 +1,188 @@
+"""Steady-solve serving: requests arrive, are admitted against an HBM budget, and advance
+together through shared batched launches until their residual has dropped by the requested
+factor; each result is then returned and its memory freed.
+
+A request is a complete case (the client sends its grid, freestream and options).  Admission:
+the request's device memory (block arrays plus its share of the LU-SGS tables) must fit in what
+the budget has left; a request that does not fit waits.  Convergence: Res_rms(1) relative to
+its value after the request's first iteration (the reference itself has no stop criterion
+besides t_end, opencfd_ec3d_v1.16a.f90:163).
+
+`sweep_requests` draws the request stream of the serving benchmark: Poisson arrivals, NACA
+4-digit sections extruded to span 1 on one of three C-grid sizes cut along i into 1-8 blocks,
+random Mach / AoA / Re, SA, MUSCL3 + Van Leer (the reference's default scheme and flux), CFL 5.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+from . import Batch, Solver, solver_from_case
+from . import cases as C
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float                 # seconds after the start of the sweep
+    case: C.Case
+    cells: int = 0
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    arrival: float
+    admitted: float
+    done: float
+    iterations: int
+    converged: bool
+    drop: float                    # final Res_rms(1) / Res_rms(1) after the first iteration
+    CL: float
+    CD: float
+    cells: int
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+DIMS = ((129, 49, 17), (193, 65, 25), (257, 97, 33))
+
+
+def sweep_requests(n: int = 256, lam: float = 2.0, seed: int = 5, dims=DIMS, max_blocks: int = 8,
+                   **overrides) -> List[Request]:
+    """The serving sweep's request stream (deterministic for a seed)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / lam, n))
+    out = []
+    for q in range(n):
+        m, pp, tk = rng.uniform(0.0, 0.04), rng.uniform(0.30, 0.50), rng.uniform(0.08, 0.15)
+        ni, nj, nk = dims[int(rng.integers(0, len(dims)))]
+        nb = int(rng.integers(1, max_blocks + 1))
+        Ma, AoA, Re = rng.uniform(0.3, 0.85), rng.uniform(-2.0, 6.0), rng.uniform(1e6, 1e7)
+        kw = dict(m=m, p=pp, t=tk, span=1.0, Ma=Ma, AoA=AoA, Re=Re, Iflag_turbulence_model=2,
+                  Iflag_Scheme=5, Iflag_Flux=5, CFL=5.0)
+        kw.update(overrides)
+        case = C.naca_cgrid(ni, nj, nk, **kw)
+        if nb > 1:
+            cuts = sorted(int(v) for v in np.linspace(1, ni, nb + 1)[1:-1].round())
+            case = C.cut_blocks(case, [[cuts, [], []]])
+        out.append(Request(q, float(t[q]), case, case.ncell))
+    return out
+
+
+class Server:
+    """Continuous batching of steady solves on one GPU.
+
+    mem_budget: bytes of device memory the admitted requests may hold together.
+    round_iters: iterations per batched step between admission/convergence checks.
+    """
+
+    def __init__(self, mem_budget: float = 200e9, round_iters: int = 20, drop: float = 1e-6,
+                 max_iters: int = 20000, graph: bool = True):
+        self.mem_budget = mem_budget
+        self.round_iters = round_iters
+        self.drop = drop
+        self.max_iters = max_iters
+        self.graph = graph
+
+    @staticmethod
+    def lu_bytes(case) -> int:
+        """Device bytes of a request's share of the batch's LU-SGS tables (sweep-ordered copies of
+        the sweep inputs: (3 NVAR + 20) doubles per cell, about 6 % tile padding)."""
+        nv = C.nvar_of(case.params)
+        return int(case.ncell * 1.06 * (3 * nv + 20) * 8)
+
+    def run(self, requests: List[Request], clock: Callable[[], float] = time.monotonic,
+            log: Optional[Callable[[str], None]] = None) -> List[Result]:
+        pending = sorted(requests, key=lambda r: r.arrival)
+        active: Dict[int, dict] = {}
+        results: List[Result] = []
+        batch = Batch()
+        batch.set_graph(self.graph)
+        used = 0
+        t0 = clock()
+        while pending or active:
+            now = clock() - t0
+            # admit what has arrived and fits
+            while pending and pending[0].arrival <= now:
+                r = pending[0]
+                s = solver_from_case(r.case)
+                need = s.mem_bytes() + self.lu_bytes(r.case)
+                if active and used + need > self.mem_budget:
+                    s.close()
+                    break
+                pending.pop(0)
+                s.step(1)
+                rms0 = float(s.norms()[1][0])
+                batch.add(s)
+                used += need
+                active[r.rid] = dict(req=r, s=s, rms0=rms0, iters=1, admitted=clock() - t0, need=need)
+                if log:
+                    log(f"t={now:7.2f}s admit {r.rid} cells={r.cells} active={len(active)} mem={used / 1e9:.1f}GB")
+            if not active:
+                if pending:
+                    wait = pending[0].arrival - (clock() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.05))
+                continue
+            batch.step(self.round_iters)
+            tnow = clock() - t0
+            for rid in list(active):
+                a = active[rid]
+                a["iters"] += self.round_iters
+                rms = float(a["s"].norms()[1][0])
+                ratio = rms / a["rms0"] if a["rms0"] > 0 else 0.0
+                conv = ratio <= self.drop
+                if conv or a["iters"] >= self.max_iters or not math.isfinite(ratio):
+                    f = a["s"].forces()
+                    results.append(Result(rid, a["req"].arrival, a["admitted"], tnow, a["iters"], conv, ratio,
+                                          float(f[9]), float(f[10]), a["req"].cells))
+                    batch.remove(a["s"])
+                    a["s"].close()
+                    used -= a["need"]
+                    del active[rid]
+                    if log:
+                        log(f"t={tnow:7.2f}s done {rid} iters={a['iters']} drop={ratio:.2e} active={len(active)}")
+        batch.close()
+        return results
+
+    def run_sequential(self, requests: List[Request], clock: Callable[[], float] = time.monotonic) -> List[Result]:
+        """The baseline: one request at a time on its own solver, in arrival order, same stop rule."""
+        results: List[Result] = []
+        t0 = clock()
+        for r in sorted(requests, key=lambda q: q.arrival):
+            now = clock() - t0
+            if r.arrival > now:
+                time.sleep(r.arrival - now)
+            adm = clock() - t0
+            s = solver_from_case(r.case)
+            s.set_graph(self.graph)
+            s.step(1)
+            rms0 = float(s.norms()[1][0])
+            iters, ratio = 1, 1.0
+            while True:
+                s.step(self.round_iters)
+                iters += self.round_iters
+                ratio = float(s.norms()[1][0]) / rms0 if rms0 > 0 else 0.0
+                if ratio <= self.drop or iters >= self.max_iters or not math.isfinite(ratio):
+                    break
+            f = s.forces()
+            results.append(Result(r.rid, r.arrival, adm, clock() - t0, iters, ratio <= self.drop, ratio,
+                                  float(f[9]), float(f[10]), r.cells))
+            s.close()
+        return results
+
+
+def summarize(results: List[Result], wall: float) -> Dict[str, float]:
+    lat = np.array([r.latency for r in results]) if results else np.zeros(1)
+    return dict(requests=len(results), converged=int(sum(r.converged for r in results)),
+                p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                solves_per_hour=3600.0 * len(results) / wall if wall > 0 else 0.0,
+                mean_iters=float(np.mean([r.iterations for r in results])) if results else 0.0,
+                cell_iters=float(sum(r.cells * r.iterations for r in results)), wall_s=wall)

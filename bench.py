@@ -192,6 +192,22 @@ def serve_main(args) -> int:
     return 0
 
 
+def copy_probe(mib: int, reps: int) -> int:
+    """A plain device copy of known size (the calibration kernel of the HBM byte counters): run
+    under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each dispatch must read and write `mib` MiB."""
+    import torch
+    n = mib * 2 ** 20 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_MiB": mib, "reps": reps, "bytes_read_per_dispatch": n * 8,
+                      "bytes_written_per_dispatch": n * 8}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +235,8 @@ def main() -> int:
     ap.add_argument("--serve-budget", type=float, default=200.0, help="serving: HBM budget of admitted requests, GB")
     ap.add_argument("--seq-sample", type=int, default=32, help="serving: requests the sequential baseline runs")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (byte-counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -227,6 +245,8 @@ def main() -> int:
               "multi-GPU sharding by block is a design note, not an implemented path.", file=sys.stderr)
         return 2
 
+    if args.copy_probe:
+        return copy_probe(args.copy_probe, 5)
     if args.serve:
         return serve_main(args)
     case = build_workload(args.workload, args.flux)

@@ -368,7 +368,9 @@ int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, int NV) {
     size_t bytes = (size_t)lusgs_fields(NV) * (size_t)P * sizeof(double);
     HIPCHK(hipMalloc(&pk, bytes));
     e.lu_allocs.push_back(pk);
-    HIPCHK(hipMemset(pk, 0, bytes));
+    // on the engine's stream: a plain hipMemset runs on the null stream, which does not order
+    // against a non-blocking stream, and could land after the geometry packing below
+    HIPCHK(hipMemsetAsync(pk, 0, bytes, e.st));
     e.lu.pk = (double*)pk;
     e.lu.P = P;
     int maxnx = 0;

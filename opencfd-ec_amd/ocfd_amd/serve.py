@@ -106,6 +106,7 @@ class Server:
         batch = Batch()
         batch.set_graph(self.graph)
         used = 0
+        self.peak_active, self.peak_used = 0, 0
         t0 = clock()
         while pending or active:
             now = clock() - t0
@@ -123,6 +124,8 @@ class Server:
                 batch.add(s)
                 used += need
                 active[r.rid] = dict(req=r, s=s, rms0=rms0, iters=1, admitted=clock() - t0, need=need)
+                self.peak_active = max(self.peak_active, len(active))
+                self.peak_used = max(self.peak_used, used)
                 if log:
                     log(f"t={now:7.2f}s admit {r.rid} cells={r.cells} active={len(active)} mem={used / 1e9:.1f}GB")
             if not active:

@@ -37,6 +37,7 @@ struct Blk {
     uint8_t *bcf[3];                     // face-row flags: bit0 lower face physical, bit1 upper face physical
     double *surf[6][3];                  // Surf1..6 (wall viscous force of the i-/j-/k-/i+/j+/k+ faces)
     int *flags;                          // [0] IF_OverLimit latch, [1] limited cells this iteration, [2] SA cells above LSAmax
+    unsigned long long *samask;          // SA cells above LSAmax, one bit per cell in (i,j,k) order (SA only)
 };
 
 __device__ __host__ __forceinline__ size_t IX(const Blk& b, int i, int j, int k) {

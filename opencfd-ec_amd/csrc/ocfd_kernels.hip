@@ -728,53 +728,92 @@ __global__ void __launch_bounds__(64) k_limit_apply(BSet S) {
 // residual); when some cell exceeds LSAmax, one work-group replays the reference loop over the
 // flagged cells in index order: an earlier neighbour contributes its final value, a later one
 // its original value, exactly as in the sequential reference.
+// SA part of limit_flow (sub_limitflow.f90): the reference visits the cells in (i,j,k) order and
+// replaces U6 above LSAmax by the mean of the admissible neighbours, the earlier neighbours with
+// their values after this pass (clamped to SAmin or already replaced), the later ones with their
+// values before it.  A flagged cell none of whose 26 neighbours is flagged reads only final values,
+// so those cells are done in parallel; the flagged cells with flagged neighbours are then done one
+// by one in cell order.  The detect kernel snapshots U6 into prim[5] and marks the flagged cells.
+__device__ __forceinline__ long sa_cell_index(const Blk& b, int i, int j, int k) {
+    return (long)(i - 1) + (long)(b.nx - 1) * ((long)(j - 1) + (long)(b.ny - 1) * (long)(k - 1));
+}
+__device__ __forceinline__ bool sa_flagged(const Blk& b, long c) { return (b.samask[c >> 6] >> (c & 63)) & 1ull; }
+
+__device__ __forceinline__ void sa_replace(const Blk& b, const Par& p, int i, int j, int k) {
+    const double SAmin = 1.0e-8;
+    double* U6 = b.U[5];
+    const double* orig = b.prim[5];
+    double s0 = 0.0; int kn = 0;
+    for (int ka = -1; ka <= 1; ka++) for (int ja = -1; ja <= 1; ja++) for (int ia = -1; ia <= 1; ia++) {
+        int i1 = i + ia, j1 = j + ja, k1 = k + ka;
+        if (i1 > 0 && i1 < b.nx && j1 > 0 && j1 < b.ny && k1 > 0 && k1 < b.nz) {
+            size_t t = IX(b, i1, j1, k1);
+            bool later = (k1 > k) || (k1 == k && (j1 > j || (j1 == j && i1 > i)));
+            double v = later ? orig[t] : U6[t];
+            if (v >= SAmin && v <= p.LSAmax) { s0 = s0 + v; kn++; }
+        }
+    }
+    U6[IX(b, i, j, k)] = kn == 0 ? p.LSAmax : s0 / (double)kn;
+}
+
 __global__ void __launch_bounds__(64) k_limit_sa_detect(BSet S) {
     OCFD_LOCATE(SP_CELL)
     size_t s = IX(b, i, j, k);
     double v = b.U[5][s];
     b.prim[5][s] = v;
-    if (v > p.LSAmax) atomicAdd(&b.flags[2], 1);
-    else if (v < 1.0e-8) b.U[5][s] = 1.0e-8;
+    if (v > p.LSAmax) {
+        atomicAdd(&b.flags[2], 1);
+        long c = sa_cell_index(b, i, j, k);
+        atomicOr(&b.samask[c >> 6], 1ull << (c & 63));
+    } else if (v < 1.0e-8) {
+        b.U[5][s] = 1.0e-8;
+    }
 }
 
-__global__ void k_limit_sa_serial(BSet S) {
+__global__ void __launch_bounds__(64) k_limit_sa_isolated(BSet S) {
+    OCFD_LOCATE(SP_CELL)
+    if (b.flags[2] == 0) return;
+    long c = sa_cell_index(b, i, j, k);
+    if (!sa_flagged(b, c)) return;
+    for (int ka = -1; ka <= 1; ka++) for (int ja = -1; ja <= 1; ja++) for (int ia = -1; ia <= 1; ia++) {
+        int i1 = i + ia, j1 = j + ja, k1 = k + ka;
+        if ((ia | ja | ka) != 0 && i1 > 0 && i1 < b.nx && j1 > 0 && j1 < b.ny && k1 > 0 && k1 < b.nz &&
+            sa_flagged(b, sa_cell_index(b, i1, j1, k1)))
+            return;                                   // clustered: left to the ordered pass
+    }
+    sa_replace(b, p, i, j, k);
+    // no flagged cell reads this cell's bit (it has no flagged neighbour), so it can go now
+    atomicAnd(&b.samask[c >> 6], ~(1ull << (c & 63)));
+}
+
+// one workgroup per block: the remaining (clustered) cells in cell order; clears the mask
+__global__ void __launch_bounds__(64) k_limit_sa_ordered(BSet S) {
     const Blk& b = S.blk[blockIdx.x];
     const Par& p = S.par[blockIdx.x];
     if (b.flags[2] == 0) return;
-    const double SAmin = 1.0e-8;
-    double* U6 = b.U[5];
-    const double* orig = b.prim[5];
-    __shared__ int sflag[64];
-    int nxm = b.nx - 1, nym = b.ny - 1;
-    long ncell = (long)nxm * nym * (b.nz - 1);
-    for (long base = 0; base < ncell; base += 64) {
-        long c = base + threadIdx.x;
-        int f = 0;
-        if (c < ncell) {
-            int i = (int)(c % nxm) + 1, j = (int)((c / nxm) % nym) + 1, k = (int)(c / ((long)nxm * nym)) + 1;
-            f = orig[IX(b, i, j, k)] > p.LSAmax;
-        }
-        sflag[threadIdx.x] = f;
+    const int nxm = b.nx - 1, nym = b.ny - 1;
+    const long ncell = (long)nxm * nym * (b.nz - 1);
+    const long nword = (ncell + 63) / 64;
+    __shared__ unsigned long long words[64];
+    for (long base = 0; base < nword; base += 64) {
+        long wi = base + threadIdx.x;
+        unsigned long long wv = wi < nword ? b.samask[wi] : 0ull;
+        words[threadIdx.x] = wv;
         __syncthreads();
         if (threadIdx.x == 0) {
-            for (int q = 0; q < 64; q++) {
-                if (!sflag[q]) continue;
-                long cc = base + q;
-                int i = (int)(cc % nxm) + 1, j = (int)((cc / nxm) % nym) + 1, k = (int)(cc / ((long)nxm * nym)) + 1;
-                double s0 = 0.0; int kn = 0;
-                for (int ka = -1; ka <= 1; ka++) for (int ja = -1; ja <= 1; ja++) for (int ia = -1; ia <= 1; ia++) {
-                    int i1 = i + ia, j1 = j + ja, k1 = k + ka;
-                    if (i1 > 0 && i1 < b.nx && j1 > 0 && j1 < b.ny && k1 > 0 && k1 < b.nz) {
-                        size_t t = IX(b, i1, j1, k1);
-                        bool later = (k1 > k) || (k1 == k && (j1 > j || (j1 == j && i1 > i)));
-                        double v = later ? orig[t] : U6[t];
-                        if (v >= SAmin && v <= p.LSAmax) { s0 = s0 + v; kn++; }
-                    }
+            for (int q = 0; q < 64 && base + q < nword; q++) {
+                unsigned long long m = words[q];
+                while (m) {
+                    int bit = __builtin_ctzll(m);
+                    m &= m - 1;
+                    long cc = (base + q) * 64 + bit;
+                    int i = (int)(cc % nxm) + 1, j = (int)((cc / nxm) % nym) + 1, k = (int)(cc / ((long)nxm * nym)) + 1;
+                    sa_replace(b, p, i, j, k);
                 }
-                U6[IX(b, i, j, k)] = kn == 0 ? p.LSAmax : s0 / (double)kn;
             }
         }
         __syncthreads();
+        if (wv) b.samask[wi] = 0ull;
     }
 }
 
@@ -1152,7 +1191,8 @@ void set_update_limit(const SetPlan& P, hipStream_t st) {
     launch_space<SP_CELL>(k_limit_apply, P, st);
     if (P.p0.NV == 6) {
         launch_space<SP_CELL>(k_limit_sa_detect, P, st);
-        hipLaunchKernelGGL(k_limit_sa_serial, dim3((unsigned)P.nb), dim3(64), 0, st, bset(P, SP_CELL));
+        launch_space<SP_CELL>(k_limit_sa_isolated, P, st);
+        hipLaunchKernelGGL(k_limit_sa_ordered, dim3((unsigned)P.nb), dim3(64), 0, st, bset(P, SP_CELL));
     }
 }
 

@@ -164,6 +164,12 @@ int alloc_block(ocfd_ctx* c, HostBlock& hb) {
     hb.allocs.push_back(fl);
     b.flags = (int*)fl;
     if ((r = dalloc(hb, &hb.force_acc, 12))) return r;
+    if (c->p.NV == 6) {
+        size_t ncell = (size_t)(hb.nx - 1) * (hb.ny - 1) * (hb.nz - 1);
+        double* m = nullptr;
+        if ((r = dalloc(hb, &m, (ncell + 63) / 64))) return r;
+        b.samask = reinterpret_cast<unsigned long long*>(m);
+    }
     // initial values of allocate_mem_Blocks (sub_init.f90:496,507-508)
     std::vector<double> h(n, 1.0 / c->p.Re);
     HIPCHK(hipMemcpy(b.mu, h.data(), n * sizeof(double), hipMemcpyHostToDevice));

@@ -166,3 +166,25 @@ def test_lu_schedules_agree(name, schedule):
         for no in range(1, len(case.blocks) + 1):
             assert np.array_equal(a.download_U(no), b.download_U(no)), f"iteration {it}: block {no} differs"
             assert np.array_equal(a.field(no, A.F_DU), b.field(no, A.F_DU))
+
+
+@pytest.mark.gpu
+def test_pipelined_lu_on_reused_memory():
+    """Solvers created one after another reuse the device memory of the ones before them: the
+    pipelined sweeps of each must still equal the hyperplane schedule bit for bit (a clear of the
+    sweep-ordered buffer that was not ordered against the engine's stream once zeroed the packed
+    geometry on reused memory)."""
+    case = C.naca_cgrid(Iflag_Flux=4)
+
+    def run(schedule):
+        s = A.solver_from_case(case)
+        s.set_lu_schedule(schedule)
+        s.step(2)
+        U = s.download_U(1)
+        s.close()
+        return U
+
+    ref = run(A.LU_HYPERPLANE)
+    for rep in range(3):
+        print(f"\n[pipelined run {rep}]", flush=True)
+        assert np.array_equal(run(A.LU_PIPELINED), ref), rep

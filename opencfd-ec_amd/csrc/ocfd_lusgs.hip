@@ -189,14 +189,28 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_sweep(const Blk* __restrict__ b
 }
 
 // ---------------------------------------------------------------------------------------------
-// The same schedule with a lighter exchange between tiles.  Only the DU values a downstream tile
-// reads (the tile's last j row and last k row) cross workgroups; they are written and read with
-// device-coherent 64-bit atomics (relaxed, agent scope), which bypass the per-XCD caches, so no
-// L2 write-back or invalidate is needed.  The producer waits for its stores to complete before
-// it publishes its progress (every PUB steps); the consumer reads the progress with the same
-// kind of atomic.  Everything a step needs that does not change during the sweep is loaded one
-// step ahead, and the cross-tile DU of the next step is fetched right after the barrier.
+// Team sweep: the same tiles and the same in-tile hyperplanes, three waves per tile.  Wave d
+// (d = 0, 1, 2 for i, j, k) evaluates the contribution of the cell's upstream neighbour in
+// direction d for the 64 cells of a step; after a workgroup barrier each wave forms its share of
+// the NV components, DU = (((Res + du_i) + du_j) + du_k) / alfa in the reference's order (the
+// backward sweep: DU + ((du_i + du_j) + du_k) / alfa), and stores it.  A step's critical path is
+// then one neighbour term and one share of the divisions instead of all of them.
+//
+// Cross-tile values travel through the sweep-ordered DU arrays themselves.  Before a sweep every
+// element of the array it writes holds a sentinel NaN (a payload no arithmetic produces); the
+// producer overwrites an element read by a neighbour tile with a device-coherent 64-bit store and
+// the consumer reloads it, device-coherent, until the sentinel is gone.  No progress counter and
+// no wait for store completion sit on the producer's path.  The forward sweep writes DUF and the
+// backward sweep DUB; the backward sweep puts the sentinel back into DUF after reading it, and the
+// unpack of DUB into the block layout puts it back into DUB.  Tiles are numbered upstream first
+// and workgroups start in index order, so every wait is on a tile that is running or done; every
+// wait is also bounded by a poll limit that raises the error flag.
 
+constexpr int LT_NT = 3 * LU_NT;
+constexpr unsigned long long LU_SENTINEL = 0x7FF4C0DEDEADBEEFull;
+
+__device__ __forceinline__ bool is_sentinel(double v) { return (unsigned long long)__double_as_longlong(v) == LU_SENTINEL; }
+__device__ __forceinline__ double sentinel_value() { return __longlong_as_double((long long)LU_SENTINEL); }
 __device__ __forceinline__ void st_coherent(double* p, double v) {
     __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -206,34 +220,179 @@ __device__ __forceinline__ double ld_coherent(const double* p) {
                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
-// per-step inputs of one lane that do not change during the sweep
-template <int NV>
-struct LuOwn {
-    double R[NV];        // Res (forward) or the forward DU (backward)
-    double U[NV];
-    double Lc[3], Lv[3], S[3], n[9];
-    double vol, dt;
+// Fields of the sweep-ordered storage: for a tile, element (q, lane) holds the cell that lane
+// handles at forward step q (the backward sweep visits the same cells at step nsteps-1-q), so a
+// step reads and writes 64 consecutive doubles per field.  AL holds the LU-SGS diagonal: alfa of
+// components 1..5 (and of 6, 7 for SST), and of the SA component in the forward and in the
+// backward sweep, which sum the viscous radii in different orders.
+template <int NV> struct Pk {
+    static constexpr int U = 0, R = NV, DUF = 2 * NV, DUB = 3 * NV, LC = 4 * NV, LV = 4 * NV + 3, AL = 4 * NV + 6,
+                         S = 4 * NV + 9, N = 4 * NV + 12, NF = 4 * NV + 21;
 };
-// the upstream-tile neighbour of one direction
+
+// value of field f of the sweep-ordered storage at box index s (DUF/DUB are never packed)
 template <int NV>
-struct LuRemote {
+__device__ __forceinline__ double pack_value(const Blk& b, const Par& p, int f, size_t s) {
+    using F = Pk<NV>;
+    if (f < F::R) return b.U[f][s];
+    if (f < F::DUF) return b.Res[f - F::R][s];
+    if (f < F::LV) return b.Lc[f - F::LC][s];
+    if (f < F::AL) return b.Lv[f - F::LV][s];
+    if (f < F::S) {
+        // du_LU_SGS (sub_LU_SGS.f90:27-37 forward, :81-90 backward)
+        const double Lv0 = b.Lv[0][s], Lv1 = b.Lv[1][s], Lv2 = b.Lv[2][s], vol = b.vol[s];
+        const double sumLc = (b.Lc[0][s] + b.Lc[1][s]) + b.Lc[2][s];
+        const double sumLv = (Lv0 + Lv1) + Lv2;
+        double alfa = vol / b.dt[s] + p.w_LU * sumLc;
+        if (p.visc == 1) alfa = alfa + 2.0 * sumLv;
+        double a = alfa;
+        if (p.visc == 1 && NV == 6) {
+            if (f == F::AL + 1) a = alfa + sumLv;
+            else if (f == F::AL + 2) a = ((alfa + Lv0) + Lv1) + Lv2;
+        }
+        return a + 0.0 * vol;
+    }
+    if (f < F::N) return b.S[f - F::S][s];
+    return b.nv[f - F::N][s];
+}
+
+constexpr int PK_W = 32;      // i-window of one transpose workgroup
+constexpr int PK_T = 256;
+
+// box -> sweep order for fields [f0,f1) and [f2,f3), or (TO_PACKED false) field range [f0,f1) of
+// the sweep order back into the box arrays of DU, writing the sentinel back behind it; one tile
+// and one i-window per workgroup, through LDS so both sides move in contiguous runs
+template <int NV, bool TO_PACKED>
+__global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
+                                                       double* __restrict__ pk, long long P, Par p, int f0, int f1, int f2, int f3) {
+    __shared__ double buf[LU_NT][PK_W + 1];
+    const LuTask T = tiles[blockIdx.x];
+    const Blk& b = blks[T.blk];
+    const int nxm = b.nx - 1;
+    const int i0 = 1 + (int)blockIdx.y * PK_W;
+    if (i0 > nxm) return;
+    const int W = min(PK_W, nxm - i0 + 1);
+    const int nsteps = nxm + T.tj + T.tk - 2;
+    const int t = threadIdx.x;
+    for (int f = f0; f < f3; f++) {
+        if (f == f1) { f = f2 - 1; continue; }
+        double* pf = pk + (size_t)f * (size_t)P + T.pofs;
+        if (TO_PACKED) {
+            for (int e = t; e < LU_NT * PK_W; e += PK_T) {
+                int r = e / PK_W, c = e % PK_W, jl = r % LU_TJ, kl = r / LU_TJ;
+                if (jl < T.tj && kl < T.tk && c < W) buf[r][c] = pack_value<NV>(b, p, f, IX(b, i0 + c, T.j0 + jl, T.k0 + kl));
+            }
+            __syncthreads();
+            for (int e = t; e < (W + LU_TJ + LU_TK - 2) * LU_NT; e += PK_T) {
+                int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % LU_TJ, kl = lane / LU_TJ;
+                int i = q - jl - kl + 1;
+                if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * LU_NT + lane] = buf[lane][i - i0];
+            }
+        } else {
+            for (int e = t; e < (W + LU_TJ + LU_TK - 2) * LU_NT; e += PK_T) {
+                int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % LU_TJ, kl = lane / LU_TJ;
+                int i = q - jl - kl + 1;
+                if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) {
+                    double* el = pf + (size_t)q * LU_NT + lane;
+                    buf[lane][i - i0] = *el;
+                    *el = sentinel_value();
+                }
+            }
+            __syncthreads();
+            double* box = b.DU[f - Pk<NV>::DUB];
+            for (int e = t; e < LU_NT * PK_W; e += PK_T) {
+                int r = e / PK_W, c = e % PK_W, jl = r % LU_TJ, kl = r / LU_TJ;
+                if (jl < T.tj && kl < T.tk && c < W) box[IX(b, i0 + c, T.j0 + jl, T.k0 + kl)] = buf[r][c];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_fill_sentinel(double* p, long long n) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+        p[e] = sentinel_value();
+}
+
+// LDS record of one cell of the previous step, read by the downstream lanes of the tile
+template <int NV>
+struct TeamSlot {
+    double U[NV], DU[NV];
+    double Lc[3], Lv[3], S[3];
+};
+
+// first component of each wave's share of the combine step
+__device__ __forceinline__ int team_m0(int NV, int d) {
+    return NV == 5 ? (d == 0 ? 0 : d == 1 ? 2 : d == 2 ? 4 : 5)
+         : NV == 6 ? 2 * d
+                   : (d == 0 ? 0 : d == 1 ? 3 : d == 2 ? 5 : 7);
+}
+
+// own inputs of a cell for wave D: the face normal and area of direction D, the values of
+// direction D the downstream lanes read, U (wave 0, for the downstream lanes), and Res (forward)
+// or the forward DU (backward) with alfa for the wave's components
+template <int NV, int D>
+struct TeamOwn {
+    double n[3], S, Lc, Lv;
+    double U[D == 0 ? NV : 1];
+    double R[3], al[3];
+};
+
+// the neighbour across the tile edge (waves 1 and 2)
+template <int NV>
+struct TeamRemote {
     double U[NV], DU[NV];
     double Lc, Lv, S;
 };
 
-// du contribution of one upstream neighbour (sub_LU_SGS.f90:42-60 forward, :94-119 backward)
+template <int NV, bool FWD, int D>
+__device__ __forceinline__ void team_load_own(const double* __restrict__ pk, long long P, size_t e, TeamOwn<NV, D>& o) {
+    using F = Pk<NV>;
+#pragma unroll
+    for (int c = 0; c < 3; c++) o.n[c] = pk[(size_t)(F::N + 3 * D + c) * P + e];
+    o.S = pk[(size_t)(F::S + D) * P + e];
+    o.Lc = pk[(size_t)(F::LC + D) * P + e];
+    o.Lv = pk[(size_t)(F::LV + D) * P + e];
+    if constexpr (D == 0) {
+#pragma unroll
+        for (int m = 0; m < NV; m++) o.U[m] = pk[(size_t)(F::U + m) * P + e];
+    }
+    const int m0 = team_m0(NV, D), m1 = team_m0(NV, D + 1);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int m = m0 + c;
+        if (m < m1) {
+            o.R[c] = pk[(size_t)((FWD ? F::R : F::DUF) + m) * P + e];
+            o.al[c] = pk[(size_t)(F::AL + ((NV == 6 && m == 5) ? (FWD ? 1 : 2) : 0)) * P + e];
+        }
+    }
+}
+
+template <int NV, bool FWD, int D>
+__device__ __forceinline__ void team_load_remote(const double* __restrict__ pk, long long P, size_t e, TeamRemote<NV>& r) {
+    using F = Pk<NV>;
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+        r.U[m] = pk[(size_t)(F::U + m) * P + e];
+        r.DU[m] = ld_coherent(pk + (size_t)((FWD ? F::DUF : F::DUB) + m) * P + e);
+    }
+    r.Lc = pk[(size_t)(F::LC + D) * P + e];
+    r.Lv = pk[(size_t)(F::LV + D) * P + e];
+    r.S = FWD ? 0.0 : pk[(size_t)(F::S + D) * P + e];
+}
+
+// du of one upstream neighbour (sub_LU_SGS.f90:42-60 forward, :94-119 backward)
 template <int NV, bool FWD>
-__device__ __forceinline__ void lu_neighbour(double* du, const double* Un_in, const double* DUn, double Lcn, double Lvn,
-                                             double Sn, const LuOwn<NV>& o, int dir, double wlu, double gamma, bool visc) {
+__device__ __forceinline__ void team_neighbour(double* du, const double* Un_in, const double* DUn, double Lcn, double Lvn,
+                                               double Sn, const double* n, double Sd, double wlu, double gamma, bool visc) {
     double Un[NV], DF[NV];
 #pragma unroll
     for (int m = 0; m < NV; m++) Un[m] = Un_in[m];
 #pragma unroll
     for (int m = 5; m < NV; m++) if (Un[m] < 0.0) Un[m] = 0.0;     // limit_vt
-    comput_dfn<NV>(DF, Un, DUn, o.n[3 * dir], o.n[3 * dir + 1], o.n[3 * dir + 2], gamma);
+    comput_dfn<NV>(DF, Un, DUn, n[0], n[1], n[2], gamma);
     const double wLc = wlu * Lcn;
     if (FWD) {
-        const double Sd = o.S[dir];
 #pragma unroll
         for (int m = 0; m < NV; m++) du[m] = 0.5 * (DF[m] * Sd + wLc * DUn[m]);
     } else {
@@ -246,128 +405,23 @@ __device__ __forceinline__ void lu_neighbour(double* du, const double* Un_in, co
     }
 }
 
-// Fields of the sweep-ordered storage: for a tile, element (q, lane) holds the cell that lane
-// handles at forward step q (the backward sweep visits the same cells at step nsteps-1-q), so
-// every step reads and writes 64 consecutive doubles per field.
-template <int NV> struct Pk {
-    static constexpr int U = 0, R = NV, DU = 2 * NV, LC = 3 * NV, LV = 3 * NV + 3, DT = 3 * NV + 6, VOL = 3 * NV + 7,
-                         S = 3 * NV + 8, N = 3 * NV + 11, NF = 3 * NV + 20;
-};
+__device__ __forceinline__ void team_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
 
 template <int NV>
-__device__ __forceinline__ double* box_field(const Blk& b, int f) {
-    if (f < NV) return b.U[f];
-    f -= NV;
-    if (f < NV) return b.Res[f];
-    f -= NV;
-    if (f < NV) return b.DU[f];
-    f -= NV;
-    if (f < 3) return b.Lc[f];
-    f -= 3;
-    if (f < 3) return b.Lv[f];
-    f -= 3;
-    if (f == 0) return b.dt;
-    if (f == 1) return b.vol;
-    f -= 2;
-    if (f < 3) return b.S[f];
-    f -= 3;
-    return b.nv[f];
-}
+struct TeamShared {
+    TeamSlot<NV> slot[2][LU_NT];
+    double dus[3][NV][LU_NT];
+    int bail;
+};
 
-constexpr int PK_W = 32;      // i-window of one transpose workgroup
-constexpr int PK_T = 256;
-
-// box <-> sweep order for fields [f0,f1) and [f2,f3) of one tile and one i-window, through LDS so
-// that both sides are read and written in contiguous runs
-template <int NV, bool TO_PACKED>
-__global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
-                                                       double* __restrict__ pk, long long P, int f0, int f1, int f2, int f3) {
-    __shared__ double buf[LU_NT][PK_W + 1];
-    const LuTask T = tiles[blockIdx.x];
-    const Blk& b = blks[T.blk];
-    const int nxm = b.nx - 1;
-    const int i0 = 1 + (int)blockIdx.y * PK_W;
-    if (i0 > nxm) return;
-    const int W = min(PK_W, nxm - i0 + 1);
-    const int nsteps = nxm + T.tj + T.tk - 2;
-    const int t = threadIdx.x;
-    for (int f = f0; f < f3; f++) {
-        if (f == f1) { f = f2 - 1; continue; }
-        double* box = box_field<NV>(b, f);
-        double* pf = pk + (size_t)f * (size_t)P + T.pofs;
-        if (TO_PACKED) {
-            for (int e = t; e < LU_NT * PK_W; e += PK_T) {
-                int r = e / PK_W, c = e % PK_W, jl = r % LU_TJ, kl = r / LU_TJ;
-                if (jl < T.tj && kl < T.tk && c < W) buf[r][c] = box[IX(b, i0 + c, T.j0 + jl, T.k0 + kl)];
-            }
-            __syncthreads();
-            for (int e = t; e < (W + LU_TJ + LU_TK - 2) * LU_NT; e += PK_T) {
-                int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % LU_TJ, kl = lane / LU_TJ;
-                int i = q - jl - kl + 1;
-                if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * LU_NT + lane] = buf[lane][i - i0];
-            }
-        } else {
-            for (int e = t; e < (W + LU_TJ + LU_TK - 2) * LU_NT; e += PK_T) {
-                int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % LU_TJ, kl = lane / LU_TJ;
-                int i = q - jl - kl + 1;
-                if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) buf[lane][i - i0] = pf[(size_t)q * LU_NT + lane];
-            }
-            __syncthreads();
-            for (int e = t; e < LU_NT * PK_W; e += PK_T) {
-                int r = e / PK_W, c = e % PK_W, jl = r % LU_TJ, kl = r / LU_TJ;
-                if (jl < T.tj && kl < T.tk && c < W) box[IX(b, i0 + c, T.j0 + jl, T.k0 + kl)] = buf[r][c];
-            }
-        }
-        __syncthreads();
-    }
-}
-
-template <int NV, bool FWD>
-__device__ __forceinline__ void lu_load_own_pk(const double* pk, long long P, size_t e, LuOwn<NV>& o) {
+template <int NV, bool FWD, int D>
+__device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, double* __restrict__ pk, long long P,
+                           int* __restrict__ err, const Par& p, int lane) {
     using F = Pk<NV>;
-#pragma unroll
-    for (int m = 0; m < NV; m++) {
-        o.R[m] = pk[(size_t)((FWD ? F::R : F::DU) + m) * P + e];
-        o.U[m] = pk[(size_t)(F::U + m) * P + e];
-    }
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        o.Lc[d] = pk[(size_t)(F::LC + d) * P + e];
-        o.Lv[d] = pk[(size_t)(F::LV + d) * P + e];
-        o.S[d] = pk[(size_t)(F::S + d) * P + e];
-    }
-#pragma unroll
-    for (int q = 0; q < 9; q++) o.n[q] = pk[(size_t)(F::N + q) * P + e];
-    o.vol = pk[(size_t)F::VOL * P + e];
-    o.dt = pk[(size_t)F::DT * P + e];
-}
-
-template <int NV, bool FWD>
-__device__ __forceinline__ void lu_load_remote_pk(const double* pk, long long P, size_t e, int dir, LuRemote<NV>& r) {
-    using F = Pk<NV>;
-#pragma unroll
-    for (int m = 0; m < NV; m++) {
-        r.U[m] = pk[(size_t)(F::U + m) * P + e];
-        r.DU[m] = ld_coherent(pk + (size_t)(F::DU + m) * P + e);
-    }
-    r.Lc = pk[(size_t)(F::LC + dir) * P + e];
-    r.Lv = pk[(size_t)(F::LV + dir) * P + e];
-    r.S = FWD ? 0.0 : pk[(size_t)(F::S + dir) * P + e];
-}
-
-template <int NV, bool FWD>
-__global__ void __launch_bounds__(LU_NT) k_lusgs_pipe(const Blk* __restrict__ blks, const LuTask* __restrict__ tasks,
-                                                      double* __restrict__ pk, long long P,
-                                                      int* __restrict__ prog, int* __restrict__ prog_other,
-                                                      int n_other, int* __restrict__ err, Par p) {
-    using F = Pk<NV>;
-    __shared__ LuSlot<NV> slot[2][LU_NT];
-    __shared__ int bail;
-    const int w = blockIdx.x, lane = threadIdx.x;
-    if (lane == 0 && w < n_other) prog_other[w] = -1;
-
-    const LuTask T = tasks[w];
-    const Blk& b = blks[T.blk];
     const int jl = lane % LU_TJ, kl = lane / LU_TJ;
     const bool live = jl < T.tj && kl < T.tk;
     const int jo = FWD ? jl : T.tj - 1 - jl, ko = FWD ? kl : T.tk - 1 - kl;
@@ -376,151 +430,123 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_pipe(const Blk* __restrict__ bl
     const double wlu = p.w_LU, gamma = p.gamma;
     const bool visc = p.visc == 1;
     const int nsteps = nxm + T.tj + T.tk - 2;
-    const bool has_j = FWD ? j != 1 : j != b.ny - 1, has_k = FWD ? k != 1 : k != b.nz - 1;
-    const bool rem_j = live && has_j && jo == 0, rem_k = live && has_k && ko == 0;
+    // the neighbour of direction D: another lane of the tile, or the upstream tile
+    const bool has_d = D == 0 ? true : D == 1 ? (FWD ? j != 1 : j != b.ny - 1) : (FWD ? k != 1 : k != b.nz - 1);
+    const bool inside = D == 0 ? true : D == 1 ? jo > 0 : ko > 0;
+    const bool remote = D > 0 && live && has_d && !inside;
+    const int nbl = D == 0 ? lane : D == 1 ? (FWD ? lane - 1 : lane + 1) : (FWD ? lane - LU_TJ : lane + LU_TJ);
+    const int pofs_r = D == 1 ? T.pofs_j : T.pofs_k;
+    const int jlu = FWD ? T.tj_up - 1 : 0, klu = FWD ? T.tk_up - 1 : 0;   // the upstream lane across the edge
     const bool pub_lane = live && (jo == T.tj - 1 || ko == T.tk - 1);
-    const bool deps = T.dep_j >= 0 || T.dep_k >= 0;
-    // the upstream lanes this lane reads across the tile edges
-    const int jlu = FWD ? T.tj_up - 1 : 0, klu = FWD ? T.tk_up - 1 : 0;
-    int known_j = -1, known_k = -1, got_j = -1, got_k = -1;
-    if (lane == 0) bail = 0;
+    const int m0 = team_m0(NV, D), m1 = team_m0(NV, D + 1);
 
     auto cell_i = [&](int q) { return FWD ? q - jo - ko + 1 : nxm - (q - jo - ko); };
     auto own_e = [&](int q) { return (size_t)T.pofs + (size_t)(FWD ? q : nsteps - 1 - q) * LU_NT + lane; };
-    auto need_j = [&](int q) { return T.dep_j >= 0 ? min(q + T.off_j, T.last_j) : -1; };
-    auto need_k = [&](int q) { return T.dep_k >= 0 ? min(q + T.off_k, T.last_k) : -1; };
-    auto wait_for = [&](int q) {
-        long polls = 0;
-        while (known_j < need_j(q) || known_k < need_k(q)) {
-            if (known_j < need_j(q)) known_j = __hip_atomic_load(prog + T.dep_j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (known_k < need_k(q)) known_k = __hip_atomic_load(prog + T.dep_k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (known_j >= need_j(q) && known_k >= need_k(q)) break;
-            if (++polls > LU_POLL_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                bail = 1;
-                return;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
+    auto rem_e = [&](int i) {
+        return D == 1 ? (size_t)pofs_r + (size_t)(i - 1 + jlu + kl) * LU_NT + jlu + LU_TJ * kl
+                      : (size_t)pofs_r + (size_t)(i - 1 + jl + klu) * LU_NT + jl + LU_TJ * klu;
     };
-    auto load_remote = [&](int q, LuRemote<NV>& rj, LuRemote<NV>& rk) {
-        const int iq = cell_i(q);
-        if (live && iq >= 1 && iq <= nxm && q < nsteps) {
-            if (rem_j) lu_load_remote_pk<NV, FWD>(pk, P, (size_t)T.pofs_j + (size_t)(iq - 1 + jlu + kl) * LU_NT + jlu + LU_TJ * kl, 1, rj);
-            if (rem_k) lu_load_remote_pk<NV, FWD>(pk, P, (size_t)T.pofs_k + (size_t)(iq - 1 + jl + klu) * LU_NT + jl + LU_TJ * klu, 2, rk);
-        }
-    };
+    auto active = [&](int q) { const int i = cell_i(q); return live && q < nsteps && i >= 1 && i <= nxm; };
 
-    LuOwn<NV> own, nxt;
-    LuRemote<NV> rj, rk, rjn, rkn;
-    double pU[NV], pDU[NV], pLc = 0.0, pLv = 0.0, pS = 0.0;
-#pragma unroll
-    for (int m = 0; m < NV; m++) { pU[m] = 0.0; pDU[m] = 0.0; }
-    {
-        int i0 = cell_i(0);
-        if (live && i0 >= 1 && i0 <= nxm) lu_load_own_pk<NV, FWD>(pk, P, own_e(0), own);
-        if (deps && lane == 0) wait_for(min(1, nsteps - 1));
-        __syncthreads();
-        if (bail) return;
-        load_remote(0, rj, rk);
+    TeamOwn<NV, D> own, nxt;
+    TeamRemote<NV> rem, rnx;
+    if (active(0)) {
+        team_load_own<NV, FWD, D>(pk, P, own_e(0), own);
+        if (remote) team_load_remote<NV, FWD, D>(pk, P, rem_e(cell_i(0)), rem);
     }
 
     for (int q = 0; q < nsteps; q++) {
-        const int buf = q & 1;
-        const int i = cell_i(q), i1 = cell_i(q + 1);
-        const bool act = live && i >= 1 && i <= nxm;
-        const bool act1 = live && i1 >= 1 && i1 <= nxm && q + 1 < nsteps;
-        if (act1) lu_load_own_pk<NV, FWD>(pk, P, own_e(q + 1), nxt);
-        load_remote(q + 1, rjn, rkn);
-        const bool look = deps && lane == 0 && q + 2 < nsteps && (known_j < need_j(q + 2) || known_k < need_k(q + 2));
-        if (look) {
-            if (T.dep_j >= 0) got_j = __hip_atomic_load(prog + T.dep_j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (T.dep_k >= 0) got_k = __hip_atomic_load(prog + T.dep_k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int b0 = q & 1, b1 = b0 ^ 1;
+        const int i = cell_i(q);
+        const bool act = active(q);
+        if (active(q + 1)) {
+            team_load_own<NV, FWD, D>(pk, P, own_e(q + 1), nxt);
+            if (remote) team_load_remote<NV, FWD, D>(pk, P, rem_e(cell_i(q + 1)), rnx);
         }
-        double oDU[NV];
+        // ---- phase A: the neighbour term of direction D ----
         if (act) {
-            double sumLc = (own.Lc[0] + own.Lc[1]) + own.Lc[2];
-            double sumLv = (own.Lv[0] + own.Lv[1]) + own.Lv[2];
-            double alfa0 = own.vol / own.dt + wlu * sumLc;
-            double alfa[NV];
+            double du[NV];
+            const bool has = D == 0 ? (FWD ? i != 1 : i != nxm) : has_d;
+            if (has) {
+                if (remote) {
+                    // the upstream tile's value: reload until the producer has replaced the sentinel
+                    long polls = 0;
+                    while (true) {
+                        bool ready = true;
 #pragma unroll
-            for (int m = 0; m < NV; m++) alfa[m] = alfa0;
-            if (visc) {
+                        for (int m = 0; m < NV; m++) ready = ready && !is_sentinel(rem.DU[m]);
+                        if (ready) break;
+                        if (++polls > LU_POLL_LIMIT ||
+                            ((polls & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            sh.bail = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        const size_t er = rem_e(i);
 #pragma unroll
-                for (int m = 0; m < NV; m++) alfa[m] = alfa[m] + 2.0 * sumLv;
-                if (NV == 6) {
-                    if (FWD) alfa[5] = alfa[5] + sumLv;
-                    else alfa[5] = ((alfa[5] + own.Lv[0]) + own.Lv[1]) + own.Lv[2];
+                        for (int m = 0; m < NV; m++)
+                            if (is_sentinel(rem.DU[m])) rem.DU[m] = ld_coherent(pk + (size_t)((FWD ? F::DUF : F::DUB) + m) * P + er);
+                    }
+                    team_neighbour<NV, FWD>(du, rem.U, rem.DU, rem.Lc, rem.Lv, rem.S, own.n, own.S, wlu, gamma, visc);
+                } else {
+                    const TeamSlot<NV>& o = sh.slot[b1][nbl];
+                    team_neighbour<NV, FWD>(du, o.U, o.DU, o.Lc[D], o.Lv[D], o.S[D], own.n, own.S, wlu, gamma, visc);
                 }
-            }
-#pragma unroll
-            for (int m = 0; m < NV; m++) alfa[m] = alfa[m] + 0.0 * own.vol;
-            double du0[NV], du1[NV], du2[NV];
-            const bool has_i = FWD ? i != 1 : i != nxm;
-            if (has_i) lu_neighbour<NV, FWD>(du0, pU, pDU, pLc, pLv, pS, own, 0, wlu, gamma, visc);
-            else {
-#pragma unroll
-                for (int m = 0; m < NV; m++) du0[m] = 0.0;
-            }
-            // (indices clamped: they are only read when the neighbour lane exists)
-            const LuSlot<NV>& oj = slot[buf ^ 1][min(max(FWD ? lane - 1 : lane + 1, 0), LU_NT - 1)];
-            const LuSlot<NV>& ok = slot[buf ^ 1][min(max(FWD ? lane - LU_TJ : lane + LU_TJ, 0), LU_NT - 1)];
-            if (has_j) {
-                if (rem_j) lu_neighbour<NV, FWD>(du1, rj.U, rj.DU, rj.Lc, rj.Lv, rj.S, own, 1, wlu, gamma, visc);
-                else lu_neighbour<NV, FWD>(du1, oj.U, oj.DU, oj.Lc[0], oj.Lv[0], oj.S[0], own, 1, wlu, gamma, visc);
             } else {
 #pragma unroll
-                for (int m = 0; m < NV; m++) du1[m] = 0.0;
-            }
-            if (has_k) {
-                if (rem_k) lu_neighbour<NV, FWD>(du2, rk.U, rk.DU, rk.Lc, rk.Lv, rk.S, own, 2, wlu, gamma, visc);
-                else lu_neighbour<NV, FWD>(du2, ok.U, ok.DU, ok.Lc[1], ok.Lv[1], ok.S[1], own, 2, wlu, gamma, visc);
-            } else {
-#pragma unroll
-                for (int m = 0; m < NV; m++) du2[m] = 0.0;
+                for (int m = 0; m < NV; m++) du[m] = 0.0;
             }
 #pragma unroll
-            for (int m = 0; m < NV; m++) {
-                double acc = FWD ? own.R[m] : 0.0;
-                acc = ((acc + du0[m]) + du1[m]) + du2[m];
-                oDU[m] = FWD ? acc / alfa[m] : own.R[m] + acc / alfa[m];
+            for (int m = 0; m < NV; m++) sh.dus[D][m][lane] = du[m];
+            TeamSlot<NV>& me = sh.slot[b0][lane];
+            me.Lc[D] = own.Lc; me.Lv[D] = own.Lv; me.S[D] = own.S;
+            if constexpr (D == 0) {
+#pragma unroll
+                for (int m = 0; m < NV; m++) me.U[m] = own.U[m];
             }
         }
-        // every memory operation issued before this point is complete: the previous step's DU
-        // stores, and this step's loads.  Publish the previous step, then store this one.
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (lane == 0 && q >= 1) __hip_atomic_store(prog + w, q - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        team_barrier();
+        if (sh.bail) return;
+        // ---- phase B: this wave's components of DU ----
         if (act) {
             const size_t e = own_e(q);
+            TeamSlot<NV>& me = sh.slot[b0][lane];
 #pragma unroll
-            for (int m = 0; m < NV; m++) {
-                double* dst = pk + (size_t)(F::DU + m) * P + e;
-                if (pub_lane) st_coherent(dst, oDU[m]);
-                else *dst = oDU[m];
+            for (int c = 0; c < 3; c++) {
+                const int m = m0 + c;
+                if (m < m1) {
+                    double acc = FWD ? own.R[c] : 0.0;
+                    acc = ((acc + sh.dus[0][m][lane]) + sh.dus[1][m][lane]) + sh.dus[2][m][lane];
+                    const double v = FWD ? acc / own.al[c] : own.R[c] + acc / own.al[c];
+                    me.DU[m] = v;
+                    double* dst = pk + (size_t)((FWD ? F::DUF : F::DUB) + m) * P + e;
+                    if (pub_lane) st_coherent(dst, v);
+                    else *dst = v;
+                    // the backward sweep has read the forward DU: the sentinel goes back for the
+                    // next forward sweep
+                    if (!FWD) pk[(size_t)(F::DUF + m) * P + e] = sentinel_value();
+                }
             }
-            LuSlot<NV>& me = slot[buf][lane];
-#pragma unroll
-            for (int m = 0; m < NV; m++) { me.U[m] = own.U[m]; me.DU[m] = oDU[m]; }
-            me.Lc[0] = own.Lc[1]; me.Lc[1] = own.Lc[2]; me.Lv[0] = own.Lv[1]; me.Lv[1] = own.Lv[2];
-            me.S[0] = own.S[1]; me.S[1] = own.S[2];
-#pragma unroll
-            for (int m = 0; m < NV; m++) { pU[m] = own.U[m]; pDU[m] = oDU[m]; }
-            pLc = own.Lc[0]; pLv = own.Lv[0]; pS = own.S[0];
         }
-        if (look) { known_j = max(known_j, got_j); known_k = max(known_k, got_k); }
-        if (deps && lane == 0 && q + 2 < nsteps) wait_for(q + 2);
-        __syncthreads();
-        if (bail) return;
+        team_barrier();
         own = nxt;
-        rj = rjn;
-        rk = rkn;
+        if (remote) rem = rnx;
     }
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    if (lane == 0) __hip_atomic_store(prog + w, nsteps - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NV, bool FWD>
+__global__ void __launch_bounds__(LT_NT) k_lusgs_team(const Blk* __restrict__ blks, const LuTask* __restrict__ tasks,
+                                                      double* __restrict__ pk, long long P, int* __restrict__ err, Par p) {
+    __shared__ TeamShared<NV> sh;
+    const int wave = threadIdx.x / LU_NT, lane = threadIdx.x % LU_NT;
+    if (threadIdx.x == 0) sh.bail = 0;
+    __syncthreads();
+    const LuTask T = tasks[blockIdx.x];
+    const Blk& b = blks[T.blk];
+    if (wave == 0) team_sweep<NV, FWD, 0>(sh, b, T, pk, P, err, p, lane);
+    else if (wave == 1) team_sweep<NV, FWD, 1>(sh, b, T, pk, P, err, p, lane);
+    else team_sweep<NV, FWD, 2>(sh, b, T, pk, P, err, p, lane);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -583,17 +609,17 @@ static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, h
         using F = Pk<NV>;
         dim3 tgrid((unsigned)L.n_fwd, (unsigned)L.nwin);
         if (forward) {
-            // this iteration's inputs into sweep order: U, Res, Lc, Lv, dt
-            hipLaunchKernelGGL((k_lu_transpose<NV, true>), tgrid, dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P,
-                               F::U, F::DU, F::LC, F::VOL);
-            hipLaunchKernelGGL((k_lusgs_pipe<NV, true>), dim3((unsigned)L.n_fwd), dim3(LU_NT), 0, st,
-                               L.blks, L.fwd, L.pk, L.P, L.prog_fwd, L.prog_bwd, L.n_bwd, L.err, p);
+            // this iteration's inputs into sweep order: U, Res, Lc, Lv and the diagonal
+            hipLaunchKernelGGL((k_lu_transpose<NV, true>), tgrid, dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p,
+                               F::U, F::DUF, F::LC, F::S);
+            hipLaunchKernelGGL((k_lusgs_team<NV, true>), dim3((unsigned)L.n_fwd), dim3(LT_NT), 0, st,
+                               L.blks, L.fwd, L.pk, L.P, L.err, p);
         } else {
-            hipLaunchKernelGGL((k_lusgs_pipe<NV, false>), dim3((unsigned)L.n_bwd), dim3(LU_NT), 0, st,
-                               L.blks, L.bwd, L.pk, L.P, L.prog_bwd, L.prog_fwd, L.n_fwd, L.err, p);
-            // DU back into the block layout
-            hipLaunchKernelGGL((k_lu_transpose<NV, false>), tgrid, dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P,
-                               F::DU, F::LC, F::LC, F::LC);
+            hipLaunchKernelGGL((k_lusgs_team<NV, false>), dim3((unsigned)L.n_bwd), dim3(LT_NT), 0, st,
+                               L.blks, L.bwd, L.pk, L.P, L.err, p);
+            // DU back into the block layout (and the sentinel back into DUB)
+            hipLaunchKernelGGL((k_lu_transpose<NV, false>), tgrid, dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p,
+                               F::DUB, F::DUB + NV, F::DUB + NV, F::DUB + NV);
         }
         return;
     }
@@ -611,21 +637,33 @@ void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fe
     else sweep_nv<7>(L, p, forward, fenced, st);
 }
 
-int lusgs_fields(int NV) { return 3 * NV + 20; }
+int lusgs_fields(int NV) { return 4 * NV + 21; }
 int lusgs_windows(int max_cells_i) { return (max_cells_i + PK_W - 1) / PK_W; }
 
-// the geometry the sweeps read (vol, S, n), into sweep order once per plan
+// the geometry the sweeps read (S, n), into sweep order once per plan, and the sentinel into DUF
+// and DUB (also after a sweep gave up waiting)
 template <int NV>
-static void pack_geometry_nv(const LuPlan& L, hipStream_t st) {
+static void pack_geometry_nv(const LuPlan& L, const Par& p, hipStream_t st) {
     using F = Pk<NV>;
     hipLaunchKernelGGL((k_lu_transpose<NV, true>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin), dim3(PK_T), 0, st,
-                       L.blks, L.fwd, L.pk, L.P, F::VOL, F::NF, F::NF, F::NF);
+                       L.blks, L.fwd, L.pk, L.P, p, F::S, F::NF, F::NF, F::NF);
+}
+template <int NV>
+static void reset_nv(const LuPlan& L, hipStream_t st) {
+    using F = Pk<NV>;
+    hipLaunchKernelGGL(k_fill_sentinel, dim3(1024), dim3(256), 0, st, L.pk + (size_t)F::DUF * (size_t)L.P, 2LL * NV * L.P);
 }
 
-void launch_lusgs_pack_geometry(const LuPlan& L, int NV, hipStream_t st) {
-    if (NV == 5) pack_geometry_nv<5>(L, st);
-    else if (NV == 6) pack_geometry_nv<6>(L, st);
-    else pack_geometry_nv<7>(L, st);
+void launch_lusgs_pack_geometry(const LuPlan& L, const Par& p, hipStream_t st) {
+    if (p.NV == 5) pack_geometry_nv<5>(L, p, st);
+    else if (p.NV == 6) pack_geometry_nv<6>(L, p, st);
+    else pack_geometry_nv<7>(L, p, st);
+}
+
+void launch_lusgs_reset(const LuPlan& L, int NV, hipStream_t st) {
+    if (NV == 5) reset_nv<5>(L, st);
+    else if (NV == 6) reset_nv<6>(L, st);
+    else reset_nv<7>(L, st);
 }
 
 }  // namespace ocfd

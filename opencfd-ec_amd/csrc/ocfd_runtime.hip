@@ -51,6 +51,7 @@ struct Engine {
     std::vector<hipEvent_t> events;           // stage boundaries of ocfd_step_timed
     int lu_schedule = OCFD_LU_PIPELINED;     // LU-SGS sweep schedule
     LuPlan lu{};                              // pipelined-sweep tables (built on first use)
+    int lu_nv = 0;
     std::vector<void*> lu_allocs;
     SetPlan set{};                            // batched-launch tables (built on first use)
     std::vector<void*> set_allocs;
@@ -353,7 +354,8 @@ int upload_table(std::vector<void*>& allocs, const std::vector<T>& v, T** out) {
 // Device tables of the pipelined sweeps: the block table, the tile lists of both sweep
 // directions, one progress counter per tile and direction (-1 = nothing done), the error flag
 // and the sweep-ordered storage with the geometry packed into it.
-int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, int NV) {
+int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) {
+    const int NV = p0.NV;
     if (e.lu.fwd) return OCFD_OK;
     std::vector<Blk> blks;
     for (const BlockRef& r : refs) blks.push_back(r.hb->d);
@@ -382,7 +384,9 @@ int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, int NV) {
     int maxnx = 0;
     for (const Blk& q : blks) maxnx = std::max(maxnx, q.nx - 1);
     e.lu.nwin = lusgs_windows(maxnx);
-    launch_lusgs_pack_geometry(e.lu, NV, e.st);
+    launch_lusgs_pack_geometry(e.lu, p0, e.st);
+    launch_lusgs_reset(e.lu, NV, e.st);
+    e.lu_nv = NV;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e.st));
     return OCFD_OK;
@@ -399,6 +403,8 @@ int check_lu_error(Engine& e) {
     HIPCHK(hipMemcpy(e.lu.prog_bwd, minus.data(), e.lu.n_bwd * sizeof(int), hipMemcpyHostToDevice));
     int z = 0;
     HIPCHK(hipMemcpy(e.lu.err, &z, sizeof(int), hipMemcpyHostToDevice));
+    launch_lusgs_reset(e.lu, e.lu_nv, e.st);
+    HIPCHK(hipStreamSynchronize(e.st));
     return fail(OCFD_EHIP, "LU-SGS sweep: a tile waited beyond its poll limit; the iteration is invalid");
 }
 
@@ -495,7 +501,7 @@ int build_set_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) 
 int engine_prepare(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, bool lu) {
     int r = build_set_plan(e, refs, p0);
     if (r) return r;
-    if (lu && e.lu_schedule != OCFD_LU_HYPERPLANE) return build_lu_plan(e, refs, p0.NV);
+    if (lu && e.lu_schedule != OCFD_LU_HYPERPLANE) return build_lu_plan(e, refs, p0);
     return OCFD_OK;
 }
 

@@ -229,9 +229,9 @@ def main() -> int:
                          "default mixed:1,fp64:4,mixed:4 on the single-block C-grid, none elsewhere)")
     ap.add_argument("--serve", type=int, default=0, help="run the serving sweep with this many requests instead")
     ap.add_argument("--rate", type=float, default=2.0, help="serving: request arrivals per second")
-    ap.add_argument("--drop", type=float, default=1e-5, help="serving: Res_rms(1) drop that ends a solve")
+    ap.add_argument("--drop", type=float, default=5e-3, help="serving: Res_rms(1) drop that ends a solve")
     ap.add_argument("--round-iters", type=int, default=20, help="serving: iterations between admission/convergence checks")
-    ap.add_argument("--max-iters", type=int, default=20000, help="serving: iteration cap per request")
+    ap.add_argument("--max-iters", type=int, default=6000, help="serving: iteration cap per request")
     ap.add_argument("--serve-budget", type=float, default=200.0, help="serving: HBM budget of admitted requests, GB")
     ap.add_argument("--seq-sample", type=int, default=32, help="serving: requests the sequential baseline runs")
     ap.add_argument("--verbose", action="store_true")

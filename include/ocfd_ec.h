@@ -161,12 +161,14 @@ int ocfd_set_graph(ocfd_ctx *ctx, int32_t enable);
 
 /* Schedule of the LU-SGS sweeps; all give the reference's DU bit for bit.
  *   OCFD_LU_PIPELINED        (default) one persistent launch per sweep: 8x8 (j,k) tiles march
- *                            along i; tile-edge DU and progress counters exchanged with
- *                            device-coherent atomics
+ *                            along i, three waves per tile (one per neighbour direction);
+ *                            tile-edge DU exchanged through sentinel-marked, device-coherent
+ *                            stores and loads
  *   OCFD_LU_HYPERPLANE       one launch per i+j+k hyperplane, the reference's own order
  *                            (sub_LU_SGS.f90:20-126)
- *   OCFD_LU_PIPELINED_FENCED the pipelined schedule with agent-scope release/acquire fences at
- *                            every step (L2 write-back and invalidate) */
+ *   OCFD_LU_PIPELINED_FENCED one wave per 8x8 tile on the block layout, progress counters with
+ *                            agent-scope release/acquire fences at every step (L2 write-back
+ *                            and invalidate) */
 #define OCFD_LU_PIPELINED 0
 #define OCFD_LU_HYPERPLANE 1
 #define OCFD_LU_PIPELINED_FENCED 2

@@ -368,17 +368,23 @@ __device__ __forceinline__ void team_load_own(const double* __restrict__ pk, lon
     }
 }
 
+// the upstream neighbour's values that do not change during the sweep
 template <int NV, bool FWD, int D>
 __device__ __forceinline__ void team_load_remote(const double* __restrict__ pk, long long P, size_t e, TeamRemote<NV>& r) {
     using F = Pk<NV>;
 #pragma unroll
-    for (int m = 0; m < NV; m++) {
-        r.U[m] = pk[(size_t)(F::U + m) * P + e];
-        r.DU[m] = ld_coherent(pk + (size_t)((FWD ? F::DUF : F::DUB) + m) * P + e);
-    }
+    for (int m = 0; m < NV; m++) r.U[m] = pk[(size_t)(F::U + m) * P + e];
     r.Lc = pk[(size_t)(F::LC + D) * P + e];
     r.Lv = pk[(size_t)(F::LV + D) * P + e];
     r.S = FWD ? 0.0 : pk[(size_t)(F::S + D) * P + e];
+}
+
+// ... and its DU, written during the sweep (may still be the sentinel)
+template <int NV, bool FWD>
+__device__ __forceinline__ void team_load_remote_du(const double* __restrict__ pk, long long P, size_t e, TeamRemote<NV>& r) {
+    using F = Pk<NV>;
+#pragma unroll
+    for (int m = 0; m < NV; m++) r.DU[m] = ld_coherent(pk + (size_t)((FWD ? F::DUF : F::DUB) + m) * P + e);
 }
 
 // du of one upstream neighbour (sub_LU_SGS.f90:42-60 forward, :94-119 backward)
@@ -405,6 +411,28 @@ __device__ __forceinline__ void team_neighbour(double* du, const double* Un_in, 
     }
 }
 
+// reload an upstream DU until the producer has replaced the sentinel; true when the wait gives up
+template <int NV, bool FWD>
+__device__ __forceinline__ bool team_wait_remote(const double* __restrict__ pk, long long P, size_t er, double* DU, int* __restrict__ err) {
+    using F = Pk<NV>;
+    long polls = 0;
+    while (true) {
+        bool ready = true;
+#pragma unroll
+        for (int m = 0; m < NV; m++) ready = ready && !is_sentinel(DU[m]);
+        if (ready) return false;
+        if (++polls > LU_POLL_LIMIT ||
+            ((polls & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return true;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int m = 0; m < NV; m++)
+            if (is_sentinel(DU[m])) DU[m] = ld_coherent(pk + (size_t)((FWD ? F::DUF : F::DUB) + m) * P + er);
+    }
+}
+
 __device__ __forceinline__ void team_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_s_barrier();
@@ -418,6 +446,11 @@ struct TeamShared {
     int bail;
 };
 
+// One wave's part of a tile's sweep.  The loop is unrolled by two with two register sets for the
+// prefetched inputs, so a step's loads are issued one full step before their use and no register
+// copy waits for them; every global load and store is issued by all lanes (inactive lanes address
+// an element of their own that nobody reads), which keeps the wait counts the compiler derives
+// independent of the lane masks.
 template <int NV, bool FWD, int D>
 __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, double* __restrict__ pk, long long P,
                            int* __restrict__ err, const Par& p, int lane) {
@@ -441,53 +474,36 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
     const int m0 = team_m0(NV, D), m1 = team_m0(NV, D + 1);
 
     auto cell_i = [&](int q) { return FWD ? q - jo - ko + 1 : nxm - (q - jo - ko); };
-    auto own_e = [&](int q) { return (size_t)T.pofs + (size_t)(FWD ? q : nsteps - 1 - q) * LU_NT + lane; };
-    auto rem_e = [&](int i) {
+    auto active = [&](int q) { const int i = cell_i(q); return live && q < nsteps && i >= 1 && i <= nxm; };
+    auto own_e = [&](int q) {
+        const int qc = min(q, nsteps - 1);
+        return (size_t)T.pofs + (size_t)(FWD ? qc : nsteps - 1 - qc) * LU_NT + lane;
+    };
+    // the remote element of step q; lanes without one (or steps outside the block) use their own
+    auto rem_e = [&](int q) {
+        if (!remote || !active(q)) return own_e(q);
+        const int i = cell_i(q);
         return D == 1 ? (size_t)pofs_r + (size_t)(i - 1 + jlu + kl) * LU_NT + jlu + LU_TJ * kl
                       : (size_t)pofs_r + (size_t)(i - 1 + jl + klu) * LU_NT + jl + LU_TJ * klu;
     };
-    auto active = [&](int q) { const int i = cell_i(q); return live && q < nsteps && i >= 1 && i <= nxm; };
 
-    TeamOwn<NV, D> own, nxt;
-    TeamRemote<NV> rem, rnx;
-    if (active(0)) {
-        team_load_own<NV, FWD, D>(pk, P, own_e(0), own);
-        if (remote) team_load_remote<NV, FWD, D>(pk, P, rem_e(cell_i(0)), rem);
-    }
-
-    for (int q = 0; q < nsteps; q++) {
+    // one step: the neighbour term (phase A), a barrier, this wave's components (phase B), a
+    // barrier; true when the workgroup gives up
+    auto step = [&](int q, const TeamOwn<NV, D>& own, TeamRemote<NV>& rem) -> bool {
         const int b0 = q & 1, b1 = b0 ^ 1;
         const int i = cell_i(q);
         const bool act = active(q);
-        if (active(q + 1)) {
-            team_load_own<NV, FWD, D>(pk, P, own_e(q + 1), nxt);
-            if (remote) team_load_remote<NV, FWD, D>(pk, P, rem_e(cell_i(q + 1)), rnx);
-        }
-        // ---- phase A: the neighbour term of direction D ----
         if (act) {
             double du[NV];
             const bool has = D == 0 ? (FWD ? i != 1 : i != nxm) : has_d;
             if (has) {
                 if (remote) {
-                    // the upstream tile's value: reload until the producer has replaced the sentinel
-                    long polls = 0;
-                    while (true) {
-                        bool ready = true;
+                    // the upstream tile's value; the wait loop sits off the common path, so the wait
+                    // counts in front of it do not cover the loads issued for later steps
+                    bool ready = true;
 #pragma unroll
-                        for (int m = 0; m < NV; m++) ready = ready && !is_sentinel(rem.DU[m]);
-                        if (ready) break;
-                        if (++polls > LU_POLL_LIMIT ||
-                            ((polls & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
-                            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            sh.bail = 1;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                        const size_t er = rem_e(i);
-#pragma unroll
-                        for (int m = 0; m < NV; m++)
-                            if (is_sentinel(rem.DU[m])) rem.DU[m] = ld_coherent(pk + (size_t)((FWD ? F::DUF : F::DUB) + m) * P + er);
-                    }
+                    for (int m = 0; m < NV; m++) ready = ready && !is_sentinel(rem.DU[m]);
+                    if (!ready && team_wait_remote<NV, FWD>(pk, P, rem_e(q), rem.DU, err)) sh.bail = 1;
                     team_neighbour<NV, FWD>(du, rem.U, rem.DU, rem.Lc, rem.Lv, rem.S, own.n, own.S, wlu, gamma, visc);
                 } else {
                     const TeamSlot<NV>& o = sh.slot[b1][nbl];
@@ -507,31 +523,53 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
             }
         }
         team_barrier();
-        if (sh.bail) return;
+        if (sh.bail) return true;
         // ---- phase B: this wave's components of DU ----
-        if (act) {
-            const size_t e = own_e(q);
-            TeamSlot<NV>& me = sh.slot[b0][lane];
+        const size_t e = own_e(q);
+        TeamSlot<NV>& me = sh.slot[b0][lane];
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const int m = m0 + c;
-                if (m < m1) {
+        for (int c = 0; c < 3; c++) {
+            const int m = m0 + c;
+            if (m < m1) {
+                double v = 0.0;
+                if (act) {
                     double acc = FWD ? own.R[c] : 0.0;
                     acc = ((acc + sh.dus[0][m][lane]) + sh.dus[1][m][lane]) + sh.dus[2][m][lane];
-                    const double v = FWD ? acc / own.al[c] : own.R[c] + acc / own.al[c];
+                    v = FWD ? acc / own.al[c] : own.R[c] + acc / own.al[c];
                     me.DU[m] = v;
-                    double* dst = pk + (size_t)((FWD ? F::DUF : F::DUB) + m) * P + e;
-                    if (pub_lane) st_coherent(dst, v);
-                    else *dst = v;
-                    // the backward sweep has read the forward DU: the sentinel goes back for the
-                    // next forward sweep
-                    if (!FWD) pk[(size_t)(F::DUF + m) * P + e] = sentinel_value();
                 }
+                // device-coherent for every lane: the elements a neighbour tile reads need it, and one
+                // kind of store for all lanes keeps the instruction stream free of lane-mask branches
+                st_coherent(pk + (size_t)((FWD ? F::DUF : F::DUB) + m) * P + e, v);
+                // the backward sweep has read the forward DU: the sentinel goes back for the next
+                // forward sweep
+                if (!FWD) pk[(size_t)(F::DUF + m) * P + e] = sentinel_value();
             }
         }
+        (void)pub_lane;
         team_barrier();
-        own = nxt;
-        if (remote) rem = rnx;
+        return false;
+    };
+
+    TeamOwn<NV, D> oA, oB;
+    TeamRemote<NV> rA, rB;
+    team_load_own<NV, FWD, D>(pk, P, own_e(0), oA);
+    team_load_own<NV, FWD, D>(pk, P, own_e(1), oB);
+    if constexpr (D > 0) {
+        team_load_remote<NV, FWD, D>(pk, P, rem_e(0), rA);
+        team_load_remote<NV, FWD, D>(pk, P, rem_e(1), rB);
+        team_load_remote_du<NV, FWD>(pk, P, rem_e(0), rA);
+    }
+    for (int q = 0; q < nsteps; q += 2) {
+        if constexpr (D > 0) team_load_remote_du<NV, FWD>(pk, P, rem_e(q + 1), rB);
+        if (step(q, oA, rA)) return;
+        team_load_own<NV, FWD, D>(pk, P, own_e(q + 2), oA);
+        if constexpr (D > 0) team_load_remote<NV, FWD, D>(pk, P, rem_e(q + 2), rA);
+        if (q + 1 >= nsteps) break;
+        if constexpr (D > 0) team_load_remote_du<NV, FWD>(pk, P, rem_e(q + 2), rA);
+        if (step(q + 1, oB, rB)) return;
+        team_load_own<NV, FWD, D>(pk, P, own_e(q + 3), oB);
+        if constexpr (D > 0) team_load_remote<NV, FWD, D>(pk, P, rem_e(q + 3), rB);
     }
 }
 

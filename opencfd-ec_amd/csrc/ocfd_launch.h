@@ -52,6 +52,7 @@ int lusgs_fields(int NV);
 int lusgs_windows(int max_cells_i);
 void launch_lusgs_pack_geometry(const LuPlan& L, const Par& p, hipStream_t st);
 void launch_lusgs_reset(const LuPlan& L, int NV, hipStream_t st);
+void launch_lusgs_pack(const LuPlan& L, const Par& p, hipStream_t st);
 void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st);
 // device geometry precompute (ocfd_geometry.hip)
 void launch_geometry(const Blk& b, const Sub* subs, int nsub, hipStream_t st);

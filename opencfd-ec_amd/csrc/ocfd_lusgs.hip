@@ -446,9 +446,9 @@ struct TeamShared {
     int bail;
 };
 
-// One wave's part of a tile's sweep.  The loop is unrolled by two with two register sets for the
-// prefetched inputs, so a step's loads are issued one full step before their use and no register
-// copy waits for them; every global load and store is issued by all lanes (inactive lanes address
+// One wave's part of a tile's sweep.  The loop is unrolled by three with three register sets for
+// the prefetched inputs, so a step's loads are issued two full steps before their use and no
+// register copy waits for them; every global load and store is issued by all lanes (inactive lanes address
 // an element of their own that nobody reads), which keeps the wait counts the compiler derives
 // independent of the lane masks.
 template <int NV, bool FWD, int D>
@@ -551,25 +551,34 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
         return false;
     };
 
-    TeamOwn<NV, D> oA, oB;
-    TeamRemote<NV> rA, rB;
+    TeamOwn<NV, D> oA, oB, oC;
+    TeamRemote<NV> rA, rB, rC;
     team_load_own<NV, FWD, D>(pk, P, own_e(0), oA);
     team_load_own<NV, FWD, D>(pk, P, own_e(1), oB);
+    team_load_own<NV, FWD, D>(pk, P, own_e(2), oC);
     if constexpr (D > 0) {
         team_load_remote<NV, FWD, D>(pk, P, rem_e(0), rA);
         team_load_remote<NV, FWD, D>(pk, P, rem_e(1), rB);
+        team_load_remote<NV, FWD, D>(pk, P, rem_e(2), rC);
         team_load_remote_du<NV, FWD>(pk, P, rem_e(0), rA);
     }
-    for (int q = 0; q < nsteps; q += 2) {
+    // three register sets: the inputs of step q + 3 are loaded as soon as step q is done with its
+    // set, two full steps before their use; an upstream DU is loaded one step before its use
+    for (int q = 0; q < nsteps; q += 3) {
         if constexpr (D > 0) team_load_remote_du<NV, FWD>(pk, P, rem_e(q + 1), rB);
         if (step(q, oA, rA)) return;
-        team_load_own<NV, FWD, D>(pk, P, own_e(q + 2), oA);
-        if constexpr (D > 0) team_load_remote<NV, FWD, D>(pk, P, rem_e(q + 2), rA);
+        team_load_own<NV, FWD, D>(pk, P, own_e(q + 3), oA);
+        if constexpr (D > 0) team_load_remote<NV, FWD, D>(pk, P, rem_e(q + 3), rA);
         if (q + 1 >= nsteps) break;
-        if constexpr (D > 0) team_load_remote_du<NV, FWD>(pk, P, rem_e(q + 2), rA);
+        if constexpr (D > 0) team_load_remote_du<NV, FWD>(pk, P, rem_e(q + 2), rC);
         if (step(q + 1, oB, rB)) return;
-        team_load_own<NV, FWD, D>(pk, P, own_e(q + 3), oB);
-        if constexpr (D > 0) team_load_remote<NV, FWD, D>(pk, P, rem_e(q + 3), rB);
+        team_load_own<NV, FWD, D>(pk, P, own_e(q + 4), oB);
+        if constexpr (D > 0) team_load_remote<NV, FWD, D>(pk, P, rem_e(q + 4), rB);
+        if (q + 2 >= nsteps) break;
+        if constexpr (D > 0) team_load_remote_du<NV, FWD>(pk, P, rem_e(q + 3), rA);
+        if (step(q + 2, oC, rC)) return;
+        team_load_own<NV, FWD, D>(pk, P, own_e(q + 5), oC);
+        if constexpr (D > 0) team_load_remote<NV, FWD, D>(pk, P, rem_e(q + 5), rC);
     }
 }
 
@@ -647,9 +656,6 @@ static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, h
         using F = Pk<NV>;
         dim3 tgrid((unsigned)L.n_fwd, (unsigned)L.nwin);
         if (forward) {
-            // this iteration's inputs into sweep order: U, Res, Lc, Lv and the diagonal
-            hipLaunchKernelGGL((k_lu_transpose<NV, true>), tgrid, dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p,
-                               F::U, F::DUF, F::LC, F::S);
             hipLaunchKernelGGL((k_lusgs_team<NV, true>), dim3((unsigned)L.n_fwd), dim3(LT_NT), 0, st,
                                L.blks, L.fwd, L.pk, L.P, L.err, p);
         } else {
@@ -667,6 +673,20 @@ static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, h
     else
         hipLaunchKernelGGL((k_lusgs_sweep<NV, false>), dim3((unsigned)L.n_bwd), dim3(LU_NT), 0, st,
                            L.blks, L.bwd, L.prog_bwd, L.prog_fwd, L.n_fwd, L.err, p);
+}
+
+// this iteration's sweep inputs into sweep order: U, Res, Lc, Lv and the diagonal
+template <int NV>
+static void pack_nv(const LuPlan& L, const Par& p, hipStream_t st) {
+    using F = Pk<NV>;
+    hipLaunchKernelGGL((k_lu_transpose<NV, true>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin), dim3(PK_T), 0, st,
+                       L.blks, L.fwd, L.pk, L.P, p, F::U, F::DUF, F::LC, F::S);
+}
+
+void launch_lusgs_pack(const LuPlan& L, const Par& p, hipStream_t st) {
+    if (p.NV == 5) pack_nv<5>(L, p, st);
+    else if (p.NV == 6) pack_nv<6>(L, p, st);
+    else pack_nv<7>(L, p, st);
 }
 
 void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st) {

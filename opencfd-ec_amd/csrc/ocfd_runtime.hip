@@ -282,6 +282,8 @@ void enqueue_stage(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, 
         break;
     case OCFD_STAGE_VISCOUS_DT:
         set_viscous_dt(P, e.st);
+        // the pipelined sweeps read their inputs in sweep order
+        if (e.lu_schedule == OCFD_LU_PIPELINED && e.lu.fwd) launch_lusgs_pack(e.lu, p0, e.st);
         break;
     case OCFD_STAGE_LU_FORWARD:
     case OCFD_STAGE_LU_BACKWARD:

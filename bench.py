@@ -43,6 +43,11 @@ HBM_PEAK_GBS = 8000.0
 PASS_WORDS = {5: (31, 32, 32, 15), 6: (73, 38, 38, 18), 7: (75, 41, 41, 21)}
 PASS_STAGES = (("residual", ("pre", "inviscid", "viscous_dt")), ("lu_forward", ("lu_forward",)),
                ("lu_backward", ("lu_backward",)), ("update", ("update",)))
+# HBM bytes per cell of each pass, measured with rocprofv3 FETCH_SIZE and WRITE_SIZE on the MI355X
+# for the single-block C-grid in FP64 and scaled by a plain device copy of known size (the summary
+# and its commands: profiles/iteration_kernels_naca0012_cgrid_257x97x33.md)
+MEASURED_TRAFFIC = {("naca0012_cgrid_257x97x33", "fp64"): {"residual": 4131.3, "lu_forward": 502.0,
+                                                           "lu_backward": 708.6, "update": 230.3}}
 FLUX_NAMES = {1: "Steger-Warming", 2: "HLL", 3: "HLLC", 4: "Roe", 5: "Van Leer", 6: "AUSMPW"}
 SCHEME_NAMES = {5: "MUSCL3", 7: "WENO5"}
 
@@ -279,8 +284,10 @@ def main() -> int:
         passes[pname] = dict(ms=t, alg_bytes=alg, gbs=alg / (t * 1e-3) / 1e9)
     dom = max(passes, key=lambda k: passes[k]["ms"])
     d = passes[dom]
+    measured = MEASURED_TRAFFIC.get((args.workload, args.precision), {}).get(dom)
     roofline = dict(bound="hbm", achieved=round(d["gbs"], 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(d["gbs"] / HBM_PEAK_GBS, 5), traffic=None,
+                    frac=round(d["gbs"] / HBM_PEAK_GBS, 5),
+                    traffic=round(measured * cells) if measured else None,
                     kernel=dom, ms_per_launch=round(d["ms"], 4), alg_bytes_per_launch=d["alg_bytes"])
     iter_bytes = sum(words) * 8.0 * cells
     out = {

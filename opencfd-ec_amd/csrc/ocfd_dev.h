@@ -139,9 +139,11 @@ struct LuTask {
     int blk, j0, k0, tj, tk;
     int dep_j, off_j, last_j;    // upstream tile along j: task index (-1: none), step offset, its last step
     int dep_k, off_k, last_k;    // upstream tile along k
-    int pofs;                    // sweep-ordered storage of this tile: element (step q, lane) at pofs + 64 q + lane
-    int pofs_j, tj_up;           // the j-upstream tile's storage offset and j extent
-    int pofs_k, tk_up;           // the k-upstream tile's storage offset and k extent
+    long long pofs;              // sweep-ordered storage of this tile: element (step q, field f, lane) at
+                                 // pofs + (q NF + f) 64 + lane
+    long long pofs_j;            // the j-upstream tile's storage offset
+    long long pofs_k;            // the k-upstream tile's storage offset
+    int tj_up, tk_up;            // the upstream tiles' j and k extents
 };
 
 // device tables of the pipelined sweeps of one context
@@ -153,7 +155,7 @@ struct LuPlan {
     int* prog_fwd = nullptr;     // last completed step per tile, -1 before the sweep
     int* prog_bwd = nullptr;
     int* err = nullptr;          // set when a wait exceeds its poll limit
-    // sweep-ordered copies of the sweep's inputs and of DU (field f of element e at pk[f * P + e])
+    // sweep-ordered copies of the sweep's inputs and of DU (layout in ocfd_lusgs.hip), P elements
     double* pk = nullptr;
     long long P = 0;
     int nwin = 0;                // i-windows per tile of the pack/unpack transposes

@@ -220,15 +220,19 @@ __device__ __forceinline__ double ld_coherent(const double* p) {
                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
-// Fields of the sweep-ordered storage: for a tile, element (q, lane) holds the cell that lane
-// handles at forward step q (the backward sweep visits the same cells at step nsteps-1-q), so a
-// step reads and writes 64 consecutive doubles per field.  AL holds the LU-SGS diagonal: alfa of
-// components 1..5 (and of 6, 7 for SST), and of the SA component in the forward and in the
-// backward sweep, which sum the viscous radii in different orders.
+// The sweep-ordered storage.  For a tile, step q (the cells its lanes visit at forward step q; the
+// backward sweep visits them at step nsteps-1-q) holds NF fields of 64 consecutive doubles, one
+// per lane: element (q, f, lane) at pofs + (q NF + f) 64 + lane.  A step's inputs therefore sit in
+// one contiguous run that a lane addresses from one pointer with constant offsets.  AL holds the
+// LU-SGS diagonal: alfa of components 1..5 (and of 6, 7 for SST), and of the SA component in the
+// forward and in the backward sweep, which sum the viscous radii in different orders.  The buffer
+// has LU_PAD_STEPS steps of padding at both ends, so loads a few steps past either end of a tile
+// (prefetches of steps that do not exist) stay inside it.
 template <int NV> struct Pk {
     static constexpr int U = 0, R = NV, DUF = 2 * NV, DUB = 3 * NV, LC = 4 * NV, LV = 4 * NV + 3, AL = 4 * NV + 6,
                          S = 4 * NV + 9, N = 4 * NV + 12, NF = 4 * NV + 21;
 };
+constexpr int LU_PAD_STEPS = 16;
 
 // value of field f of the sweep-ordered storage at box index s (DUF/DUB are never packed)
 template <int NV>
@@ -274,9 +278,10 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
     const int W = min(PK_W, nxm - i0 + 1);
     const int nsteps = nxm + T.tj + T.tk - 2;
     const int t = threadIdx.x;
+    constexpr int NF = Pk<NV>::NF;
     for (int f = f0; f < f3; f++) {
         if (f == f1) { f = f2 - 1; continue; }
-        double* pf = pk + (size_t)f * (size_t)P + T.pofs;
+        double* pf = pk + (size_t)T.pofs + (size_t)f * LU_NT;     // element (q, f, lane) at pf[q NF 64 + lane]
         if (TO_PACKED) {
             for (int e = t; e < LU_NT * PK_W; e += PK_T) {
                 int r = e / PK_W, c = e % PK_W, jl = r % LU_TJ, kl = r / LU_TJ;
@@ -286,14 +291,14 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
             for (int e = t; e < (W + LU_TJ + LU_TK - 2) * LU_NT; e += PK_T) {
                 int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % LU_TJ, kl = lane / LU_TJ;
                 int i = q - jl - kl + 1;
-                if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * LU_NT + lane] = buf[lane][i - i0];
+                if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * NF * LU_NT + lane] = buf[lane][i - i0];
             }
         } else {
             for (int e = t; e < (W + LU_TJ + LU_TK - 2) * LU_NT; e += PK_T) {
                 int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % LU_TJ, kl = lane / LU_TJ;
                 int i = q - jl - kl + 1;
                 if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) {
-                    double* el = pf + (size_t)q * LU_NT + lane;
+                    double* el = pf + (size_t)q * NF * LU_NT + lane;
                     buf[lane][i - i0] = *el;
                     *el = sentinel_value();
                 }
@@ -309,9 +314,15 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
     }
 }
 
+// the sentinel into every DUF and DUB element (and the padding)
+template <int NV>
 __global__ void k_fill_sentinel(double* p, long long n) {
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
-        p[e] = sentinel_value();
+    using F = Pk<NV>;
+    const long long pad = (long long)LU_PAD_STEPS * F::NF * LU_NT;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const int f = (int)(((e - pad) / LU_NT) % F::NF);
+        if (e < pad || e >= n - pad || (f >= F::DUF && f < F::DUB + NV)) p[e] = sentinel_value();
+    }
 }
 
 // LDS record of one cell of the previous step, read by the downstream lanes of the tile
@@ -345,46 +356,48 @@ struct TeamRemote {
     double Lc, Lv, S;
 };
 
+// loads from a lane's pointer into the step's run of fields: field f at e[f * 64]
 template <int NV, bool FWD, int D>
-__device__ __forceinline__ void team_load_own(const double* __restrict__ pk, long long P, size_t e, TeamOwn<NV, D>& o) {
+__device__ __forceinline__ void team_load_own(const double* __restrict__ e, TeamOwn<NV, D>& o) {
     using F = Pk<NV>;
 #pragma unroll
-    for (int c = 0; c < 3; c++) o.n[c] = pk[(size_t)(F::N + 3 * D + c) * P + e];
-    o.S = pk[(size_t)(F::S + D) * P + e];
-    o.Lc = pk[(size_t)(F::LC + D) * P + e];
-    o.Lv = pk[(size_t)(F::LV + D) * P + e];
+    for (int c = 0; c < 3; c++) o.n[c] = e[(F::N + 3 * D + c) * LU_NT];
+    o.S = e[(F::S + D) * LU_NT];
+    o.Lc = e[(F::LC + D) * LU_NT];
+    o.Lv = e[(F::LV + D) * LU_NT];
     if constexpr (D == 0) {
 #pragma unroll
-        for (int m = 0; m < NV; m++) o.U[m] = pk[(size_t)(F::U + m) * P + e];
+        for (int m = 0; m < NV; m++) o.U[m] = e[(F::U + m) * LU_NT];
     }
-    const int m0 = team_m0(NV, D), m1 = team_m0(NV, D + 1);
+    constexpr int m0 = NV == 5 ? 2 * D : NV == 6 ? 2 * D : (D == 0 ? 0 : 2 * D + 1);
+    constexpr int m1 = NV == 5 ? (D == 2 ? 5 : 2 * D + 2) : NV == 6 ? 2 * D + 2 : 2 * D + 3;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const int m = m0 + c;
         if (m < m1) {
-            o.R[c] = pk[(size_t)((FWD ? F::R : F::DUF) + m) * P + e];
-            o.al[c] = pk[(size_t)(F::AL + ((NV == 6 && m == 5) ? (FWD ? 1 : 2) : 0)) * P + e];
+            o.R[c] = e[((FWD ? F::R : F::DUF) + m) * LU_NT];
+            o.al[c] = e[(F::AL + ((NV == 6 && m == 5) ? (FWD ? 1 : 2) : 0)) * LU_NT];
         }
     }
 }
 
 // the upstream neighbour's values that do not change during the sweep
 template <int NV, bool FWD, int D>
-__device__ __forceinline__ void team_load_remote(const double* __restrict__ pk, long long P, size_t e, TeamRemote<NV>& r) {
+__device__ __forceinline__ void team_load_remote(const double* __restrict__ e, TeamRemote<NV>& r) {
     using F = Pk<NV>;
 #pragma unroll
-    for (int m = 0; m < NV; m++) r.U[m] = pk[(size_t)(F::U + m) * P + e];
-    r.Lc = pk[(size_t)(F::LC + D) * P + e];
-    r.Lv = pk[(size_t)(F::LV + D) * P + e];
-    r.S = FWD ? 0.0 : pk[(size_t)(F::S + D) * P + e];
+    for (int m = 0; m < NV; m++) r.U[m] = e[(F::U + m) * LU_NT];
+    r.Lc = e[(F::LC + D) * LU_NT];
+    r.Lv = e[(F::LV + D) * LU_NT];
+    r.S = FWD ? 0.0 : e[(F::S + D) * LU_NT];
 }
 
 // ... and its DU, written during the sweep (may still be the sentinel)
 template <int NV, bool FWD>
-__device__ __forceinline__ void team_load_remote_du(const double* __restrict__ pk, long long P, size_t e, TeamRemote<NV>& r) {
+__device__ __forceinline__ void team_load_remote_du(const double* __restrict__ e, TeamRemote<NV>& r) {
     using F = Pk<NV>;
 #pragma unroll
-    for (int m = 0; m < NV; m++) r.DU[m] = ld_coherent(pk + (size_t)((FWD ? F::DUF : F::DUB) + m) * P + e);
+    for (int m = 0; m < NV; m++) r.DU[m] = ld_coherent(e + ((FWD ? F::DUF : F::DUB) + m) * LU_NT);
 }
 
 // du of one upstream neighbour (sub_LU_SGS.f90:42-60 forward, :94-119 backward)
@@ -413,7 +426,7 @@ __device__ __forceinline__ void team_neighbour(double* du, const double* Un_in, 
 
 // reload an upstream DU until the producer has replaced the sentinel; true when the wait gives up
 template <int NV, bool FWD>
-__device__ __forceinline__ bool team_wait_remote(const double* __restrict__ pk, long long P, size_t er, double* DU, int* __restrict__ err) {
+__device__ __forceinline__ bool team_wait_remote(const double* __restrict__ er, double* DU, int* __restrict__ err) {
     using F = Pk<NV>;
     long polls = 0;
     while (true) {
@@ -429,7 +442,7 @@ __device__ __forceinline__ bool team_wait_remote(const double* __restrict__ pk, 
         __builtin_amdgcn_s_sleep(1);
 #pragma unroll
         for (int m = 0; m < NV; m++)
-            if (is_sentinel(DU[m])) DU[m] = ld_coherent(pk + (size_t)((FWD ? F::DUF : F::DUB) + m) * P + er);
+            if (is_sentinel(DU[m])) DU[m] = ld_coherent(er + ((FWD ? F::DUF : F::DUB) + m) * LU_NT);
     }
 }
 
@@ -448,13 +461,16 @@ struct TeamShared {
 
 // One wave's part of a tile's sweep.  The loop is unrolled by three with three register sets for
 // the prefetched inputs, so a step's loads are issued two full steps before their use and no
-// register copy waits for them; every global load and store is issued by all lanes (inactive lanes address
-// an element of their own that nobody reads), which keeps the wait counts the compiler derives
-// independent of the lane masks.
+// register copy waits for them; every global load and store is issued by all lanes (inactive lanes
+// address elements nobody reads, inside the buffer's padding where the steps run past a tile),
+// which keeps the wait counts the compiler derives independent of the lane masks.  Each lane walks
+// its own stream of elements and the upstream tile's with one pointer each, advanced by one step
+// of NF x 64 elements per step.
 template <int NV, bool FWD, int D>
-__device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, double* __restrict__ pk, long long P,
+__device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, double* __restrict__ pk,
                            int* __restrict__ err, const Par& p, int lane) {
     using F = Pk<NV>;
+    constexpr long STEP = (long)F::NF * LU_NT;                 // elements per step of a tile
     const int jl = lane % LU_TJ, kl = lane / LU_TJ;
     const bool live = jl < T.tj && kl < T.tk;
     const int jo = FWD ? jl : T.tj - 1 - jl, ko = FWD ? kl : T.tk - 1 - kl;
@@ -468,30 +484,36 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
     const bool inside = D == 0 ? true : D == 1 ? jo > 0 : ko > 0;
     const bool remote = D > 0 && live && has_d && !inside;
     const int nbl = D == 0 ? lane : D == 1 ? (FWD ? lane - 1 : lane + 1) : (FWD ? lane - LU_TJ : lane + LU_TJ);
-    const int pofs_r = D == 1 ? T.pofs_j : T.pofs_k;
-    const int jlu = FWD ? T.tj_up - 1 : 0, klu = FWD ? T.tk_up - 1 : 0;   // the upstream lane across the edge
     const bool pub_lane = live && (jo == T.tj - 1 || ko == T.tk - 1);
-    const int m0 = team_m0(NV, D), m1 = team_m0(NV, D + 1);
+    constexpr int m0 = NV == 5 ? 2 * D : NV == 6 ? 2 * D : (D == 0 ? 0 : 2 * D + 1);
+    constexpr int m1 = NV == 5 ? (D == 2 ? 5 : 2 * D + 2) : NV == 6 ? 2 * D + 2 : 2 * D + 3;
 
-    auto cell_i = [&](int q) { return FWD ? q - jo - ko + 1 : nxm - (q - jo - ko); };
-    auto active = [&](int q) { const int i = cell_i(q); return live && q < nsteps && i >= 1 && i <= nxm; };
-    auto own_e = [&](int q) {
-        const int qc = min(q, nsteps - 1);
-        return (size_t)T.pofs + (size_t)(FWD ? qc : nsteps - 1 - qc) * LU_NT + lane;
-    };
-    // the remote element of step q; lanes without one (or steps outside the block) use their own
-    auto rem_e = [&](int q) {
-        if (!remote || !active(q)) return own_e(q);
-        const int i = cell_i(q);
-        return D == 1 ? (size_t)pofs_r + (size_t)(i - 1 + jlu + kl) * LU_NT + jlu + LU_TJ * kl
-                      : (size_t)pofs_r + (size_t)(i - 1 + jl + klu) * LU_NT + jl + LU_TJ * klu;
-    };
+    // cell i of this lane at step q is i0 + dir q
+    const int i0 = FWD ? 1 - jo - ko : nxm + jo + ko;
+    const int dir = FWD ? 1 : -1;
+    // this lane's element at step 0 and per step
+    const long own0 = (long)T.pofs + (FWD ? 0L : (long)(nsteps - 1) * STEP) + lane;
+    const long dstep = FWD ? STEP : -STEP;
+    // the upstream element: the upstream tile stores by forward step, the lane across the edge is
+    // jlu + 8 kl (j) or jl + 8 klu (k); lanes without one follow their own stream
+    long rem0 = own0;
+    if (remote) {
+        const int jlu = FWD ? T.tj_up - 1 : 0, klu = FWD ? T.tk_up - 1 : 0;
+        const long long pofs_r = D == 1 ? T.pofs_j : T.pofs_k;
+        const int lane_r = D == 1 ? jlu + LU_TJ * kl : jl + LU_TJ * klu;
+        // forward step of the upstream cell at this lane's step 0 (forward: i - 1 + jl_r + kl_r)
+        const int s0 = FWD ? (D == 1 ? jlu : klu) : nxm + (D == 1 ? T.tk : T.tj) - 2;
+        rem0 = (long)pofs_r + (long)s0 * STEP + lane_r;
+    }
+    auto own_p = [&](int q) { return pk + own0 + (long)q * dstep; };
+    auto rem_p = [&](int q) { return pk + rem0 + (long)q * dstep; };
+    auto active = [&](int q) { const int i = i0 + dir * q; return live && q < nsteps && i >= 1 && i <= nxm; };
 
     // one step: the neighbour term (phase A), a barrier, this wave's components (phase B), a
     // barrier; true when the workgroup gives up
     auto step = [&](int q, const TeamOwn<NV, D>& own, TeamRemote<NV>& rem) -> bool {
         const int b0 = q & 1, b1 = b0 ^ 1;
-        const int i = cell_i(q);
+        const int i = i0 + dir * q;
         const bool act = active(q);
         if (act) {
             double du[NV];
@@ -503,7 +525,7 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
                     bool ready = true;
 #pragma unroll
                     for (int m = 0; m < NV; m++) ready = ready && !is_sentinel(rem.DU[m]);
-                    if (!ready && team_wait_remote<NV, FWD>(pk, P, rem_e(q), rem.DU, err)) sh.bail = 1;
+                    if (!ready && team_wait_remote<NV, FWD>(rem_p(q), rem.DU, err)) sh.bail = 1;
                     team_neighbour<NV, FWD>(du, rem.U, rem.DU, rem.Lc, rem.Lv, rem.S, own.n, own.S, wlu, gamma, visc);
                 } else {
                     const TeamSlot<NV>& o = sh.slot[b1][nbl];
@@ -525,7 +547,7 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
         team_barrier();
         if (sh.bail) return true;
         // ---- phase B: this wave's components of DU ----
-        const size_t e = own_e(q);
+        double* e = own_p(q);
         TeamSlot<NV>& me = sh.slot[b0][lane];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -540,10 +562,10 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
                 }
                 // device-coherent for every lane: the elements a neighbour tile reads need it, and one
                 // kind of store for all lanes keeps the instruction stream free of lane-mask branches
-                st_coherent(pk + (size_t)((FWD ? F::DUF : F::DUB) + m) * P + e, v);
+                st_coherent(e + ((FWD ? F::DUF : F::DUB) + m) * LU_NT, v);
                 // the backward sweep has read the forward DU: the sentinel goes back for the next
                 // forward sweep
-                if (!FWD) pk[(size_t)(F::DUF + m) * P + e] = sentinel_value();
+                if (!FWD) e[(F::DUF + m) * LU_NT] = sentinel_value();
             }
         }
         (void)pub_lane;
@@ -553,47 +575,47 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
 
     TeamOwn<NV, D> oA, oB, oC;
     TeamRemote<NV> rA, rB, rC;
-    team_load_own<NV, FWD, D>(pk, P, own_e(0), oA);
-    team_load_own<NV, FWD, D>(pk, P, own_e(1), oB);
-    team_load_own<NV, FWD, D>(pk, P, own_e(2), oC);
+    team_load_own<NV, FWD, D>(own_p(0), oA);
+    team_load_own<NV, FWD, D>(own_p(1), oB);
+    team_load_own<NV, FWD, D>(own_p(2), oC);
     if constexpr (D > 0) {
-        team_load_remote<NV, FWD, D>(pk, P, rem_e(0), rA);
-        team_load_remote<NV, FWD, D>(pk, P, rem_e(1), rB);
-        team_load_remote<NV, FWD, D>(pk, P, rem_e(2), rC);
-        team_load_remote_du<NV, FWD>(pk, P, rem_e(0), rA);
+        team_load_remote<NV, FWD, D>(rem_p(0), rA);
+        team_load_remote<NV, FWD, D>(rem_p(1), rB);
+        team_load_remote<NV, FWD, D>(rem_p(2), rC);
+        team_load_remote_du<NV, FWD>(rem_p(0), rA);
     }
     // three register sets: the inputs of step q + 3 are loaded as soon as step q is done with its
     // set, two full steps before their use; an upstream DU is loaded one step before its use
     for (int q = 0; q < nsteps; q += 3) {
-        if constexpr (D > 0) team_load_remote_du<NV, FWD>(pk, P, rem_e(q + 1), rB);
+        if constexpr (D > 0) team_load_remote_du<NV, FWD>(rem_p(q + 1), rB);
         if (step(q, oA, rA)) return;
-        team_load_own<NV, FWD, D>(pk, P, own_e(q + 3), oA);
-        if constexpr (D > 0) team_load_remote<NV, FWD, D>(pk, P, rem_e(q + 3), rA);
+        team_load_own<NV, FWD, D>(own_p(q + 3), oA);
+        if constexpr (D > 0) team_load_remote<NV, FWD, D>(rem_p(q + 3), rA);
         if (q + 1 >= nsteps) break;
-        if constexpr (D > 0) team_load_remote_du<NV, FWD>(pk, P, rem_e(q + 2), rC);
+        if constexpr (D > 0) team_load_remote_du<NV, FWD>(rem_p(q + 2), rC);
         if (step(q + 1, oB, rB)) return;
-        team_load_own<NV, FWD, D>(pk, P, own_e(q + 4), oB);
-        if constexpr (D > 0) team_load_remote<NV, FWD, D>(pk, P, rem_e(q + 4), rB);
+        team_load_own<NV, FWD, D>(own_p(q + 4), oB);
+        if constexpr (D > 0) team_load_remote<NV, FWD, D>(rem_p(q + 4), rB);
         if (q + 2 >= nsteps) break;
-        if constexpr (D > 0) team_load_remote_du<NV, FWD>(pk, P, rem_e(q + 3), rA);
+        if constexpr (D > 0) team_load_remote_du<NV, FWD>(rem_p(q + 3), rA);
         if (step(q + 2, oC, rC)) return;
-        team_load_own<NV, FWD, D>(pk, P, own_e(q + 5), oC);
-        if constexpr (D > 0) team_load_remote<NV, FWD, D>(pk, P, rem_e(q + 5), rC);
+        team_load_own<NV, FWD, D>(own_p(q + 5), oC);
+        if constexpr (D > 0) team_load_remote<NV, FWD, D>(rem_p(q + 5), rC);
     }
 }
 
 template <int NV, bool FWD>
 __global__ void __launch_bounds__(LT_NT) k_lusgs_team(const Blk* __restrict__ blks, const LuTask* __restrict__ tasks,
-                                                      double* __restrict__ pk, long long P, int* __restrict__ err, Par p) {
+                                                      double* __restrict__ pk, int* __restrict__ err, Par p) {
     __shared__ TeamShared<NV> sh;
     const int wave = threadIdx.x / LU_NT, lane = threadIdx.x % LU_NT;
     if (threadIdx.x == 0) sh.bail = 0;
     __syncthreads();
     const LuTask T = tasks[blockIdx.x];
     const Blk& b = blks[T.blk];
-    if (wave == 0) team_sweep<NV, FWD, 0>(sh, b, T, pk, P, err, p, lane);
-    else if (wave == 1) team_sweep<NV, FWD, 1>(sh, b, T, pk, P, err, p, lane);
-    else team_sweep<NV, FWD, 2>(sh, b, T, pk, P, err, p, lane);
+    if (wave == 0) team_sweep<NV, FWD, 0>(sh, b, T, pk, err, p, lane);
+    else if (wave == 1) team_sweep<NV, FWD, 1>(sh, b, T, pk, err, p, lane);
+    else team_sweep<NV, FWD, 2>(sh, b, T, pk, err, p, lane);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -602,9 +624,10 @@ __global__ void __launch_bounds__(LT_NT) k_lusgs_team(const Blk* __restrict__ bl
 // Tiles of every block for one sweep direction, numbered so that upstream tiles come first.
 // The sweep-ordered storage of a tile does not depend on the direction: tiles are laid out block
 // after block in forward order, each taking (nsteps x 64) elements.
-std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, long long* total) {
+std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, int NF, long long* total) {
     std::vector<LuTask> out;
-    long long pbase = 0;
+    const long long pad = (long long)LU_PAD_STEPS * NF * LU_NT;
+    long long pbase = pad;
     for (size_t bi = 0; bi < blks.size(); bi++) {
         const Blk& b = blks[bi];
         int nj = b.ny - 1, nk = b.nz - 1;
@@ -616,9 +639,9 @@ std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, long
         std::vector<long long> pofs((size_t)NJt * NKt);
         for (int K = 0; K < NKt; K++) for (int J = 0; J < NJt; J++) {
             pofs[J + NJt * K] = pbase;
-            pbase += (long long)((b.nx - 1) + size_j(J) + size_k(K) - 2) * LU_NT;
+            pbase += (long long)((b.nx - 1) + size_j(J) + size_k(K) - 2) * NF * LU_NT;
         }
-        auto P = [&](int J, int K) { return (int)pofs[J + NJt * K]; };
+        auto P = [&](int J, int K) { return pofs[J + NJt * K]; };
         // position of tile (J,K) in this block's list
         auto idx = [&](int J, int K) { return forward ? J + NJt * K : (NJt - 1 - J) + NJt * (NKt - 1 - K); };
         std::vector<LuTask> blk_tasks((size_t)NJt * NKt);
@@ -646,7 +669,7 @@ std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, long
         }
         out.insert(out.end(), blk_tasks.begin(), blk_tasks.end());
     }
-    if (total) *total = pbase;
+    if (total) *total = pbase + pad;
     return out;
 }
 
@@ -657,10 +680,10 @@ static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, h
         dim3 tgrid((unsigned)L.n_fwd, (unsigned)L.nwin);
         if (forward) {
             hipLaunchKernelGGL((k_lusgs_team<NV, true>), dim3((unsigned)L.n_fwd), dim3(LT_NT), 0, st,
-                               L.blks, L.fwd, L.pk, L.P, L.err, p);
+                               L.blks, L.fwd, L.pk, L.err, p);
         } else {
             hipLaunchKernelGGL((k_lusgs_team<NV, false>), dim3((unsigned)L.n_bwd), dim3(LT_NT), 0, st,
-                               L.blks, L.bwd, L.pk, L.P, L.err, p);
+                               L.blks, L.bwd, L.pk, L.err, p);
             // DU back into the block layout (and the sentinel back into DUB)
             hipLaunchKernelGGL((k_lu_transpose<NV, false>), tgrid, dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p,
                                F::DUB, F::DUB + NV, F::DUB + NV, F::DUB + NV);
@@ -709,7 +732,7 @@ static void pack_geometry_nv(const LuPlan& L, const Par& p, hipStream_t st) {
 template <int NV>
 static void reset_nv(const LuPlan& L, hipStream_t st) {
     using F = Pk<NV>;
-    hipLaunchKernelGGL(k_fill_sentinel, dim3(1024), dim3(256), 0, st, L.pk + (size_t)F::DUF * (size_t)L.P, 2LL * NV * L.P);
+    hipLaunchKernelGGL(k_fill_sentinel<NV>, dim3(1024), dim3(256), 0, st, L.pk, L.P);
 }
 
 void launch_lusgs_pack_geometry(const LuPlan& L, const Par& p, hipStream_t st) {

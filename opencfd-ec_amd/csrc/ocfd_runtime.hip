@@ -362,7 +362,7 @@ int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) {
     std::vector<Blk> blks;
     for (const BlockRef& r : refs) blks.push_back(r.hb->d);
     long long P = 0;
-    std::vector<LuTask> f = lusgs_tasks(blks, true, &P), b = lusgs_tasks(blks, false, nullptr);
+    std::vector<LuTask> f = lusgs_tasks(blks, true, lusgs_fields(NV), &P), b = lusgs_tasks(blks, false, lusgs_fields(NV), nullptr);
     std::vector<int> minus(std::max(f.size(), b.size()), -1), zero(4, 0);
     std::vector<int> mf(minus.begin(), minus.begin() + f.size()), mb(minus.begin(), minus.begin() + b.size());
     Blk* pb; LuTask *pf, *pw; int *qf, *qb, *pe;
@@ -375,7 +375,7 @@ int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) {
     e.lu.n_fwd = (int)f.size(); e.lu.n_bwd = (int)b.size();
     e.lu.prog_fwd = qf; e.lu.prog_bwd = qb; e.lu.err = pe;
     void* pk = nullptr;
-    size_t bytes = (size_t)lusgs_fields(NV) * (size_t)P * sizeof(double);
+    size_t bytes = (size_t)P * sizeof(double);
     HIPCHK(hipMalloc(&pk, bytes));
     e.lu_allocs.push_back(pk);
     // on the engine's stream: a plain hipMemset runs on the null stream, which does not order

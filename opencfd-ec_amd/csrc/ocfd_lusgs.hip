@@ -264,8 +264,8 @@ constexpr int PK_W = 32;      // i-window of one transpose workgroup
 constexpr int PK_T = 256;
 
 // box -> sweep order for fields [f0,f1) and [f2,f3), or (TO_PACKED false) field range [f0,f1) of
-// the sweep order back into the box arrays of DU, writing the sentinel back behind it; one tile
-// and one i-window per workgroup, through LDS so both sides move in contiguous runs
+// the sweep order back into the box arrays of DU, writing the sentinel back behind it; one tile,
+// one i-window and one field per workgroup, through LDS so both sides move in contiguous runs
 template <int NV, bool TO_PACKED>
 __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
                                                        double* __restrict__ pk, long long P, Par p, int f0, int f1, int f2, int f3) {
@@ -279,8 +279,10 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
     const int nsteps = nxm + T.tj + T.tk - 2;
     const int t = threadIdx.x;
     constexpr int NF = Pk<NV>::NF;
-    for (int f = f0; f < f3; f++) {
-        if (f == f1) { f = f2 - 1; continue; }
+    // one field per workgroup (grid z): field z of [f0,f1) then [f2,f3)
+    const int z = (int)blockIdx.z;
+    const int f = z < f1 - f0 ? f0 + z : f2 + (z - (f1 - f0));
+    {
         double* pf = pk + (size_t)T.pofs + (size_t)f * LU_NT;     // element (q, f, lane) at pf[q NF 64 + lane]
         if (TO_PACKED) {
             for (int e = t; e < LU_NT * PK_W; e += PK_T) {
@@ -310,7 +312,6 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
                 if (jl < T.tj && kl < T.tk && c < W) box[IX(b, i0 + c, T.j0 + jl, T.k0 + kl)] = buf[r][c];
             }
         }
-        __syncthreads();
     }
 }
 
@@ -677,7 +678,7 @@ template <int NV>
 static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st) {
     if (!fenced) {
         using F = Pk<NV>;
-        dim3 tgrid((unsigned)L.n_fwd, (unsigned)L.nwin);
+        dim3 tgrid((unsigned)L.n_fwd, (unsigned)L.nwin, (unsigned)NV);
         if (forward) {
             hipLaunchKernelGGL((k_lusgs_team<NV, true>), dim3((unsigned)L.n_fwd), dim3(LT_NT), 0, st,
                                L.blks, L.fwd, L.pk, L.err, p);
@@ -702,8 +703,8 @@ static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, h
 template <int NV>
 static void pack_nv(const LuPlan& L, const Par& p, hipStream_t st) {
     using F = Pk<NV>;
-    hipLaunchKernelGGL((k_lu_transpose<NV, true>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin), dim3(PK_T), 0, st,
-                       L.blks, L.fwd, L.pk, L.P, p, F::U, F::DUF, F::LC, F::S);
+    hipLaunchKernelGGL((k_lu_transpose<NV, true>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin, (unsigned)(F::DUF + (F::S - F::LC))),
+                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p, F::U, F::DUF, F::LC, F::S);
 }
 
 void launch_lusgs_pack(const LuPlan& L, const Par& p, hipStream_t st) {
@@ -726,8 +727,8 @@ int lusgs_windows(int max_cells_i) { return (max_cells_i + PK_W - 1) / PK_W; }
 template <int NV>
 static void pack_geometry_nv(const LuPlan& L, const Par& p, hipStream_t st) {
     using F = Pk<NV>;
-    hipLaunchKernelGGL((k_lu_transpose<NV, true>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin), dim3(PK_T), 0, st,
-                       L.blks, L.fwd, L.pk, L.P, p, F::S, F::NF, F::NF, F::NF);
+    hipLaunchKernelGGL((k_lu_transpose<NV, true>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin, (unsigned)(F::NF - F::S)),
+                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p, F::S, F::NF, F::NF, F::NF);
 }
 template <int NV>
 static void reset_nv(const LuPlan& L, hipStream_t st) {

@@ -190,21 +190,24 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_sweep(const Blk* __restrict__ b
 
 // ---------------------------------------------------------------------------------------------
 // Team sweep: the same tiles and the same in-tile hyperplanes, three waves per tile.  Wave d
-// (d = 0, 1, 2 for i, j, k) evaluates the contribution of the cell's upstream neighbour in
-// direction d for the 64 cells of a step; after a workgroup barrier each wave forms its share of
-// the NV components, DU = (((Res + du_i) + du_j) + du_k) / alfa in the reference's order (the
-// backward sweep: DU + ((du_i + du_j) + du_k) / alfa), and stores it.  A step's critical path is
-// then one neighbour term and one share of the divisions instead of all of them.
+// (d = 0, 1, 2 for i, j, k) owns the contribution of the upstream neighbour in direction d.  At
+// step q each wave first completes the DU of the cell it reads, the one its lane's neighbour
+// computed at step q - 1 (its own previous cell for d = 0, the lane across the j or k step for
+// d = 1, 2): DU = (((Res + du_i) + du_j) + du_k) / alfa in the reference's order (backward sweep:
+// DU + ((du_i + du_j) + du_k) / alfa) from the three waves' terms of step q - 1 in LDS.  It then
+// evaluates the neighbour term of its direction for its own cell and writes it to LDS.  One
+// workgroup barrier per step separates the terms of a step from their use in the next; the LDS
+// terms are double-buffered by step parity.  Wave 0 stores each completed DU of its own cell.
 //
 // Cross-tile values travel through the sweep-ordered DU arrays themselves.  Before a sweep every
 // element of the array it writes holds a sentinel NaN (a payload no arithmetic produces); the
 // producer overwrites an element read by a neighbour tile with a device-coherent 64-bit store and
 // the consumer reloads it, device-coherent, until the sentinel is gone.  No progress counter and
 // no wait for store completion sit on the producer's path.  The forward sweep writes DUF and the
-// backward sweep DUB; the backward sweep puts the sentinel back into DUF after reading it, and the
-// unpack of DUB into the block layout puts it back into DUB.  Tiles are numbered upstream first
-// and workgroups start in index order, so every wait is on a tile that is running or done; every
-// wait is also bounded by a poll limit that raises the error flag.
+// backward sweep DUB; the unpack of DUB into the block layout puts the sentinel back into both.
+// Tiles are numbered upstream first and workgroups start in index order, so every wait is on a
+// tile that is running or done; every wait is also bounded by a poll limit that raises the error
+// flag.
 
 constexpr int LT_NT = 3 * LU_NT;
 constexpr unsigned long long LU_SENTINEL = 0x7FF4C0DEDEADBEEFull;
@@ -264,8 +267,9 @@ constexpr int PK_W = 32;      // i-window of one transpose workgroup
 constexpr int PK_T = 256;
 
 // box -> sweep order for fields [f0,f1) and [f2,f3), or (TO_PACKED false) field range [f0,f1) of
-// the sweep order back into the box arrays of DU, writing the sentinel back behind it; one tile,
-// one i-window and one field per workgroup, through LDS so both sides move in contiguous runs
+// the sweep order (the backward DU) back into the box arrays of DU, writing the sentinel back
+// into it and into the forward DU; one tile, one i-window and one field per workgroup, through
+// LDS so both sides move in contiguous runs
 template <int NV, bool TO_PACKED>
 __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
                                                        double* __restrict__ pk, long long P, Par p, int f0, int f1, int f2, int f3) {
@@ -303,6 +307,7 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
                     double* el = pf + (size_t)q * NF * LU_NT + lane;
                     buf[lane][i - i0] = *el;
                     *el = sentinel_value();
+                    el[(Pk<NV>::DUF - Pk<NV>::DUB) * LU_NT] = sentinel_value();     // the forward DU too
                 }
             }
             __syncthreads();
@@ -326,79 +331,43 @@ __global__ void k_fill_sentinel(double* p, long long n) {
     }
 }
 
-// LDS record of one cell of the previous step, read by the downstream lanes of the tile
-template <int NV>
-struct TeamSlot {
-    double U[NV], DU[NV];
-    double Lc[3], Lv[3], S[3];
-};
 
-// first component of each wave's share of the combine step
-__device__ __forceinline__ int team_m0(int NV, int d) {
-    return NV == 5 ? (d == 0 ? 0 : d == 1 ? 2 : d == 2 ? 4 : 5)
-         : NV == 6 ? 2 * d
-                   : (d == 0 ? 0 : d == 1 ? 3 : d == 2 ? 5 : 7);
-}
-
-// own inputs of a cell for wave D: the face normal and area of direction D, the values of
-// direction D the downstream lanes read, U (wave 0, for the downstream lanes), and Res (forward)
-// or the forward DU (backward) with alfa for the wave's components
-template <int NV, int D>
-struct TeamOwn {
-    double n[3], S, Lc, Lv;
-    double U[D == 0 ? NV : 1];
-    double R[3], al[3];
-};
-
-// the neighbour across the tile edge (waves 1 and 2)
-template <int NV>
-struct TeamRemote {
-    double U[NV], DU[NV];
-    double Lc, Lv, S;
-};
-
-// loads from a lane's pointer into the step's run of fields: field f at e[f * 64]
+// inputs of one step of wave D: its own cell's face normal and (forward) area in direction D;
+// the neighbour's U, Lc, Lv and (backward) S; and the Res (forward) or forward DU (backward) with
+// the diagonal of the cell whose DU the wave completes at this step
 template <int NV, bool FWD, int D>
-__device__ __forceinline__ void team_load_own(const double* __restrict__ e, TeamOwn<NV, D>& o) {
+struct TeamIn {
+    double n[3], S;
+    double U[NV], Lc, Lv, Sn;
+    double R[NV], al[2];
+};
+
+// eo: own element of the step, en: the neighbour's, es: the element of the cell whose DU is
+// completed; field f of an element at e[f * 64]
+template <int NV, bool FWD, int D>
+__device__ __forceinline__ void team_load(const double* __restrict__ eo, const double* __restrict__ en,
+                                          const double* __restrict__ es, TeamIn<NV, FWD, D>& t) {
     using F = Pk<NV>;
 #pragma unroll
-    for (int c = 0; c < 3; c++) o.n[c] = e[(F::N + 3 * D + c) * LU_NT];
-    o.S = e[(F::S + D) * LU_NT];
-    o.Lc = e[(F::LC + D) * LU_NT];
-    o.Lv = e[(F::LV + D) * LU_NT];
-    if constexpr (D == 0) {
+    for (int c = 0; c < 3; c++) t.n[c] = eo[(F::N + 3 * D + c) * LU_NT];
+    t.S = FWD ? eo[(F::S + D) * LU_NT] : 0.0;
 #pragma unroll
-        for (int m = 0; m < NV; m++) o.U[m] = e[(F::U + m) * LU_NT];
-    }
-    constexpr int m0 = NV == 5 ? 2 * D : NV == 6 ? 2 * D : (D == 0 ? 0 : 2 * D + 1);
-    constexpr int m1 = NV == 5 ? (D == 2 ? 5 : 2 * D + 2) : NV == 6 ? 2 * D + 2 : 2 * D + 3;
+    for (int m = 0; m < NV; m++) t.U[m] = en[(F::U + m) * LU_NT];
+    t.Lc = en[(F::LC + D) * LU_NT];
+    t.Lv = en[(F::LV + D) * LU_NT];
+    t.Sn = FWD ? 0.0 : en[(F::S + D) * LU_NT];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const int m = m0 + c;
-        if (m < m1) {
-            o.R[c] = e[((FWD ? F::R : F::DUF) + m) * LU_NT];
-            o.al[c] = e[(F::AL + ((NV == 6 && m == 5) ? (FWD ? 1 : 2) : 0)) * LU_NT];
-        }
-    }
+    for (int m = 0; m < NV; m++) t.R[m] = es[((FWD ? F::R : F::DUF) + m) * LU_NT];
+    t.al[0] = es[F::AL * LU_NT];
+    t.al[1] = NV == 6 ? es[(F::AL + (FWD ? 1 : 2)) * LU_NT] : 0.0;
 }
 
-// the upstream neighbour's values that do not change during the sweep
-template <int NV, bool FWD, int D>
-__device__ __forceinline__ void team_load_remote(const double* __restrict__ e, TeamRemote<NV>& r) {
-    using F = Pk<NV>;
-#pragma unroll
-    for (int m = 0; m < NV; m++) r.U[m] = e[(F::U + m) * LU_NT];
-    r.Lc = e[(F::LC + D) * LU_NT];
-    r.Lv = e[(F::LV + D) * LU_NT];
-    r.S = FWD ? 0.0 : e[(F::S + D) * LU_NT];
-}
-
-// ... and its DU, written during the sweep (may still be the sentinel)
+// the upstream tile's DU of a cell (may still be the sentinel)
 template <int NV, bool FWD>
-__device__ __forceinline__ void team_load_remote_du(const double* __restrict__ e, TeamRemote<NV>& r) {
+__device__ __forceinline__ void team_load_du(const double* __restrict__ e, double* DU) {
     using F = Pk<NV>;
 #pragma unroll
-    for (int m = 0; m < NV; m++) r.DU[m] = ld_coherent(e + ((FWD ? F::DUF : F::DUB) + m) * LU_NT);
+    for (int m = 0; m < NV; m++) DU[m] = ld_coherent(e + ((FWD ? F::DUF : F::DUB) + m) * LU_NT);
 }
 
 // du of one upstream neighbour (sub_LU_SGS.f90:42-60 forward, :94-119 backward)
@@ -447,26 +416,38 @@ __device__ __forceinline__ bool team_wait_remote(const double* __restrict__ er, 
     }
 }
 
+// a for the lanes whose bit of mk is set, else b: one v_cndmask per half, so the choice cannot
+// become a masked overwrite of a register that a load may still be filling
+__device__ __forceinline__ double pick(unsigned long long mk, double a, double b) {
+    const unsigned long long ua = (unsigned long long)__double_as_longlong(a), ub = (unsigned long long)__double_as_longlong(b);
+    unsigned lo, hi;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(lo) : "v"((unsigned)ub), "v"((unsigned)ua), "s"(mk));
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(hi) : "v"((unsigned)(ub >> 32)), "v"((unsigned)(ua >> 32)), "s"(mk));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ __forceinline__ void team_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// LDS of a tile team: the records of the previous step (double-buffered by step parity) that
+// the downstream lanes read, one array per value so a wave's 64 lanes touch 64 consecutive
+// doubles, and the three waves' neighbour terms of the current step
 template <int NV>
 struct TeamShared {
-    TeamSlot<NV> slot[2][LU_NT];
-    double dus[3][NV][LU_NT];
+    double dus[2][3][NV][LU_NT];      // [step parity][direction][component][lane]
     int bail;
 };
 
 // One wave's part of a tile's sweep.  The loop is unrolled by three with three register sets for
-// the prefetched inputs, so a step's loads are issued two full steps before their use and no
-// register copy waits for them; every global load and store is issued by all lanes (inactive lanes
-// address elements nobody reads, inside the buffer's padding where the steps run past a tile),
-// which keeps the wait counts the compiler derives independent of the lane masks.  Each lane walks
-// its own stream of elements and the upstream tile's with one pointer each, advanced by one step
-// of NF x 64 elements per step.
+// the prefetched inputs, so a step's loads are issued two full steps before their use; an upstream
+// tile's DU is loaded one step before its use.  Every global load and store is issued by all
+// lanes (lanes without a cell address elements nobody reads, inside the tile's storage or the
+// buffer's padding), which keeps the wait counts the compiler derives independent of the lane
+// masks.  Each lane walks its streams of elements with one pointer each, advanced by one step of
+// NF x 64 elements per step.
 template <int NV, bool FWD, int D>
 __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, double* __restrict__ pk,
                            int* __restrict__ err, const Par& p, int lane) {
@@ -484,10 +465,9 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
     const bool has_d = D == 0 ? true : D == 1 ? (FWD ? j != 1 : j != b.ny - 1) : (FWD ? k != 1 : k != b.nz - 1);
     const bool inside = D == 0 ? true : D == 1 ? jo > 0 : ko > 0;
     const bool remote = D > 0 && live && has_d && !inside;
-    const int nbl = D == 0 ? lane : D == 1 ? (FWD ? lane - 1 : lane + 1) : (FWD ? lane - LU_TJ : lane + LU_TJ);
-    const bool pub_lane = live && (jo == T.tj - 1 || ko == T.tk - 1);
-    constexpr int m0 = NV == 5 ? 2 * D : NV == 6 ? 2 * D : (D == 0 ? 0 : 2 * D + 1);
-    constexpr int m1 = NV == 5 ? (D == 2 ? 5 : 2 * D + 2) : NV == 6 ? 2 * D + 2 : 2 * D + 3;
+    const unsigned long long rmask = __builtin_amdgcn_ballot_w64(remote);
+    // the lane whose cell of the previous step this lane reads (lanes without one read their own)
+    const int src = D == 0 || !inside ? lane : D == 1 ? (FWD ? lane - 1 : lane + 1) : (FWD ? lane - LU_TJ : lane + LU_TJ);
 
     // cell i of this lane at step q is i0 + dir q
     const int i0 = FWD ? 1 - jo - ko : nxm + jo + ko;
@@ -495,6 +475,8 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
     // this lane's element at step 0 and per step
     const long own0 = (long)T.pofs + (FWD ? 0L : (long)(nsteps - 1) * STEP) + lane;
     const long dstep = FWD ? STEP : -STEP;
+    // the element of lane src one step back
+    const long src0 = own0 - dstep + (src - lane);
     // the upstream element: the upstream tile stores by forward step, the lane across the edge is
     // jlu + 8 kl (j) or jl + 8 klu (k); lanes without one follow their own stream
     long rem0 = own0;
@@ -506,102 +488,90 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
         const int s0 = FWD ? (D == 1 ? jlu : klu) : nxm + (D == 1 ? T.tk : T.tj) - 2;
         rem0 = (long)pofs_r + (long)s0 * STEP + lane_r;
     }
+    const long nb0 = remote ? rem0 : src0;
     auto own_p = [&](int q) { return pk + own0 + (long)q * dstep; };
+    auto src_p = [&](int q) { return pk + src0 + (long)q * dstep; };
+    auto nb_p = [&](int q) { return pk + nb0 + (long)q * dstep; };
     auto rem_p = [&](int q) { return pk + rem0 + (long)q * dstep; };
+    auto load = [&](int q, TeamIn<NV, FWD, D>& t) { team_load<NV, FWD, D>(own_p(q), nb_p(q), src_p(q), t); };
     auto active = [&](int q) { const int i = i0 + dir * q; return live && q < nsteps && i >= 1 && i <= nxm; };
 
-    // one step: the neighbour term (phase A), a barrier, this wave's components (phase B), a
-    // barrier; true when the workgroup gives up
-    auto step = [&](int q, const TeamOwn<NV, D>& own, TeamRemote<NV>& rem) -> bool {
-        const int b0 = q & 1, b1 = b0 ^ 1;
-        const int i = i0 + dir * q;
-        const bool act = active(q);
-        if (act) {
-            double du[NV];
-            const bool has = D == 0 ? (FWD ? i != 1 : i != nxm) : has_d;
-            if (has) {
-                if (remote) {
-                    // the upstream tile's value; the wait loop sits off the common path, so the wait
-                    // counts in front of it do not cover the loads issued for later steps
-                    bool ready = true;
+    // DU of the cell of lane src at step q - 1, from the terms of that step
+    auto complete = [&](int q, const TeamIn<NV, FWD, D>& in, double* DUs) {
+        const int pb = (q & 1) ^ 1;
 #pragma unroll
-                    for (int m = 0; m < NV; m++) ready = ready && !is_sentinel(rem.DU[m]);
-                    if (!ready && team_wait_remote<NV, FWD>(rem_p(q), rem.DU, err)) sh.bail = 1;
-                    team_neighbour<NV, FWD>(du, rem.U, rem.DU, rem.Lc, rem.Lv, rem.S, own.n, own.S, wlu, gamma, visc);
-                } else {
-                    const TeamSlot<NV>& o = sh.slot[b1][nbl];
-                    team_neighbour<NV, FWD>(du, o.U, o.DU, o.Lc[D], o.Lv[D], o.S[D], own.n, own.S, wlu, gamma, visc);
-                }
-            } else {
-#pragma unroll
-                for (int m = 0; m < NV; m++) du[m] = 0.0;
-            }
-#pragma unroll
-            for (int m = 0; m < NV; m++) sh.dus[D][m][lane] = du[m];
-            TeamSlot<NV>& me = sh.slot[b0][lane];
-            me.Lc[D] = own.Lc; me.Lv[D] = own.Lv; me.S[D] = own.S;
-            if constexpr (D == 0) {
-#pragma unroll
-                for (int m = 0; m < NV; m++) me.U[m] = own.U[m];
-            }
+        for (int m = 0; m < NV; m++) {
+            const double al = in.al[(NV == 6 && m == 5) ? 1 : 0];
+            double acc = FWD ? in.R[m] : 0.0;
+            acc = ((acc + sh.dus[pb][0][m][src]) + sh.dus[pb][1][m][src]) + sh.dus[pb][2][m][src];
+            DUs[m] = FWD ? acc / al : in.R[m] + acc / al;
         }
-        team_barrier();
-        if (sh.bail) return true;
-        // ---- phase B: this wave's components of DU ----
-        double* e = own_p(q);
-        TeamSlot<NV>& me = sh.slot[b0][lane];
+    };
+    // wave 0 stores the DU of its own cell of step q - 1 (device-coherent: the elements a
+    // neighbour tile reads need it, and one kind of store for all lanes)
+    auto store = [&](int q, const double* DUs) {
+        if (D == 0 && q >= 1) {
+            double* e = own_p(q - 1);
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const int m = m0 + c;
-            if (m < m1) {
-                double v = 0.0;
-                if (act) {
-                    double acc = FWD ? own.R[c] : 0.0;
-                    acc = ((acc + sh.dus[0][m][lane]) + sh.dus[1][m][lane]) + sh.dus[2][m][lane];
-                    v = FWD ? acc / own.al[c] : own.R[c] + acc / own.al[c];
-                    me.DU[m] = v;
-                }
-                // device-coherent for every lane: the elements a neighbour tile reads need it, and one
-                // kind of store for all lanes keeps the instruction stream free of lane-mask branches
-                st_coherent(e + ((FWD ? F::DUF : F::DUB) + m) * LU_NT, v);
-                // the backward sweep has read the forward DU: the sentinel goes back for the next
-                // forward sweep
-                if (!FWD) e[(F::DUF + m) * LU_NT] = sentinel_value();
-            }
+            for (int m = 0; m < NV; m++) st_coherent(e + ((FWD ? F::DUF : F::DUB) + m) * LU_NT, DUs[m]);
         }
-        (void)pub_lane;
-        team_barrier();
-        return false;
     };
 
-    TeamOwn<NV, D> oA, oB, oC;
-    TeamRemote<NV> rA, rB, rC;
-    team_load_own<NV, FWD, D>(own_p(0), oA);
-    team_load_own<NV, FWD, D>(own_p(1), oB);
-    team_load_own<NV, FWD, D>(own_p(2), oC);
-    if constexpr (D > 0) {
-        team_load_remote<NV, FWD, D>(rem_p(0), rA);
-        team_load_remote<NV, FWD, D>(rem_p(1), rB);
-        team_load_remote<NV, FWD, D>(rem_p(2), rC);
-        team_load_remote_du<NV, FWD>(rem_p(0), rA);
-    }
-    // three register sets: the inputs of step q + 3 are loaded as soon as step q is done with its
-    // set, two full steps before their use; an upstream DU is loaded one step before its use
+    // one step: complete the read cell's DU, then this direction's term of the own cell, a
+    // barrier; true when the workgroup gives up
+    auto step = [&](int q, const TeamIn<NV, FWD, D>& in, double* rdu) -> bool {
+        double DUs[NV];
+        complete(q, in, DUs);
+        store(q, DUs);
+        const int i = i0 + dir * q;
+        if (active(q)) {
+            const bool has = D == 0 ? (FWD ? i != 1 : i != nxm) : has_d;
+            if (D > 0 && remote) {
+                // wait off the common path, so the wait counts in front of it do not cover the
+                // loads issued for later steps
+                bool ready = true;
+#pragma unroll
+                for (int m = 0; m < NV; m++) ready = ready && !is_sentinel(rdu[m]);
+                if (!ready && team_wait_remote<NV, FWD>(rem_p(q), rdu, err)) sh.bail = 1;
+            }
+            double DUn[NV];
+#pragma unroll
+            for (int m = 0; m < NV; m++) DUn[m] = D > 0 ? pick(rmask, rdu[m], DUs[m]) : DUs[m];
+            double du[NV];
+            team_neighbour<NV, FWD>(du, in.U, DUn, in.Lc, in.Lv, in.Sn, in.n, in.S, wlu, gamma, visc);
+#pragma unroll
+            for (int m = 0; m < NV; m++) sh.dus[q & 1][D][m][lane] = has ? du[m] : 0.0;
+        }
+        team_barrier();
+        return sh.bail != 0;
+    };
+
+    TeamIn<NV, FWD, D> iA, iB, iC;
+    double rA[NV], rB[NV], rC[NV];
+    load(0, iA);
+    load(1, iB);
+    load(2, iC);
+    if constexpr (D > 0) team_load_du<NV, FWD>(rem_p(0), rA);
     for (int q = 0; q < nsteps; q += 3) {
-        if constexpr (D > 0) team_load_remote_du<NV, FWD>(rem_p(q + 1), rB);
-        if (step(q, oA, rA)) return;
-        team_load_own<NV, FWD, D>(own_p(q + 3), oA);
-        if constexpr (D > 0) team_load_remote<NV, FWD, D>(rem_p(q + 3), rA);
+        if constexpr (D > 0) team_load_du<NV, FWD>(rem_p(q + 1), rB);
+        if (step(q, iA, rA)) return;
+        load(q + 3, iA);
         if (q + 1 >= nsteps) break;
-        if constexpr (D > 0) team_load_remote_du<NV, FWD>(rem_p(q + 2), rC);
-        if (step(q + 1, oB, rB)) return;
-        team_load_own<NV, FWD, D>(own_p(q + 4), oB);
-        if constexpr (D > 0) team_load_remote<NV, FWD, D>(rem_p(q + 4), rB);
+        if constexpr (D > 0) team_load_du<NV, FWD>(rem_p(q + 2), rC);
+        if (step(q + 1, iB, rB)) return;
+        load(q + 4, iB);
         if (q + 2 >= nsteps) break;
-        if constexpr (D > 0) team_load_remote_du<NV, FWD>(rem_p(q + 3), rA);
-        if (step(q + 2, oC, rC)) return;
-        team_load_own<NV, FWD, D>(own_p(q + 5), oC);
-        if constexpr (D > 0) team_load_remote<NV, FWD, D>(rem_p(q + 5), rC);
+        if constexpr (D > 0) team_load_du<NV, FWD>(rem_p(q + 3), rA);
+        if (step(q + 2, iC, rC)) return;
+        load(q + 5, iC);
+    }
+    // the DU of the last step
+    if constexpr (D == 0) {
+        TeamIn<NV, FWD, D> last;
+        load(nsteps, last);
+        double DUs[NV];
+        complete(nsteps, last, DUs);
+        store(nsteps, DUs);
     }
 }
 

@@ -96,6 +96,7 @@ def variant_rate(name: str, flux: int, precision: int, batch: int, steps: int, w
         run.set_graph(graph)
         for m in members:
             run.add(m)
+    tj, _ = run.autotune_lu(1)
     run.step(warmup)
     t0 = time.perf_counter()
     run.step(steps)               # returns after the device has finished
@@ -105,7 +106,7 @@ def variant_rate(name: str, flux: int, precision: int, batch: int, steps: int, w
         run.close()
     for m in members:
         m.close()
-    return dict(precision="fp64" if precision == 0 else "mixed", batch=batch, cells=cells,
+    return dict(precision="fp64" if precision == 0 else "mixed", batch=batch, cells=cells, lu_tile=f"{tj}x{64 // tj}",
                 value=round(cells * steps / wall / 1e6, 3), ms_per_step=round(wall * 1e3 / steps, 4),
                 res_rms1_max=max(rms))
 
@@ -224,6 +225,8 @@ def main() -> int:
                          "the C-grid within a few iterations)")
     ap.add_argument("--graph", type=int, default=1, help="replay each stage as a captured hipGraph")
     ap.add_argument("--lu", choices=("pipelined", "hyperplane", "fenced"), default="pipelined", help="LU-SGS sweep schedule")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="iterations per LU-SGS tile shape of the autotuner before the warm-up (0: keep 8x8)")
     ap.add_argument("--cpu-iters", type=int, default=20, help="iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
@@ -261,6 +264,10 @@ def main() -> int:
     nv = s.nv
     s.set_graph(bool(args.graph))
     s.set_lu_schedule({"pipelined": A.LU_PIPELINED, "hyperplane": A.LU_HYPERPLANE, "fenced": A.LU_PIPELINED_FENCED}[args.lu])
+    tune = None
+    if args.lu == "pipelined" and args.autotune > 0:
+        tj, tms = s.autotune_lu(args.autotune)          # ordinary iterations, untimed
+        tune = dict(kept=f"{tj}x{64 // tj}", ms_per_sweep_pair={f"{t}x{64 // t}": round(v, 4) for t, v in tms.items()})
     for _ in range(args.warmup):
         s.step(1)
     s.sync()
@@ -312,8 +319,10 @@ def main() -> int:
             "flux": FLUX_NAMES[int(case.params["Iflag_Flux"])],
             "graph": bool(args.graph),
             "lu_schedule": args.lu,
+            "lu_tile": tune["kept"] if tune else ("8x8" if args.lu != "hyperplane" else None),
             "precision": args.precision,
         },
+        "lu_autotune": tune,
         "roofline": roofline,
         "iteration_hbm": {"alg_bytes_per_cell": sum(words) * 8, "achieved": round(iter_bytes / (ms_per_step * 1e-3) / 1e9, 2),
                           "frac": round(iter_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},

@@ -160,10 +160,10 @@ int ocfd_step_timed(ocfd_ctx *ctx, int32_t niter, double *stage_ms);
 int ocfd_set_graph(ocfd_ctx *ctx, int32_t enable);
 
 /* Schedule of the LU-SGS sweeps; all give the reference's DU bit for bit.
- *   OCFD_LU_PIPELINED        (default) one persistent launch per sweep: 8x8 (j,k) tiles march
- *                            along i, three waves per tile (one per neighbour direction);
- *                            tile-edge DU exchanged through sentinel-marked, device-coherent
- *                            stores and loads
+ *   OCFD_LU_PIPELINED        (default) one persistent launch per sweep: (j,k) tiles of 64
+ *                            columns (8x8 unless set otherwise) march along i, three waves per
+ *                            tile (one per neighbour direction); tile-edge DU exchanged through
+ *                            sentinel-marked, device-coherent stores and loads
  *   OCFD_LU_HYPERPLANE       one launch per i+j+k hyperplane, the reference's own order
  *                            (sub_LU_SGS.f90:20-126)
  *   OCFD_LU_PIPELINED_FENCED one wave per 8x8 tile on the block layout, progress counters with
@@ -173,6 +173,20 @@ int ocfd_set_graph(ocfd_ctx *ctx, int32_t enable);
 #define OCFD_LU_HYPERPLANE 1
 #define OCFD_LU_PIPELINED_FENCED 2
 int ocfd_set_lu_schedule(ocfd_ctx *ctx, int32_t schedule);
+
+/* Tile shape of OCFD_LU_PIPELINED: tj x (64/tj) (j,k) columns per tile, tj = 4, 8 (default) or
+ * 16.  Results do not depend on it (bit for bit); the sweep time does.  No counterpart in the
+ * reference. */
+int ocfd_set_lu_tile(ocfd_ctx *ctx, int32_t tj);
+int ocfd_get_lu_tile(ocfd_ctx *ctx, int32_t *tj);
+
+/* Tile-shape autotuner of OCFD_LU_PIPELINED.  No counterpart in the reference.  Runs niter
+ * ordinary iterations with each tile shape (tj = 4, 8, 16), times the two sweeps of each with HIP
+ * events and keeps the fastest shape.  The solve advances by 3 niter iterations exactly as plain
+ * iterations would advance it (every shape gives the same result bit for bit); counters and time
+ * advance likewise.  best_tj (may be NULL) receives the shape kept, ms (may be NULL, 3 entries)
+ * the device time of one forward plus backward sweep for tj = 4, 8, 16. */
+int ocfd_autotune_lu(ocfd_ctx *ctx, int32_t niter, int32_t *best_tj, double *ms);
 
 /* Batches.  No counterpart in the reference: several contexts (solves of different cases with
  * the same NVAR, schemes, flux, turbulence model, limiter, gamma and w_LU) advance together,
@@ -187,6 +201,10 @@ int ocfd_batch_add(ocfd_batch *batch, ocfd_ctx *ctx);
 int ocfd_batch_remove(ocfd_batch *batch, ocfd_ctx *ctx);
 int ocfd_batch_step(ocfd_batch *batch, int32_t niter);
 int ocfd_batch_set_graph(ocfd_batch *batch, int32_t enable);
+/* ocfd_set_lu_tile and ocfd_autotune_lu for the batch's shared launches (every member advances
+ * by 3 niter iterations). */
+int ocfd_batch_set_lu_tile(ocfd_batch *batch, int32_t tj);
+int ocfd_batch_autotune_lu(ocfd_batch *batch, int32_t niter, int32_t *best_tj, double *ms);
 
 /* Device memory held by the context's block arrays, in bytes (admission control). */
 int ocfd_mem_bytes(ocfd_ctx *ctx, int64_t *bytes);

@@ -159,6 +159,7 @@ struct LuPlan {
     double* pk = nullptr;
     long long P = 0;
     int nwin = 0;                // i-windows per tile of the pack/unpack transposes
+    int tj = 8;                  // tile width in j (tiles are tj x 64/tj (j,k) columns)
 };
 
 }  // namespace ocfd

@@ -230,12 +230,13 @@ __device__ __forceinline__ double ld_coherent(const double* p) {
 // LU-SGS diagonal: alfa of components 1..5 (and of 6, 7 for SST), and of the SA component in the
 // forward and in the backward sweep, which sum the viscous radii in different orders.  The buffer
 // has LU_PAD_STEPS steps of padding at both ends, so loads a few steps past either end of a tile
-// (prefetches of steps that do not exist) stay inside it.
+// (prefetches of steps that do not exist, at most the larger tile side + 2 steps before an
+// upstream tile's first step) stay inside it.
 template <int NV> struct Pk {
     static constexpr int U = 0, R = NV, DUF = 2 * NV, DUB = 3 * NV, LC = 4 * NV, LV = 4 * NV + 3, AL = 4 * NV + 6,
                          S = 4 * NV + 9, N = 4 * NV + 12, NF = 4 * NV + 21;
 };
-constexpr int LU_PAD_STEPS = 16;
+constexpr int LU_PAD_STEPS = 24;
 
 // value of field f of the sweep-ordered storage at box index s (DUF/DUB are never packed)
 template <int NV>
@@ -270,7 +271,7 @@ constexpr int PK_T = 256;
 // the sweep order (the backward DU) back into the box arrays of DU, writing the sentinel back
 // into it and into the forward DU; one tile, one i-window and one field per workgroup, through
 // LDS so both sides move in contiguous runs
-template <int NV, bool TO_PACKED>
+template <int NV, bool TO_PACKED, int TJ>
 __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
                                                        double* __restrict__ pk, long long P, Par p, int f0, int f1, int f2, int f3) {
     __shared__ double buf[LU_NT][PK_W + 1];
@@ -290,18 +291,18 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
         double* pf = pk + (size_t)T.pofs + (size_t)f * LU_NT;     // element (q, f, lane) at pf[q NF 64 + lane]
         if (TO_PACKED) {
             for (int e = t; e < LU_NT * PK_W; e += PK_T) {
-                int r = e / PK_W, c = e % PK_W, jl = r % LU_TJ, kl = r / LU_TJ;
+                int r = e / PK_W, c = e % PK_W, jl = r % TJ, kl = r / TJ;
                 if (jl < T.tj && kl < T.tk && c < W) buf[r][c] = pack_value<NV>(b, p, f, IX(b, i0 + c, T.j0 + jl, T.k0 + kl));
             }
             __syncthreads();
-            for (int e = t; e < (W + LU_TJ + LU_TK - 2) * LU_NT; e += PK_T) {
-                int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % LU_TJ, kl = lane / LU_TJ;
+            for (int e = t; e < (W + TJ + LU_NT / TJ - 2) * LU_NT; e += PK_T) {
+                int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % TJ, kl = lane / TJ;
                 int i = q - jl - kl + 1;
                 if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * NF * LU_NT + lane] = buf[lane][i - i0];
             }
         } else {
-            for (int e = t; e < (W + LU_TJ + LU_TK - 2) * LU_NT; e += PK_T) {
-                int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % LU_TJ, kl = lane / LU_TJ;
+            for (int e = t; e < (W + TJ + LU_NT / TJ - 2) * LU_NT; e += PK_T) {
+                int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % TJ, kl = lane / TJ;
                 int i = q - jl - kl + 1;
                 if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) {
                     double* el = pf + (size_t)q * NF * LU_NT + lane;
@@ -313,7 +314,7 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
             __syncthreads();
             double* box = b.DU[f - Pk<NV>::DUB];
             for (int e = t; e < LU_NT * PK_W; e += PK_T) {
-                int r = e / PK_W, c = e % PK_W, jl = r % LU_TJ, kl = r / LU_TJ;
+                int r = e / PK_W, c = e % PK_W, jl = r % TJ, kl = r / TJ;
                 if (jl < T.tj && kl < T.tk && c < W) box[IX(b, i0 + c, T.j0 + jl, T.k0 + kl)] = buf[r][c];
             }
         }
@@ -448,12 +449,12 @@ struct TeamShared {
 // buffer's padding), which keeps the wait counts the compiler derives independent of the lane
 // masks.  Each lane walks its streams of elements with one pointer each, advanced by one step of
 // NF x 64 elements per step.
-template <int NV, bool FWD, int D>
+template <int NV, bool FWD, int D, int TJ>
 __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, double* __restrict__ pk,
                            int* __restrict__ err, const Par& p, int lane) {
     using F = Pk<NV>;
     constexpr long STEP = (long)F::NF * LU_NT;                 // elements per step of a tile
-    const int jl = lane % LU_TJ, kl = lane / LU_TJ;
+    const int jl = lane % TJ, kl = lane / TJ;
     const bool live = jl < T.tj && kl < T.tk;
     const int jo = FWD ? jl : T.tj - 1 - jl, ko = FWD ? kl : T.tk - 1 - kl;
     const int j = T.j0 + jl, k = T.k0 + kl;
@@ -467,7 +468,7 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
     const bool remote = D > 0 && live && has_d && !inside;
     const unsigned long long rmask = __builtin_amdgcn_ballot_w64(remote);
     // the lane whose cell of the previous step this lane reads (lanes without one read their own)
-    const int src = D == 0 || !inside ? lane : D == 1 ? (FWD ? lane - 1 : lane + 1) : (FWD ? lane - LU_TJ : lane + LU_TJ);
+    const int src = D == 0 || !inside ? lane : D == 1 ? (FWD ? lane - 1 : lane + 1) : (FWD ? lane - TJ : lane + TJ);
 
     // cell i of this lane at step q is i0 + dir q
     const int i0 = FWD ? 1 - jo - ko : nxm + jo + ko;
@@ -483,7 +484,7 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
     if (remote) {
         const int jlu = FWD ? T.tj_up - 1 : 0, klu = FWD ? T.tk_up - 1 : 0;
         const long long pofs_r = D == 1 ? T.pofs_j : T.pofs_k;
-        const int lane_r = D == 1 ? jlu + LU_TJ * kl : jl + LU_TJ * klu;
+        const int lane_r = D == 1 ? jlu + TJ * kl : jl + TJ * klu;
         // forward step of the upstream cell at this lane's step 0 (forward: i - 1 + jl_r + kl_r)
         const int s0 = FWD ? (D == 1 ? jlu : klu) : nxm + (D == 1 ? T.tk : T.tj) - 2;
         rem0 = (long)pofs_r + (long)s0 * STEP + lane_r;
@@ -575,7 +576,7 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
     }
 }
 
-template <int NV, bool FWD>
+template <int NV, bool FWD, int TJ>
 __global__ void __launch_bounds__(LT_NT) k_lusgs_team(const Blk* __restrict__ blks, const LuTask* __restrict__ tasks,
                                                       double* __restrict__ pk, int* __restrict__ err, Par p) {
     __shared__ TeamShared<NV> sh;
@@ -584,9 +585,9 @@ __global__ void __launch_bounds__(LT_NT) k_lusgs_team(const Blk* __restrict__ bl
     __syncthreads();
     const LuTask T = tasks[blockIdx.x];
     const Blk& b = blks[T.blk];
-    if (wave == 0) team_sweep<NV, FWD, 0>(sh, b, T, pk, err, p, lane);
-    else if (wave == 1) team_sweep<NV, FWD, 1>(sh, b, T, pk, err, p, lane);
-    else team_sweep<NV, FWD, 2>(sh, b, T, pk, err, p, lane);
+    if (wave == 0) team_sweep<NV, FWD, 0, TJ>(sh, b, T, pk, err, p, lane);
+    else if (wave == 1) team_sweep<NV, FWD, 1, TJ>(sh, b, T, pk, err, p, lane);
+    else team_sweep<NV, FWD, 2, TJ>(sh, b, T, pk, err, p, lane);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -595,17 +596,18 @@ __global__ void __launch_bounds__(LT_NT) k_lusgs_team(const Blk* __restrict__ bl
 // Tiles of every block for one sweep direction, numbered so that upstream tiles come first.
 // The sweep-ordered storage of a tile does not depend on the direction: tiles are laid out block
 // after block in forward order, each taking (nsteps x 64) elements.
-std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, int NF, long long* total) {
+std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, int NF, int TJ, long long* total) {
+    const int TK = LU_NT / TJ;
     std::vector<LuTask> out;
     const long long pad = (long long)LU_PAD_STEPS * NF * LU_NT;
     long long pbase = pad;
     for (size_t bi = 0; bi < blks.size(); bi++) {
         const Blk& b = blks[bi];
         int nj = b.ny - 1, nk = b.nz - 1;
-        int NJt = (nj + LU_TJ - 1) / LU_TJ, NKt = (nk + LU_TK - 1) / LU_TK;
+        int NJt = (nj + TJ - 1) / TJ, NKt = (nk + TK - 1) / TK;
         int base = (int)out.size();
-        auto size_j = [&](int J) { return std::min(LU_TJ, nj - J * LU_TJ); };
-        auto size_k = [&](int K) { return std::min(LU_TK, nk - K * LU_TK); };
+        auto size_j = [&](int J) { return std::min(TJ, nj - J * TJ); };
+        auto size_k = [&](int K) { return std::min(TK, nk - K * TK); };
         // storage offsets in forward tile order
         std::vector<long long> pofs((size_t)NJt * NKt);
         for (int K = 0; K < NKt; K++) for (int J = 0; J < NJt; J++) {
@@ -618,7 +620,7 @@ std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, int 
         std::vector<LuTask> blk_tasks((size_t)NJt * NKt);
         for (int K = 0; K < NKt; K++) for (int J = 0; J < NJt; J++) {
             LuTask t{};
-            t.blk = (int)bi; t.j0 = 1 + J * LU_TJ; t.k0 = 1 + K * LU_TK; t.tj = size_j(J); t.tk = size_k(K);
+            t.blk = (int)bi; t.j0 = 1 + J * TJ; t.k0 = 1 + K * TK; t.tj = size_j(J); t.tk = size_k(K);
             t.pofs = P(J, K);
             int Ju = forward ? J - 1 : J + 1, Ku = forward ? K - 1 : K + 1;
             t.dep_j = t.dep_k = -1;
@@ -644,23 +646,24 @@ std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, int 
     return out;
 }
 
-template <int NV>
+template <int NV, int TJ>
 static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st) {
     if (!fenced) {
         using F = Pk<NV>;
         dim3 tgrid((unsigned)L.n_fwd, (unsigned)L.nwin, (unsigned)NV);
         if (forward) {
-            hipLaunchKernelGGL((k_lusgs_team<NV, true>), dim3((unsigned)L.n_fwd), dim3(LT_NT), 0, st,
+            hipLaunchKernelGGL((k_lusgs_team<NV, true, TJ>), dim3((unsigned)L.n_fwd), dim3(LT_NT), 0, st,
                                L.blks, L.fwd, L.pk, L.err, p);
         } else {
-            hipLaunchKernelGGL((k_lusgs_team<NV, false>), dim3((unsigned)L.n_bwd), dim3(LT_NT), 0, st,
+            hipLaunchKernelGGL((k_lusgs_team<NV, false, TJ>), dim3((unsigned)L.n_bwd), dim3(LT_NT), 0, st,
                                L.blks, L.bwd, L.pk, L.err, p);
-            // DU back into the block layout (and the sentinel back into DUB)
-            hipLaunchKernelGGL((k_lu_transpose<NV, false>), tgrid, dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p,
+            // DU back into the block layout (and the sentinel back into DUB and DUF)
+            hipLaunchKernelGGL((k_lu_transpose<NV, false, TJ>), tgrid, dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p,
                                F::DUB, F::DUB + NV, F::DUB + NV, F::DUB + NV);
         }
         return;
     }
+    // the fenced sweep's plans always use 8x8 tiles
     if (forward)
         hipLaunchKernelGGL((k_lusgs_sweep<NV, true>), dim3((unsigned)L.n_fwd), dim3(LU_NT), 0, st,
                            L.blks, L.fwd, L.prog_fwd, L.prog_bwd, L.n_bwd, L.err, p);
@@ -670,47 +673,59 @@ static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, h
 }
 
 // this iteration's sweep inputs into sweep order: U, Res, Lc, Lv and the diagonal
-template <int NV>
+template <int NV, int TJ>
 static void pack_nv(const LuPlan& L, const Par& p, hipStream_t st) {
     using F = Pk<NV>;
-    hipLaunchKernelGGL((k_lu_transpose<NV, true>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin, (unsigned)(F::DUF + (F::S - F::LC))),
+    hipLaunchKernelGGL((k_lu_transpose<NV, true, TJ>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin, (unsigned)(F::DUF + (F::S - F::LC))),
                        dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p, F::U, F::DUF, F::LC, F::S);
 }
 
-void launch_lusgs_pack(const LuPlan& L, const Par& p, hipStream_t st) {
-    if (p.NV == 5) pack_nv<5>(L, p, st);
-    else if (p.NV == 6) pack_nv<6>(L, p, st);
-    else pack_nv<7>(L, p, st);
+// the geometry the sweeps read (S, n), into sweep order once per plan
+template <int NV, int TJ>
+static void pack_geometry_nv(const LuPlan& L, const Par& p, hipStream_t st) {
+    using F = Pk<NV>;
+    hipLaunchKernelGGL((k_lu_transpose<NV, true, TJ>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin, (unsigned)(F::NF - F::S)),
+                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p, F::S, F::NF, F::NF, F::NF);
 }
 
-void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st) {
-    if (p.NV == 5) sweep_nv<5>(L, p, forward, fenced, st);
-    else if (p.NV == 6) sweep_nv<6>(L, p, forward, fenced, st);
-    else sweep_nv<7>(L, p, forward, fenced, st);
+// NVAR and the plan's tile width as template arguments
+template <template <int, int> class K, typename... A>
+static void by_shape(int NV, int tj, A&&... a) {
+    switch (NV * 100 + tj) {
+    case 504: K<5, 4>::run(a...); break;
+    case 508: K<5, 8>::run(a...); break;
+    case 516: K<5, 16>::run(a...); break;
+    case 604: K<6, 4>::run(a...); break;
+    case 608: K<6, 8>::run(a...); break;
+    case 616: K<6, 16>::run(a...); break;
+    case 704: K<7, 4>::run(a...); break;
+    case 708: K<7, 8>::run(a...); break;
+    default: K<7, 16>::run(a...); break;
+    }
 }
+template <int NV, int TJ> struct SweepK { static void run(const LuPlan& L, const Par& p, bool f, bool fe, hipStream_t st) { sweep_nv<NV, TJ>(L, p, f, fe, st); } };
+template <int NV, int TJ> struct PackK { static void run(const LuPlan& L, const Par& p, hipStream_t st) { pack_nv<NV, TJ>(L, p, st); } };
+template <int NV, int TJ> struct GeoK { static void run(const LuPlan& L, const Par& p, hipStream_t st) { pack_geometry_nv<NV, TJ>(L, p, st); } };
+
+void launch_lusgs_pack(const LuPlan& L, const Par& p, hipStream_t st) { by_shape<PackK>(p.NV, L.tj, L, p, st); }
+
+void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st) {
+    by_shape<SweepK>(p.NV, fenced ? 8 : L.tj, L, p, forward, fenced, st);
+}
+
+bool lusgs_tile_ok(int tj) { return tj == 4 || tj == 8 || tj == 16; }
 
 int lusgs_fields(int NV) { return 4 * NV + 21; }
 int lusgs_windows(int max_cells_i) { return (max_cells_i + PK_W - 1) / PK_W; }
 
-// the geometry the sweeps read (S, n), into sweep order once per plan, and the sentinel into DUF
-// and DUB (also after a sweep gave up waiting)
-template <int NV>
-static void pack_geometry_nv(const LuPlan& L, const Par& p, hipStream_t st) {
-    using F = Pk<NV>;
-    hipLaunchKernelGGL((k_lu_transpose<NV, true>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin, (unsigned)(F::NF - F::S)),
-                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p, F::S, F::NF, F::NF, F::NF);
-}
+// the sentinel into DUF and DUB (also after a sweep gave up waiting)
 template <int NV>
 static void reset_nv(const LuPlan& L, hipStream_t st) {
     using F = Pk<NV>;
     hipLaunchKernelGGL(k_fill_sentinel<NV>, dim3(1024), dim3(256), 0, st, L.pk, L.P);
 }
 
-void launch_lusgs_pack_geometry(const LuPlan& L, const Par& p, hipStream_t st) {
-    if (p.NV == 5) pack_geometry_nv<5>(L, p, st);
-    else if (p.NV == 6) pack_geometry_nv<6>(L, p, st);
-    else pack_geometry_nv<7>(L, p, st);
-}
+void launch_lusgs_pack_geometry(const LuPlan& L, const Par& p, hipStream_t st) { by_shape<GeoK>(p.NV, L.tj, L, p, st); }
 
 void launch_lusgs_reset(const LuPlan& L, int NV, hipStream_t st) {
     if (NV == 5) reset_nv<5>(L, st);

@@ -50,6 +50,7 @@ struct Engine {
     hipGraph_t graph[OCFD_NSTAGES] = {};
     std::vector<hipEvent_t> events;           // stage boundaries of ocfd_step_timed
     int lu_schedule = OCFD_LU_PIPELINED;     // LU-SGS sweep schedule
+    int lu_tj = 8;                            // tile width of the pipelined sweep
     LuPlan lu{};                              // pipelined-sweep tables (built on first use)
     int lu_nv = 0;
     std::vector<void*> lu_allocs;
@@ -362,7 +363,8 @@ int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) {
     std::vector<Blk> blks;
     for (const BlockRef& r : refs) blks.push_back(r.hb->d);
     long long P = 0;
-    std::vector<LuTask> f = lusgs_tasks(blks, true, lusgs_fields(NV), &P), b = lusgs_tasks(blks, false, lusgs_fields(NV), nullptr);
+    const int tj = e.lu_schedule == OCFD_LU_PIPELINED_FENCED ? 8 : e.lu_tj;
+    std::vector<LuTask> f = lusgs_tasks(blks, true, lusgs_fields(NV), tj, &P), b = lusgs_tasks(blks, false, lusgs_fields(NV), tj, nullptr);
     std::vector<int> minus(std::max(f.size(), b.size()), -1), zero(4, 0);
     std::vector<int> mf(minus.begin(), minus.begin() + f.size()), mb(minus.begin(), minus.begin() + b.size());
     Blk* pb; LuTask *pf, *pw; int *qf, *qb, *pe;
@@ -386,6 +388,7 @@ int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) {
     int maxnx = 0;
     for (const Blk& q : blks) maxnx = std::max(maxnx, q.nx - 1);
     e.lu.nwin = lusgs_windows(maxnx);
+    e.lu.tj = tj;
     launch_lusgs_pack_geometry(e.lu, p0, e.st);
     launch_lusgs_reset(e.lu, NV, e.st);
     e.lu_nv = NV;
@@ -545,6 +548,44 @@ int engine_step(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, int
             }
         }
     }
+    return OCFD_OK;
+}
+
+int engine_set_lu_tile(Engine& e, int tj) {
+    if (!lusgs_tile_ok(tj)) return fail(OCFD_EARG, "LU-SGS tile width must be 4, 8 or 16");
+    if (tj != e.lu_tj) { drop_graphs(e); drop_lu_plan(e); e.lu_tj = tj; }
+    return OCFD_OK;
+}
+
+int engine_set_lu_schedule(Engine& e, int schedule) {
+    if (schedule != OCFD_LU_PIPELINED && schedule != OCFD_LU_HYPERPLANE && schedule != OCFD_LU_PIPELINED_FENCED)
+        return fail(OCFD_EARG, "unknown LU-SGS schedule");
+    // the fenced sweep plans 8x8 tiles, the team sweep the engine's tile width
+    if (schedule != e.lu_schedule) { drop_graphs(e); drop_lu_plan(e); e.lu_schedule = schedule; }
+    return OCFD_OK;
+}
+
+// The LU-SGS tile-shape autotuner: niter ordinary iterations with each tile shape (4x16, 8x8,
+// 16x4 (j,k) columns), timing the two sweeps of each with HIP events, then the fastest shape is
+// kept.  Every shape gives the same result bit for bit, so the solve advances exactly as by
+// 3 niter plain iterations.
+int engine_autotune(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, int niter, int* best, double* ms3) {
+    if (e.lu_schedule != OCFD_LU_PIPELINED) return fail(OCFD_EARG, "tile shapes apply to the pipelined LU-SGS schedule");
+    if (niter < 1) return fail(OCFD_EARG, "niter < 1");
+    static const int shapes[3] = {4, 8, 16};
+    double t[3];
+    int r;
+    for (int q = 0; q < 3; q++) {
+        if ((r = engine_set_lu_tile(e, shapes[q])) || (r = engine_prepare(e, refs, p0, true))) return r;
+        double sm[OCFD_NSTAGES];
+        if ((r = engine_step(e, refs, p0, niter, sm))) return r;
+        t[q] = (sm[OCFD_STAGE_LU_FORWARD] + sm[OCFD_STAGE_LU_BACKWARD]) / niter;
+        if (ms3) ms3[q] = t[q];
+    }
+    int k = 0;
+    for (int q = 1; q < 3; q++) if (t[q] < t[k]) k = q;
+    if ((r = engine_set_lu_tile(e, shapes[k]))) return r;
+    if (best) *best = shapes[k];
     return OCFD_OK;
 }
 
@@ -1031,9 +1072,28 @@ int ocfd_set_graph(ocfd_ctx* c, int32_t enable) {
 
 int ocfd_set_lu_schedule(ocfd_ctx* c, int32_t schedule) {
     if (!c) return fail(OCFD_EARG, "null context");
-    if (schedule != OCFD_LU_PIPELINED && schedule != OCFD_LU_HYPERPLANE && schedule != OCFD_LU_PIPELINED_FENCED)
-        return fail(OCFD_EARG, "unknown LU-SGS schedule");
-    if (schedule != c->e.lu_schedule) { drop_graphs(c->e); c->e.lu_schedule = schedule; }
+    return engine_set_lu_schedule(c->e, schedule);
+}
+
+int ocfd_set_lu_tile(ocfd_ctx* c, int32_t tj) {
+    if (!c) return fail(OCFD_EARG, "null context");
+    return engine_set_lu_tile(c->e, tj);
+}
+
+int ocfd_get_lu_tile(ocfd_ctx* c, int32_t* tj) {
+    if (!c || !tj) return fail(OCFD_EARG, "null context or output");
+    *tj = c->e.lu_tj;
+    return OCFD_OK;
+}
+
+int ocfd_autotune_lu(ocfd_ctx* c, int32_t niter, int32_t* best_tj, double* ms) {
+    int r = prepare(c);
+    if (r) return r;
+    for (HostBlock& hb : c->blocks) if ((r = check_subs(c, hb))) return r;
+    int best = 0;
+    if ((r = engine_autotune(c->e, c->refs(), c->p, niter, &best, ms))) return r;
+    if (best_tj) *best_tj = best;
+    for (int it = 0; it < 3 * niter; it++) { c->tt = c->tt + c->p.dt_global; c->kstep = c->kstep + 1; }
     return OCFD_OK;
 }
 
@@ -1104,6 +1164,24 @@ int ocfd_batch_step(ocfd_batch* b, int32_t niter) {
     if ((r = engine_step(b->e, refs, p0, niter, nullptr))) return r;
     for (ocfd_ctx* c : b->members)
         for (int it = 0; it < niter; it++) { c->tt = c->tt + c->p.dt_global; c->kstep = c->kstep + 1; }
+    return OCFD_OK;
+}
+
+int ocfd_batch_set_lu_tile(ocfd_batch* b, int32_t tj) {
+    if (!b) return fail(OCFD_EARG, "null batch");
+    return engine_set_lu_tile(b->e, tj);
+}
+
+int ocfd_batch_autotune_lu(ocfd_batch* b, int32_t niter, int32_t* best_tj, double* ms) {
+    if (!b) return fail(OCFD_EARG, "null batch");
+    if (b->members.empty()) return fail(OCFD_EARG, "empty batch");
+    const Par& p0 = b->members[0]->p;
+    std::vector<BlockRef> refs = b->refs();
+    int best = 0, r;
+    if ((r = engine_autotune(b->e, refs, p0, niter, &best, ms))) return r;
+    if (best_tj) *best_tj = best;
+    for (ocfd_ctx* c : b->members)
+        for (int it = 0; it < 3 * niter; it++) { c->tt = c->tt + c->p.dt_global; c->kstep = c->kstep + 1; }
     return OCFD_OK;
 }
 

@@ -157,6 +157,18 @@ module ocfd_ec
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function ocfd_set_lu_tile(ctx, tj) bind(C, name="ocfd_set_lu_tile")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tj
+    end function
+    integer(c_int) function ocfd_autotune_lu(ctx, niter, best_tj, ms) bind(C, name="ocfd_autotune_lu")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: niter
+      integer(c_int32_t), intent(out) :: best_tj
+      real(c_double), intent(out) :: ms(3)
+    end function
   end interface
 
 contains

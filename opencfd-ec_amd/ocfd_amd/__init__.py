@@ -56,10 +56,11 @@ EXPORTS = [
     "ocfd_setup_geometry", "ocfd_upload_U", "ocfd_download_U", "ocfd_download_field", "ocfd_init_flow",
     "ocfd_step_lusgs", "ocfd_residual", "ocfd_update_limit", "ocfd_bc", "ocfd_halo", "ocfd_force_vt_kw",
     "ocfd_res_norms", "ocfd_forces", "ocfd_block_status", "ocfd_get_counters", "ocfd_set_graph", "ocfd_sync",
-    "ocfd_step_timed", "ocfd_set_lu_schedule",
+    "ocfd_step_timed", "ocfd_set_lu_schedule", "ocfd_set_lu_tile", "ocfd_get_lu_tile", "ocfd_autotune_lu",
     "ocfd_batch_create", "ocfd_batch_destroy", "ocfd_batch_add", "ocfd_batch_remove", "ocfd_batch_step",
-    "ocfd_batch_set_graph", "ocfd_mem_bytes",
+    "ocfd_batch_set_graph", "ocfd_batch_set_lu_tile", "ocfd_batch_autotune_lu", "ocfd_mem_bytes",
 ]
+LU_TILES = (4, 8, 16)
 LU_PIPELINED, LU_HYPERPLANE, LU_PIPELINED_FENCED = 0, 1, 2
 
 # stages of one iteration (OCFD_STAGE_* in include/ocfd_ec.h)
@@ -94,6 +95,11 @@ def lib():
         L.ocfd_set_graph.argtypes = [vp, i32]
         L.ocfd_step_timed.argtypes = [vp, i32, dp]
         L.ocfd_set_lu_schedule.argtypes = [vp, i32]
+        L.ocfd_set_lu_tile.argtypes = [vp, i32]
+        L.ocfd_get_lu_tile.argtypes = [vp, ip]
+        L.ocfd_autotune_lu.argtypes = [vp, i32, ip, dp]
+        L.ocfd_batch_set_lu_tile.argtypes = [vp, i32]
+        L.ocfd_batch_autotune_lu.argtypes = [vp, i32, ip, dp]
         L.ocfd_batch_create.argtypes = [C.POINTER(vp)]
         L.ocfd_batch_destroy.argtypes = [vp]
         L.ocfd_batch_add.argtypes = [vp, vp]
@@ -322,6 +328,18 @@ class Solver:
         return dict(zip(STAGES, ms.tolist()))
     def set_graph(self, on: bool): _check(lib().ocfd_set_graph(self._h, 1 if on else 0))
     def set_lu_schedule(self, schedule: int): _check(lib().ocfd_set_lu_schedule(self._h, schedule))
+    def set_lu_tile(self, tj: int): _check(lib().ocfd_set_lu_tile(self._h, tj))
+
+    def lu_tile(self) -> int:
+        v = C.c_int32()
+        _check(lib().ocfd_get_lu_tile(self._h, C.byref(v)))
+        return v.value
+
+    def autotune_lu(self, n: int = 2):
+        """n iterations per LU tile shape; keeps the fastest.  Returns (tj, {tj: ms per sweep pair})."""
+        best, ms = C.c_int32(), np.zeros(3)
+        _check(lib().ocfd_autotune_lu(self._h, n, C.byref(best), _dp(ms)))
+        return best.value, dict(zip(LU_TILES, ms.tolist()))
 
     def norms(self):
         mx = np.zeros(self.nv); rms = np.zeros(self.nv)
@@ -384,6 +402,15 @@ class Batch:
 
     def set_graph(self, on: bool):
         _check(lib().ocfd_batch_set_graph(self._h, 1 if on else 0))
+
+    def set_lu_tile(self, tj: int):
+        _check(lib().ocfd_batch_set_lu_tile(self._h, tj))
+
+    def autotune_lu(self, n: int = 2):
+        """n iterations per LU tile shape for every member; keeps the fastest."""
+        best, ms = C.c_int32(), np.zeros(3)
+        _check(lib().ocfd_batch_autotune_lu(self._h, n, C.byref(best), _dp(ms)))
+        return best.value, dict(zip(LU_TILES, ms.tolist()))
 
 
 def solver_from_case(case, precision: int = 0, states: Optional[Sequence[np.ndarray]] = None) -> Solver:

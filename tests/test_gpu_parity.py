@@ -146,20 +146,23 @@ def test_c_grid_257x97x33(flux, iters, path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("schedule", ["pipelined", "fenced"])
+@pytest.mark.parametrize("schedule", ["pipelined", "pipelined_4x16", "pipelined_16x4", "fenced"])
 @pytest.mark.parametrize("name", ["channel_roe_muscl3", "channel_cut6", "cgrid_sa_limiter", "cgrid_sst_muscl3_ausm",
                                   "cgrid_laminar_vl", "cgrid_sa_cut12"])
 def test_lu_schedules_agree(name, schedule):
-    """The pipelined tile schedule of the LU-SGS sweeps and the reference's hyperplane order give
-    the same state bit for bit, iteration after iteration (graph replay included)."""
-    print(f"\n[{name}]", flush=True)
+    """The pipelined tile schedule of the LU-SGS sweeps (every tile shape) and the reference's
+    hyperplane order give the same state bit for bit, iteration after iteration (graph replay
+    included)."""
+    print(f"\n[{name} {schedule}]", flush=True)
     cs = CASES[name]
     case = cs.builder()
     U0 = C.perturbed_state(case, seed=cs.seed, amp=cs.amp)
     a = A.solver_from_case(case, states=U0)
     b = A.solver_from_case(case, states=U0)
     a.set_lu_schedule(A.LU_HYPERPLANE)
-    b.set_lu_schedule(A.LU_PIPELINED if schedule == "pipelined" else A.LU_PIPELINED_FENCED)
+    b.set_lu_schedule(A.LU_PIPELINED_FENCED if schedule == "fenced" else A.LU_PIPELINED)
+    if schedule.startswith("pipelined_"):
+        b.set_lu_tile(int(schedule.split("_")[1].split("x")[0]))
     b.set_graph(True)
     for it in range(4):
         a.step(1); b.step(1)
@@ -188,3 +191,48 @@ def test_pipelined_lu_on_reused_memory():
     for rep in range(3):
         print(f"\n[pipelined run {rep}]", flush=True)
         assert np.array_equal(run(A.LU_PIPELINED), ref), rep
+
+
+@pytest.mark.gpu
+def test_autotune_lu_advances_like_plain_iterations():
+    """The tile-shape autotuner runs ordinary iterations (n per shape): afterwards the state,
+    counters and norms equal 3 n plain iterations, and the shape kept is one it timed."""
+    cs = CASES["cgrid_sa_cut12"]
+    case = cs.builder()
+    U0 = C.perturbed_state(case, seed=cs.seed, amp=cs.amp)
+    a = A.solver_from_case(case, states=U0)
+    b = A.solver_from_case(case, states=U0)
+    a.step(6)
+    tj, ms = b.autotune_lu(2)
+    print(f"\nkept tj={tj}, ms per sweep pair {ms}", flush=True)
+    assert tj in A.LU_TILES and b.lu_tile() == tj
+    assert all(v > 0 for v in ms.values())
+    assert ms[tj] == min(ms.values())
+    for no in range(1, len(case.blocks) + 1):
+        assert np.array_equal(a.download_U(no), b.download_U(no)), f"block {no}"
+    assert a.counters() == b.counters()
+    assert np.array_equal(a.norms()[1], b.norms()[1])
+    b.step(2); a.step(2)
+    for no in range(1, len(case.blocks) + 1):
+        assert np.array_equal(a.download_U(no), b.download_U(no)), f"block {no} after two more"
+
+
+@pytest.mark.gpu
+def test_batch_autotune_lu():
+    """A batch's autotuner advances every member by 3 n iterations, each equal to the member
+    stepped alone."""
+    cases = [C.naca_cgrid(65, 25, 9, wall_spacing=0.02, Iflag_Flux=4),
+             C.cut_blocks(C.naca_cgrid(65, 25, 9, m=0.02, p=0.4, Ma=0.7, AoA=-1.0, wall_spacing=0.02, Iflag_Flux=4), [[[17, 40], [12], [5]]])]
+    S = [A.solver_from_case(cs) for cs in cases]
+    bt = A.Batch()
+    for s in S:
+        bt.add(s)
+    tj, ms = bt.autotune_lu(1)
+    assert tj in A.LU_TILES
+    for cs, s in zip(cases, S):
+        ref = A.solver_from_case(cs)
+        ref.step(3)
+        for no in range(1, len(cs.blocks) + 1):
+            assert np.array_equal(s.download_U(no), ref.download_U(no))
+        assert s.counters()[0] == 3
+    bt.close()

@@ -226,7 +226,9 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=1, help="replay each stage as a captured hipGraph")
     ap.add_argument("--lu", choices=("pipelined", "hyperplane", "fenced"), default="pipelined", help="LU-SGS sweep schedule")
     ap.add_argument("--autotune", type=int, default=1,
-                    help="iterations per LU-SGS tile shape of the autotuner before the warm-up (0: keep 8x8)")
+                    help="iterations per LU-SGS tile shape of the autotuner before the warm-up (0: no tuning)")
+    ap.add_argument("--lu-tile", type=int, choices=(4, 8, 16), default=None,
+                    help="fixed LU-SGS tile width instead of autotuning (profiling runs)")
     ap.add_argument("--cpu-iters", type=int, default=20, help="iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
@@ -260,14 +262,18 @@ def main() -> int:
     case = build_workload(args.workload, args.flux)
     cells = ncells(case)
     precision = 0 if args.precision == "fp64" else 1
+    print(f"# {args.workload}: {cells} cells in {len(case.blocks)} blocks; setting up on the device", file=sys.stderr, flush=True)
     s = A.solver_from_case(case, precision=precision)   # geometry, wall distance, freestream: on the device
     nv = s.nv
     s.set_graph(bool(args.graph))
     s.set_lu_schedule({"pipelined": A.LU_PIPELINED, "hyperplane": A.LU_HYPERPLANE, "fenced": A.LU_PIPELINED_FENCED}[args.lu])
     tune = None
-    if args.lu == "pipelined" and args.autotune > 0:
+    if args.lu_tile:
+        s.set_lu_tile(args.lu_tile)
+    elif args.lu == "pipelined" and args.autotune > 0:
         tj, tms = s.autotune_lu(args.autotune)          # ordinary iterations, untimed
         tune = dict(kept=f"{tj}x{64 // tj}", ms_per_sweep_pair={f"{t}x{64 // t}": round(v, 4) for t, v in tms.items()})
+        print(f"# LU tile autotune: {tune}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         s.step(1)
     s.sync()
@@ -279,6 +285,7 @@ def main() -> int:
     ms_per_step = wall * 1e3 / args.steps
     value = cells * args.steps / wall / 1e6
 
+    s_tile = s.lu_tile() if args.lu == "pipelined" else 8
     mx, rms = s.norms()
     overlimit = max(s.status(no)[0] for no in range(1, len(case.blocks) + 1))
 
@@ -289,13 +296,19 @@ def main() -> int:
         t = sum(stage_ms[x] for x in stages) / args.steps
         alg = w * 8.0 * cells
         passes[pname] = dict(ms=t, alg_bytes=alg, gbs=alg / (t * 1e-3) / 1e9)
-    dom = max(passes, key=lambda k: passes[k]["ms"])
+    # the roofline's kernel: the forward LU-SGS sweep, one launch per iteration
+    # (k_lusgs_team<NV,true>), so its stage events time exactly that kernel.  On the 257x97x33
+    # C-grid it is the largest single kernel of the iteration (the residual pass is about twenty
+    # kernels, the largest of them a fifth of the sweep's time); on the large multi-block grids the
+    # residual kernels dominate, and `passes` reports every pass.
+    dom = "lu_forward" if args.lu != "hyperplane" else max(passes, key=lambda k: passes[k]["ms"])
     d = passes[dom]
     measured = MEASURED_TRAFFIC.get((args.workload, args.precision), {}).get(dom)
     roofline = dict(bound="hbm", achieved=round(d["gbs"], 2), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(d["gbs"] / HBM_PEAK_GBS, 5),
                     traffic=round(measured * cells) if measured else None,
-                    kernel=dom, ms_per_launch=round(d["ms"], 4), alg_bytes_per_launch=d["alg_bytes"])
+                    kernel=f"k_lusgs_team<{nv}, true>" if dom == "lu_forward" else dom, stage=dom,
+                    ms_per_launch=round(d["ms"], 4), alg_bytes_per_launch=d["alg_bytes"])
     iter_bytes = sum(words) * 8.0 * cells
     out = {
         "metric": "Mcell-iter/s/GPU + %HBM roofline",
@@ -319,7 +332,7 @@ def main() -> int:
             "flux": FLUX_NAMES[int(case.params["Iflag_Flux"])],
             "graph": bool(args.graph),
             "lu_schedule": args.lu,
-            "lu_tile": tune["kept"] if tune else ("8x8" if args.lu != "hyperplane" else None),
+            "lu_tile": (tune["kept"] if tune else f"{s_tile}x{64 // s_tile}") if args.lu != "hyperplane" else None,
             "precision": args.precision,
         },
         "lu_autotune": tune,

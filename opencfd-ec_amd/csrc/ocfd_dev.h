@@ -80,18 +80,33 @@ struct BSet {
     const int* tile0;
 };
 
+// Workgroups of a one-dimensional grid are handed to the 8 XCDs in turn (workgroup t to XCD
+// t mod 8).  Renumbering so that XCD x runs the x-th eighth of the tiles in order keeps the
+// j/k neighbour rows of a stencil, which are also tiles of the same launch, in that XCD's L2;
+// the last n mod 8 tiles keep their numbers (a bijection on [0, n)).
+__device__ __forceinline__ int xcd_tile(int t, int n) {
+    const int full = n & ~7;
+    return t < full ? (t & 7) * (full >> 3) + (t >> 3) : t;
+}
+
 template <int SP>
 __device__ __forceinline__ bool locate(const BSet& S, int& bi, int& i, int& j, int& k) {
-    const int t = (int)blockIdx.x;
+    const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     bi = S.tile_blk[t];
     int n0, n1, n2, o0, o1, o2;
     space_dims(SP, S.blk[bi], n0, n1, n2, o0, o1, o2);
     const int lt = t - S.tile0[bi], T0 = (n0 + 63) / 64;
     const int a = (lt % T0) * 64 + (int)threadIdx.x, r = lt / T0;
     i = o0 + a;
-    j = o1 + r % n1;
-    k = o2 + r / n1;
-    (void)n2;
+    if (SP == SP_FK) {
+        // k-face kernels read stencils along k: rows run k fastest, so the eight rows of a stencil
+        // are neighbouring tiles (and, after the renumbering above, tiles of the same XCD)
+        k = o2 + r % n2;
+        j = o1 + r / n2;
+    } else {
+        j = o1 + r % n1;
+        k = o2 + r / n1;
+    }
     return a < n0;
 }
 

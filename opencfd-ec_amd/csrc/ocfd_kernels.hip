@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(64) k_sst_finish(BSet S) {
 
 // ---------------------------------------------------------------------------------------------
 // flux_inviscous_i/j/k (sub_Residual.f90:182-705): one thread per face, face flux stored
-template <int DIR, typename T>
+template <int DIR, typename T, int RECON>
 __global__ void __launch_bounds__(64) k_inv_flux(BSet S) {
     OCFD_LOCATE(SP_FI + DIR)
     size_t s = IX(b, i, j, k);
@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(64) k_inv_flux(BSet S) {
     T s3x = s1y * s2z - s1z * s2y, s3y = s1z * s2x - s1x * s2z, s3z = s1x * s2y - s1y * s2x;
     const long st = DIR == 0 ? 1 : DIR == 1 ? (long)b.NI : (long)b.NI * b.NJ;
     T UL[5], UR[5];
-    if (p.recon == 0) {
+    if constexpr (RECON == 0) {
         for (int m = 0; m < 5; m++) {
             const T* a = prim_of<T>(b, m) + s;
             T u[8];
@@ -378,13 +378,15 @@ __global__ void __launch_bounds__(64) k_inv_flux(BSet S) {
         }
     } else {
         T U0[5][8];
+#pragma unroll
         for (int m = 0; m < 5; m++) {
             const double* a = b.U[m] + s;
 #pragma unroll
             for (int q = 0; q < 8; q++) U0[m][q] = T(a[(q - 4) * st]);
         }
         T QL[5], QR[5];
-        if (p.recon == 1) {
+        if constexpr (RECON == 1) {
+#pragma unroll
             for (int m = 0; m < 5; m++) { QL[m] = scheme_fP(U0[m] + 3, scheme); QR[m] = scheme_fm(U0[m] + 3, scheme); }
         } else {
             // Reconstuction_Characteristic (sub_Residual.f90:1291-1347)
@@ -403,6 +405,7 @@ __global__ void __launch_bounds__(64) k_inv_flux(BSet S) {
                           {-tmp3 * v1, o, z, -v1 * tmp5, v1 * tmp5}, {-tmp3 * w1, z, o, -w1 * tmp5, w1 * tmp5},
                           {-tmp3 * V2, v1, w1, ((c1 * u1 - V2) - tmp0) * tmp5, ((c1 * u1 + V2) + tmp0) * tmp5}};
             T VL[5], VR[5];
+#pragma unroll
             for (int m = 0; m < 5; m++) {
                 T V0[8];
                 for (int q = 0; q < 8; q++) { T a = T(0); for (int jj = 0; jj < 5; jj++) a = a + S[m][jj] * U0[jj][q]; V0[q] = a; }
@@ -1159,11 +1162,19 @@ void set_pre(const SetPlan& P, hipStream_t st) {
     }
 }
 
+template <typename T, int R>
+static void inviscid_r(const SetPlan& P, hipStream_t st) {
+    launch_space<SP_FI>(k_inv_flux<0, T, R>, P, st);
+    launch_space<SP_FJ>(k_inv_flux<1, T, R>, P, st);
+    launch_space<SP_FK>(k_inv_flux<2, T, R>, P, st);
+}
+// the reconstruction variable set (IFlag_Reconstruction) as a template argument: each kernel
+// holds only its own reconstruction
 template <typename T>
 static void inviscid_t(const SetPlan& P, hipStream_t st) {
-    launch_space<SP_FI>(k_inv_flux<0, T>, P, st);
-    launch_space<SP_FJ>(k_inv_flux<1, T>, P, st);
-    launch_space<SP_FK>(k_inv_flux<2, T>, P, st);
+    if (P.p0.recon == 0) inviscid_r<T, 0>(P, st);
+    else if (P.p0.recon == 1) inviscid_r<T, 1>(P, st);
+    else inviscid_r<T, 2>(P, st);
     launch_subs(k_inv_kbc<T>, P, P.kbc, st);
 }
 void set_inviscid(const SetPlan& P, hipStream_t st) {

@@ -91,6 +91,40 @@ def test_iteration_matches_oracle(name, path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("path", ["dropin", "device_geometry"])
+def test_converged_forces_match_oracle(path):
+    """Converged forces, FP64: a 33x13x3 laminar NACA 0012 C-grid (Re 2000, Ma 0.5, AoA 2,
+    MUSCL3 + Roe, CFL 5) run 15000 iterations on the device and in the oracle, by which point
+    Res_rms(1) has dropped below 1e-10 of its first value and CL, CD move by less than 1e-9
+    relative per 1000 iterations; CL, CD and the pitching moment agree within 1e-8 relative."""
+    case = C.naca_cgrid(33, 13, 3, wall_spacing=0.02, Re=2000.0, Ma=0.5, AoA=2.0, Iflag_turbulence_model=0,
+                        Iflag_Scheme=5, Iflag_Flux=4, CFL=5.0)
+    o = oracle.Oracle(case)
+    o.init_flow()
+    if path == "dropin":
+        s = A.Solver(case.params)
+        for no, (b, t) in enumerate(zip(case.blocks, A.inc_tables(case)), start=1):
+            s.add_block_geom(no, b.dims, t, oracle_geometry(o, no, b.dims, False))
+        s.init_flow(); s.bc(); s.halo()
+    else:
+        s = A.solver_from_case(case)
+    s.set_graph(True)
+    o.step(1); s.step(1)
+    r0 = o.monitor()["res_rms"][0]
+    o.step(14999); s.step(14999)
+    st = o.monitor()
+    drop = st["res_rms"][0] / r0
+    f = s.forces()
+    got = {"CL": f[9], "CD": f[10], "CM": f[8]}
+    ref = {"CL": st["CL"], "CD": st["CD"], "CM": st["Ft"][8]}
+    print(f"\n[{path}] drop {drop:.2e}  " + "  ".join(f"{k} {got[k]:.15f} / {ref[k]:.15f}" for k in got), flush=True)
+    assert drop < 1e-10
+    for k in got:
+        assert abs(got[k] - ref[k]) <= 1e-8 * abs(ref[k]), (k, got[k], ref[k])
+    s.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name", ["channel_cut6", "cgrid_sa_limiter", "cgrid_sst_muscl3_ausm"])
 def test_graph_replay_matches_eager(name):
     """ocfd_step_lusgs with the iteration captured in a hipGraph equals the eager launches, replay

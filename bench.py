@@ -156,7 +156,8 @@ def serve_main(args) -> int:
     advance together through shared launches until their Res_rms(1) has dropped by --drop; the
     baseline runs the plain solver on one request at a time to the same criterion."""
     from ocfd_amd import serve as SV
-    reqs = SV.sweep_requests(args.serve, lam=args.rate, seed=5)
+    dims = SV.DIMS if args.serve_dims == "small" else SV.DIMS_SURVEY
+    reqs = SV.sweep_requests(args.serve, lam=args.rate, seed=5, dims=dims)
     srv = SV.Server(mem_budget=args.serve_budget * 1e9, round_iters=args.round_iters, drop=args.drop,
                     max_iters=args.max_iters, graph=bool(args.graph))
     log = (lambda m: print("# " + m, file=sys.stderr, flush=True)) if args.verbose else None
@@ -186,7 +187,7 @@ def serve_main(args) -> int:
         "dtype": "f64",
         "data": "synthetic requests generated in-process (NACA 4-digit C-grids, random Mach/AoA/Re)",
         "config": {"workload": f"naca_sweep_{args.serve}", "requests": args.serve, "arrival_rate_per_s": args.rate,
-                   "dims": [list(d) for d in SV.DIMS], "drop": args.drop, "round_iters": args.round_iters,
+                   "dims": [list(d) for d in dims], "drop": args.drop, "round_iters": args.round_iters,
                    "max_iters": args.max_iters, "mem_budget_GB": args.serve_budget},
         "served": {k2: (round(v, 4) if isinstance(v, float) else v) for k2, v in served.items()},
         "sequential": dict({k2: (round(v, 4) if isinstance(v, float) else v) for k2, v in sequential.items()},
@@ -244,6 +245,8 @@ def main() -> int:
     ap.add_argument("--max-iters", type=int, default=6000, help="serving: iteration cap per request")
     ap.add_argument("--serve-budget", type=float, default=200.0, help="serving: HBM budget of admitted requests, GB")
     ap.add_argument("--seq-sample", type=int, default=32, help="serving: requests the sequential baseline runs")
+    ap.add_argument("--serve-dims", choices=("small", "survey"), default="small",
+                    help="serving: request grids 97x33x9/129x49x13/193x65x17 (small) or 129x49x17/193x65x25/257x97x33")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (byte-counter calibration)")

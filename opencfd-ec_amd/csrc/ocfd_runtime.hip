@@ -35,6 +35,10 @@ struct HostBlock {
     double* force_acc = nullptr;     // 12 doubles: 9 accumulators + centroid
     std::vector<double> nodes[3];    // node coordinates awaiting ocfd_setup_geometry
     size_t bytes = 0;                // device memory of the block's arrays
+    // the block's arrays are carved out of one device allocation: a sizing pass measures it
+    char* slab = nullptr;
+    size_t slab_off = 0;
+    bool sizing = false;
     size_t n() const { return (size_t)d.NI * d.NJ * d.NK; }
 };
 
@@ -100,27 +104,25 @@ struct ocfd_batch {
 
 namespace {
 
+// n bytes of the block's slab, 256-byte aligned (the sizing pass only counts)
+void* carve(HostBlock& hb, size_t n) {
+    const size_t at = hb.slab_off;
+    hb.slab_off += (n + 255) & ~(size_t)255;
+    return hb.sizing ? nullptr : hb.slab + at;
+}
+
 int dalloc(HostBlock& hb, double** p, size_t n) {
-    void* q = nullptr;
-    hb.bytes += n * sizeof(double);
-    HIPCHK(hipMalloc(&q, n * sizeof(double)));
-    HIPCHK(hipMemset(q, 0, n * sizeof(double)));
-    hb.allocs.push_back(q);
-    *p = (double*)q;
+    *p = (double*)carve(hb, n * sizeof(double));
     return OCFD_OK;
 }
 
 int balloc(HostBlock& hb, uint8_t** p, size_t n) {
-    void* q = nullptr;
-    HIPCHK(hipMalloc(&q, n));
-    HIPCHK(hipMemset(q, 0, n));
-    hb.allocs.push_back(q);
-    *p = (uint8_t*)q;
+    *p = (uint8_t*)carve(hb, n);
     return OCFD_OK;
 }
 
-// allocate all device arrays of one block (the same padded box for every array)
-int alloc_block(ocfd_ctx* c, HostBlock& hb) {
+// the device arrays of one block (the same padded box for every array), carved out of the slab
+int carve_block(ocfd_ctx* c, HostBlock& hb) {
     Blk& b = hb.d;
     b.nx = hb.nx; b.ny = hb.ny; b.nz = hb.nz;
     b.NI = hb.nx + 7; b.NJ = hb.ny + 7; b.NK = hb.nz + 7;
@@ -160,11 +162,7 @@ int alloc_block(ocfd_ctx* c, HostBlock& hb) {
     if ((r = balloc(hb, &b.bcf[0], jk)) || (r = balloc(hb, &b.bcf[1], ik)) || (r = balloc(hb, &b.bcf[2], ij))) return r;
     size_t fsz[6] = {jk, ik, ij, jk, ik, ij};
     for (int f = 0; f < 6; f++) for (int q = 0; q < 3; q++) if ((r = dalloc(hb, &b.surf[f][q], fsz[f]))) return r;
-    void* fl = nullptr;
-    HIPCHK(hipMalloc(&fl, 16 * sizeof(int)));
-    HIPCHK(hipMemset(fl, 0, 16 * sizeof(int)));
-    hb.allocs.push_back(fl);
-    b.flags = (int*)fl;
+    b.flags = (int*)carve(hb, 16 * sizeof(int));
     if ((r = dalloc(hb, &hb.force_acc, 12))) return r;
     if (c->p.NV == 6) {
         size_t ncell = (size_t)(hb.nx - 1) * (hb.ny - 1) * (hb.nz - 1);
@@ -172,6 +170,25 @@ int alloc_block(ocfd_ctx* c, HostBlock& hb) {
         if ((r = dalloc(hb, &m, (ncell + 63) / 64))) return r;
         b.samask = reinterpret_cast<unsigned long long*>(m);
     }
+    return OCFD_OK;
+}
+
+// allocate all device arrays of one block: one zeroed allocation, so a block costs its arrays'
+// bytes and not one allocation granule per array (small blocks have about 130 arrays)
+int alloc_block(ocfd_ctx* c, HostBlock& hb) {
+    Blk& b = hb.d;
+    int r;
+    hb.sizing = true; hb.slab_off = 0;
+    if ((r = carve_block(c, hb))) return r;
+    const size_t total = hb.slab_off;
+    void* q = nullptr;
+    HIPCHK(hipMalloc(&q, total));
+    hb.allocs.push_back(q);
+    HIPCHK(hipMemset(q, 0, total));
+    hb.slab = (char*)q; hb.sizing = false; hb.slab_off = 0;
+    if ((r = carve_block(c, hb))) return r;
+    hb.bytes = total;
+    size_t n = hb.n();
     // initial values of allocate_mem_Blocks (sub_init.f90:496,507-508)
     std::vector<double> h(n, 1.0 / c->p.Re);
     HIPCHK(hipMemcpy(b.mu, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
@@ -1139,6 +1156,9 @@ int ocfd_batch_add(ocfd_batch* b, ocfd_ctx* c) {
     HIPCHK(hipStreamSynchronize(c->e.st));
     b->members.push_back(c);
     drop_plans(b->e);
+    // the member steps through the batch's launches now: its own tables (the sweep storage is
+    // about 380 B per cell) are rebuilt only if it is stepped alone again
+    drop_plans(c->e);
     return OCFD_OK;
 }
 

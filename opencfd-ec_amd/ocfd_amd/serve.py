@@ -54,6 +54,7 @@ class Result:
 
 
 DIMS = ((97, 33, 9), (129, 49, 13), (193, 65, 17))
+DIMS_SURVEY = ((129, 49, 17), (193, 65, 25), (257, 97, 33))
 
 
 def sweep_requests(n: int = 256, lam: float = 2.0, seed: int = 5, dims=DIMS, max_blocks: int = 8,
@@ -95,10 +96,16 @@ class Server:
 
     @staticmethod
     def lu_bytes(case) -> int:
-        """Device bytes of a request's share of the batch's LU-SGS tables (sweep-ordered copies of
-        the sweep inputs: (3 NVAR + 20) doubles per cell, about 6 % tile padding)."""
+        """Device bytes of a request's share of the batch's tables: the LU-SGS sweep storage
+        (4 NVAR + 21 doubles per lane and step of every 64-column (j,k) tile, a tile marching
+        nx - 1 + tj + tk - 2 steps; the widest tile shape) and 16 MB for the launch tables."""
         nv = C.nvar_of(case.params)
-        return int(case.ncell * 1.06 * (3 * nv + 20) * 8)
+        steps = 0
+        for b in case.blocks:
+            nx, ny, nz = b.dims
+            tiles = -(-(ny - 1) // 4) * -(-(nz - 1) // 16)
+            steps += max(tiles, -(-(ny - 1) // 16) * -(-(nz - 1) // 4)) * (nx - 1 + 18)
+        return int(steps * 64 * (4 * nv + 21) * 8 + 16e6)
 
     def run(self, requests: List[Request], clock: Callable[[], float] = time.monotonic,
             log: Optional[Callable[[str], None]] = None) -> List[Result]:

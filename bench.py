@@ -165,13 +165,14 @@ def serve_main(args) -> int:
     res = srv.run(reqs, log=log)
     wall = time.perf_counter() - t0
     served = SV.summarize(res, wall)
+    print(f"# served: {json.dumps(served)}", file=sys.stderr, flush=True)
     k = min(args.seq_sample, len(reqs))
     sub = sorted(reqs, key=lambda r: r.arrival)[:k]
     # the baseline starts each request as soon as the previous one is done (no idle gaps): its
     # solves/hour is an upper bound for the sequential solver at this arrival rate
     sub = [SV.Request(r.rid, 0.0, r.case, r.cells) for r in sub]
     t0 = time.perf_counter()
-    seq = srv.run_sequential(sub)
+    seq = srv.run_sequential(sub, log=log)
     swall = time.perf_counter() - t0
     sequential = SV.summarize(seq, swall)
     same = {r.rid: r for r in res}

@@ -57,7 +57,7 @@ struct Sub {
 // ---------------------------------------------------------------------------------------------
 // Block sets: every block of a context (of every case it holds) goes through the same launch.
 // A launch over an iteration space covers the tiles of all blocks; tile t belongs to block
-// tile_blk[t] and is that block's (t - tile0[block])-th tile of 64 points along i.
+// tile_blk[t] and is that block's (t - tile0[block])-th tile of 64 points (i fastest).
 
 enum Space { SP_BOX = 0, SP_CELL, SP_FI, SP_FJ, SP_FK, SP_N0, SP_COUNT };
 
@@ -95,9 +95,12 @@ __device__ __forceinline__ bool locate(const BSet& S, int& bi, int& i, int& j, i
     bi = S.tile_blk[t];
     int n0, n1, n2, o0, o1, o2;
     space_dims(SP, S.blk[bi], n0, n1, n2, o0, o1, o2);
-    const int lt = t - S.tile0[bi], T0 = (n0 + 63) / 64;
-    const int a = (lt % T0) * 64 + (int)threadIdx.x, r = lt / T0;
-    i = o0 + a;
+    // the block's points in i-fastest order, 64 per tile: a tile runs on into the next row when
+    // a row is shorter or not a multiple of 64 (cut blocks of 16-cell rows keep every lane busy)
+    const int a = (t - S.tile0[bi]) * 64 + (int)threadIdx.x;
+    if (a >= n0 * n1 * n2) return false;
+    const int r = a / n0;
+    i = o0 + (a - r * n0);
     if (SP == SP_FK) {
         // k-face kernels read stencils along k: rows run k fastest, so the eight rows of a stencil
         // are neighbouring tiles (and, after the renumbering above, tiles of the same XCD)
@@ -107,7 +110,7 @@ __device__ __forceinline__ bool locate(const BSet& S, int& bi, int& i, int& j, i
         j = o1 + r % n1;
         k = o2 + r / n1;
     }
-    return a < n0;
+    return true;
 }
 
 #define OCFD_LOCATE(SP)                                  \

@@ -471,7 +471,7 @@ int build_set_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) 
             int n0, n1, n2, o0, o1, o2;
             space_dims(sp, blks[bi], n0, n1, n2, o0, o1, o2);
             t0.push_back((int)tb.size());
-            long nt = (long)((n0 + 63) / 64) * n1 * n2;
+            long nt = ((long)n0 * n1 * n2 + 63) / 64;
             for (long a = 0; a < nt; a++) tb.push_back(bi);
         }
         P.ntiles[sp] = (int)tb.size();

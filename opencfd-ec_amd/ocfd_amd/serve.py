@@ -164,7 +164,8 @@ class Server:
         batch.close()
         return results
 
-    def run_sequential(self, requests: List[Request], clock: Callable[[], float] = time.monotonic) -> List[Result]:
+    def run_sequential(self, requests: List[Request], clock: Callable[[], float] = time.monotonic,
+                       log: Optional[Callable[[str], None]] = None) -> List[Result]:
         """The baseline: one request at a time on its own solver, in arrival order, same stop rule."""
         results: List[Result] = []
         t0 = clock()
@@ -188,6 +189,8 @@ class Server:
             results.append(Result(r.rid, r.arrival, adm, clock() - t0, iters, ratio <= self.drop, ratio,
                                   float(f[9]), float(f[10]), r.cells))
             s.close()
+            if log:
+                log(f"t={clock() - t0:7.2f}s sequential done {r.rid} cells={r.cells} iters={iters} drop={ratio:.2e}")
         return results
 
 

@@ -200,6 +200,8 @@ int ocfd_batch_destroy(ocfd_batch *batch);
 int ocfd_batch_add(ocfd_batch *batch, ocfd_ctx *ctx);
 int ocfd_batch_remove(ocfd_batch *batch, ocfd_ctx *ctx);
 int ocfd_batch_step(ocfd_batch *batch, int32_t niter);
+/* ocfd_step_timed for the batch's shared launches. */
+int ocfd_batch_step_timed(ocfd_batch *batch, int32_t niter, double *stage_ms);
 int ocfd_batch_set_graph(ocfd_batch *batch, int32_t enable);
 /* ocfd_set_lu_tile and ocfd_autotune_lu for the batch's shared launches (every member advances
  * by 3 niter iterations). */

@@ -1174,17 +1174,25 @@ int ocfd_batch_remove(ocfd_batch* b, ocfd_ctx* c) {
     return fail(OCFD_EARG, "context not in the batch");
 }
 
-int ocfd_batch_step(ocfd_batch* b, int32_t niter) {
+static int batch_step(ocfd_batch* b, int32_t niter, double* stage_ms) {
     if (!b) return fail(OCFD_EARG, "null batch");
     if (b->members.empty()) return OCFD_OK;
+    if (stage_ms && niter < 1) return fail(OCFD_EARG, "niter < 1");
     const Par& p0 = b->members[0]->p;
     std::vector<BlockRef> refs = b->refs();
     int r = engine_prepare(b->e, refs, p0, true);
     if (r) return r;
-    if ((r = engine_step(b->e, refs, p0, niter, nullptr))) return r;
+    if ((r = engine_step(b->e, refs, p0, niter, stage_ms))) return r;
     for (ocfd_ctx* c : b->members)
         for (int it = 0; it < niter; it++) { c->tt = c->tt + c->p.dt_global; c->kstep = c->kstep + 1; }
     return OCFD_OK;
+}
+
+int ocfd_batch_step(ocfd_batch* b, int32_t niter) { return batch_step(b, niter, nullptr); }
+
+int ocfd_batch_step_timed(ocfd_batch* b, int32_t niter, double* stage_ms) {
+    if (!stage_ms) return fail(OCFD_EARG, "null stage_ms");
+    return batch_step(b, niter, stage_ms);
 }
 
 int ocfd_batch_set_lu_tile(ocfd_batch* b, int32_t tj) {

@@ -58,7 +58,7 @@ EXPORTS = [
     "ocfd_res_norms", "ocfd_forces", "ocfd_block_status", "ocfd_get_counters", "ocfd_set_graph", "ocfd_sync",
     "ocfd_step_timed", "ocfd_set_lu_schedule", "ocfd_set_lu_tile", "ocfd_get_lu_tile", "ocfd_autotune_lu",
     "ocfd_batch_create", "ocfd_batch_destroy", "ocfd_batch_add", "ocfd_batch_remove", "ocfd_batch_step",
-    "ocfd_batch_set_graph", "ocfd_batch_set_lu_tile", "ocfd_batch_autotune_lu", "ocfd_mem_bytes",
+    "ocfd_batch_set_graph", "ocfd_batch_set_lu_tile", "ocfd_batch_autotune_lu", "ocfd_batch_step_timed", "ocfd_mem_bytes",
 ]
 LU_TILES = (4, 8, 16)
 LU_PIPELINED, LU_HYPERPLANE, LU_PIPELINED_FENCED = 0, 1, 2
@@ -100,6 +100,7 @@ def lib():
         L.ocfd_autotune_lu.argtypes = [vp, i32, ip, dp]
         L.ocfd_batch_set_lu_tile.argtypes = [vp, i32]
         L.ocfd_batch_autotune_lu.argtypes = [vp, i32, ip, dp]
+        L.ocfd_batch_step_timed.argtypes = [vp, i32, dp]
         L.ocfd_batch_create.argtypes = [C.POINTER(vp)]
         L.ocfd_batch_destroy.argtypes = [vp]
         L.ocfd_batch_add.argtypes = [vp, vp]
@@ -405,6 +406,12 @@ class Batch:
 
     def set_lu_tile(self, tj: int):
         _check(lib().ocfd_batch_set_lu_tile(self._h, tj))
+
+    def step_timed(self, n: int) -> Dict[str, float]:
+        """n iterations; device ms per stage summed over them (HIP events)."""
+        ms = np.zeros(len(STAGES))
+        _check(lib().ocfd_batch_step_timed(self._h, n, _dp(ms)))
+        return dict(zip(STAGES, ms.tolist()))
 
     def autotune_lu(self, n: int = 2):
         """n iterations per LU tile shape for every member; keeps the fastest."""

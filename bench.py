@@ -200,6 +200,31 @@ def serve_main(args) -> int:
     return 0
 
 
+def precision_check(args) -> int:
+    """CL, CD and CM of the workload after N iterations from freestream in FP64 and in mixed
+    precision (FP32 reconstruction, fluxes and SA terms), and their relative differences."""
+    case = build_workload(args.workload, args.flux)
+    out = {}
+    for prec in (0, 1):
+        s = A.solver_from_case(case, precision=prec)
+        s.set_graph(bool(args.graph))
+        s.step(1)
+        r0 = float(s.norms()[1][0])
+        done = 1
+        while done < args.precision_check:
+            n = min(50, args.precision_check - done)
+            s.step(n)
+            done += n
+            print(f"# precision {prec}: {done} iterations", file=sys.stderr, flush=True)
+        f = s.forces()
+        out["fp64" if prec == 0 else "mixed"] = dict(CL=float(f[9]), CD=float(f[10]), CM=float(f[8]),
+                                                     res_rms1_drop=float(s.norms()[1][0]) / r0)
+        s.close()
+    rel = {k: abs(out["mixed"][k] - out["fp64"][k]) / max(abs(out["fp64"][k]), 1e-30) for k in ("CL", "CD", "CM")}
+    print(json.dumps({"workload": args.workload, "iterations": args.precision_check, **out, "relative_difference": rel}), flush=True)
+    return 0
+
+
 def copy_probe(mib: int, reps: int) -> int:
     """A plain device copy of known size (the calibration kernel of the HBM byte counters): run
     under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each dispatch must read and write `mib` MiB."""
@@ -249,6 +274,9 @@ def main() -> int:
     ap.add_argument("--serve-dims", choices=("small", "survey"), default="small",
                     help="serving: request grids 97x33x9/129x49x13/193x65x17 (small) or 129x49x17/193x65x25/257x97x33")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--precision-check", type=int, default=0, metavar="N",
+                    help="only run N iterations of the workload in FP64 and in mixed precision from freestream "
+                         "and print the force coefficients of both")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -261,6 +289,8 @@ def main() -> int:
 
     if args.copy_probe:
         return copy_probe(args.copy_probe, 5)
+    if args.precision_check:
+        return precision_check(args)
     if args.serve:
         return serve_main(args)
     case = build_workload(args.workload, args.flux)

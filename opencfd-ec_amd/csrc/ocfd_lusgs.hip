@@ -380,7 +380,7 @@ __device__ __forceinline__ void team_neighbour(double* du, const double* Un_in, 
     for (int m = 0; m < NV; m++) Un[m] = Un_in[m];
 #pragma unroll
     for (int m = 5; m < NV; m++) if (Un[m] < 0.0) Un[m] = 0.0;     // limit_vt
-    comput_dfn<NV>(DF, Un, DUn, n[0], n[1], n[2], gamma);
+    comput_dfn_nr<NV>(DF, Un, DUn, n[0], n[1], n[2], gamma);
     const double wLc = wlu * Lcn;
     if (FWD) {
 #pragma unroll
@@ -500,12 +500,15 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
     // DU of the cell of lane src at step q - 1, from the terms of that step
     auto complete = [&](int q, const TeamIn<NV, FWD, D>& in, double* DUs) {
         const int pb = (q & 1) ^ 1;
+        // components 1-5 (and 7) share one diagonal, the SA component has its own
+        const double r0 = recip_nr(in.al[0]), r1 = NV == 6 ? recip_nr(in.al[1]) : 0.0;
 #pragma unroll
         for (int m = 0; m < NV; m++) {
-            const double al = in.al[(NV == 6 && m == 5) ? 1 : 0];
+            const bool own = NV == 6 && m == 5;
+            const double al = in.al[own ? 1 : 0], ra = own ? r1 : r0;
             double acc = FWD ? in.R[m] : 0.0;
             acc = ((acc + sh.dus[pb][0][m][src]) + sh.dus[pb][1][m][src]) + sh.dus[pb][2][m][src];
-            DUs[m] = FWD ? acc / al : in.R[m] + acc / al;
+            DUs[m] = FWD ? div_nr(acc, al, ra) : in.R[m] + div_nr(acc, al, ra);
         }
     };
     // wave 0 stores the DU of its own cell of step q - 1 (device-coherent: the elements a

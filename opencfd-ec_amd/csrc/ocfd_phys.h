@@ -22,18 +22,44 @@ template <typename T> __device__ __forceinline__ T minmod(T a, T b) {
 
 // u points at u(0) of a stencil u(-3..4) (face between u(0) and u(1)).
 // WENO5-JS left/right values (sub_scheme.f90:101-140)
+// Double division a / b with the divisor's reciprocal shared: v_rcp_f64 refined by two Newton
+// steps, then q = a r corrected once by the residual a - b q.  These are the steps of the
+// compiler's IEEE division without its operand pre-scaling (v_div_scale) and special-value
+// fix-up (v_div_fixup), which change nothing when a, b and a / b lie well inside the normal range
+// (|a| above about 1e-290 and |b| between about 1e-290 and 1e290), so the bits are those of a / b
+// there; divisions by one divisor then cost one reciprocal between them.
+__device__ __forceinline__ double recip_nr(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ double div_nr(double a, double b, double r) {
+    const double q = a * r;
+    const double e = __builtin_fma(-b, q, a);
+    return __builtin_fma(e, r, q);
+}
+
+// a / b in the compute type: the shared-reciprocal sequence above for double, the compiler's
+// division for float
+template <typename T> __device__ __forceinline__ T dvd(T a, T b) {
+    if constexpr (sizeof(T) == 8) return div_nr(a, b, recip_nr(b));
+    else return a / b;
+}
+
 template <typename T> __device__ __forceinline__ T weno5_L(const T* u) {
     const T ep = T(1.0e-6), C03 = T(3.0 / 10.0), C13 = T(3.0 / 5.0), C23 = T(1.0 / 10.0);
     const T c1312 = T(13.0 / 12.0), c14 = T(0.25);
     T S0 = c1312 * sq((u[0] - T(2) * u[1]) + u[2]) + c14 * sq((T(3) * u[0] - T(4) * u[1]) + u[2]);
     T S1 = c1312 * sq((u[-1] - T(2) * u[0]) + u[1]) + c14 * sq(u[-1] - u[1]);
     T S2 = c1312 * sq((u[-2] - T(2) * u[-1]) + u[0]) + c14 * sq((u[-2] - T(4) * u[-1]) + T(3) * u[0]);
-    T a0 = C03 / sq(ep + S0), a1 = C13 / sq(ep + S1), a2 = C23 / sq(ep + S2);
+    T a0 = dvd(C03, sq(ep + S0)), a1 = dvd(C13, sq(ep + S1)), a2 = dvd(C23, sq(ep + S2));
     T am = (a0 + a1) + a2;
     T q03 = (T(1.0 / 3.0) * u[0] + T(5.0 / 6.0) * u[1]) - T(1.0 / 6.0) * u[2];
     T q13 = (-T(1.0 / 6.0) * u[-1] + T(5.0 / 6.0) * u[0]) + T(1.0 / 3.0) * u[1];
     T q23 = (T(1.0 / 3.0) * u[-2] - T(7.0 / 6.0) * u[-1]) + T(11.0 / 6.0) * u[0];
-    return ((a0 * q03 + a1 * q13) + a2 * q23) / am;
+    return dvd((a0 * q03 + a1 * q13) + a2 * q23, am);
 }
 template <typename T> __device__ __forceinline__ T weno5_R(const T* u) {
     const T ep = T(1.0e-6), C03 = T(3.0 / 10.0), C13 = T(3.0 / 5.0), C23 = T(1.0 / 10.0);
@@ -41,12 +67,12 @@ template <typename T> __device__ __forceinline__ T weno5_R(const T* u) {
     T S0 = c1312 * sq((u[1] - T(2) * u[0]) + u[-1]) + c14 * sq((T(3) * u[1] - T(4) * u[0]) + u[-1]);
     T S1 = c1312 * sq((u[2] - T(2) * u[1]) + u[0]) + c14 * sq(u[2] - u[0]);
     T S2 = c1312 * sq((u[3] - T(2) * u[2]) + u[1]) + c14 * sq((u[3] - T(4) * u[2]) + T(3) * u[1]);
-    T a0 = C03 / sq(ep + S0), a1 = C13 / sq(ep + S1), a2 = C23 / sq(ep + S2);
+    T a0 = dvd(C03, sq(ep + S0)), a1 = dvd(C13, sq(ep + S1)), a2 = dvd(C23, sq(ep + S2));
     T am = (a0 + a1) + a2;
     T q03 = (T(1.0 / 3.0) * u[1] + T(5.0 / 6.0) * u[0]) - T(1.0 / 6.0) * u[-1];
     T q13 = (-T(1.0 / 6.0) * u[2] + T(5.0 / 6.0) * u[1]) + T(1.0 / 3.0) * u[0];
     T q23 = (T(1.0 / 3.0) * u[3] - T(7.0 / 6.0) * u[2]) + T(11.0 / 6.0) * u[1];
-    return ((a0 * q03 + a1 * q13) + a2 * q23) / am;
+    return dvd((a0 * q03 + a1 * q13) + a2 * q23, am);
 }
 
 // scheme_fP / scheme_fm (sub_scheme.f90:6-96); U points at u(0)
@@ -313,6 +339,26 @@ __device__ __forceinline__ void comput_dfn(double* DF, const double* U, const do
     double p1 = (gamma - 1.0) * (U[4] - (0.5 * ((U[1] * U[1] + U[2] * U[2]) + U[3] * U[3])) / U[0]);
     double un2 = ((U2[1] * n1 + U2[2] * n2) + U2[3] * n3) / U2[0];
     double p2 = (gamma - 1.0) * (U2[4] - (0.5 * ((U2[1] * U2[1] + U2[2] * U2[2]) + U2[3] * U2[3])) / U2[0]);
+    DF[0] = U2[0] * un2 - U[0] * un1;
+    DF[1] = (U2[1] * un2 + p2 * n1) - (U[1] * un1 + p1 * n1);
+    DF[2] = (U2[2] * un2 + p2 * n2) - (U[2] * un1 + p1 * n2);
+    DF[3] = (U2[3] * un2 + p2 * n3) - (U[3] * un1 + p1 * n3);
+    DF[4] = (U2[4] + p2) * un2 - (U[4] + p1) * un1;
+#pragma unroll
+    for (int m = 5; m < NV; m++) DF[m] = U2[m] * un2 - U[m] * un1;
+}
+
+// the same expression with the two divisions by each density sharing one reciprocal
+template <int NV>
+__device__ __forceinline__ void comput_dfn_nr(double* DF, const double* U, const double* DU, double n1, double n2, double n3, double gamma) {
+    double U2[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) U2[m] = U[m] + DU[m];
+    const double r1 = recip_nr(U[0]), r2 = recip_nr(U2[0]);
+    double un1 = div_nr((U[1] * n1 + U[2] * n2) + U[3] * n3, U[0], r1);
+    double p1 = (gamma - 1.0) * (U[4] - div_nr(0.5 * ((U[1] * U[1] + U[2] * U[2]) + U[3] * U[3]), U[0], r1));
+    double un2 = div_nr((U2[1] * n1 + U2[2] * n2) + U2[3] * n3, U2[0], r2);
+    double p2 = (gamma - 1.0) * (U2[4] - div_nr(0.5 * ((U2[1] * U2[1] + U2[2] * U2[2]) + U2[3] * U2[3]), U2[0], r2));
     DF[0] = U2[0] * un2 - U[0] * un1;
     DF[1] = (U2[1] * un2 + p2 * n1) - (U[1] * un1 + p1 * n1);
     DF[2] = (U2[2] * un2 + p2 * n2) - (U[2] * un1 + p1 * n2);

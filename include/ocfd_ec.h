@@ -202,6 +202,13 @@ int ocfd_batch_remove(ocfd_batch *batch, ocfd_ctx *ctx);
 int ocfd_batch_step(ocfd_batch *batch, int32_t niter);
 /* ocfd_step_timed for the batch's shared launches. */
 int ocfd_batch_step_timed(ocfd_batch *batch, int32_t niter, double *stage_ms);
+
+/* Self-check of the kernels' shared-reciprocal FP64 division (the IEEE division sequence without
+ * its operand pre-scaling and special-value fix-up, used where several divisions share a divisor).
+ * No counterpart in the reference.  Divides n hashed operand pairs (numerator exponents within
+ * +-numerator_exponent, divisor exponents within +-40, signs and exact zeros included) both ways
+ * on the current device and returns the number of quotients whose values differ. */
+int ocfd_check_division(int64_t n, uint64_t seed, int32_t numerator_exponent, int64_t *mismatches);
 int ocfd_batch_set_graph(ocfd_batch *batch, int32_t enable);
 /* ocfd_set_lu_tile and ocfd_autotune_lu for the batch's shared launches (every member advances
  * by 3 niter iterations). */

@@ -16,7 +16,8 @@ __global__ void __launch_bounds__(64) k_prims(BSet S) {
     size_t s = IX(b, i, j, k);
     double p00 = 1.0 / (p.gamma * p.Ma * p.Ma);
     double d = b.U[0][s];
-    double u = b.U[1][s] / d, v = b.U[2][s] / d, w = b.U[3][s] / d;
+    const double rd = recip_nr(d);
+    double u = div_nr(b.U[1][s], d, rd), v = div_nr(b.U[2][s], d, rd), w = div_nr(b.U[3][s], d, rd);
     double T = (b.U[4][s] - (0.5 * d) * ((u * u + v * v) + w * w)) / (p.Cv * d);
     b.prim[0][s] = d; b.prim[1][s] = u; b.prim[2][s] = v; b.prim[3][s] = w;
     b.prim[4][s] = (p00 * d) * T; b.prim[5][s] = T; b.prim[6][s] = sqrt(T) / p.Ma;
@@ -1274,6 +1275,45 @@ void launch_halo(const Blk& src, const Blk& dst, const Sub& B, int NV, hipStream
 
 void launch_force_sub(const Blk& b, const Par& p, const Sub& B, double* acc12, hipStream_t st) {
     hipLaunchKernelGGL(k_force_sub, dim3(1), dim3(64), 0, st, b, p, B, acc12);
+}
+
+// ---------------------------------------------------------------------------------------------
+// self-check of the shared-reciprocal division: n operand pairs from a counter-based hash, the
+// numerator's exponent within +-2^emax, the divisor's within +-2^40 (signs and exact zeros
+// included); counts the quotients whose bits differ from the compiler's division
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+    x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33;
+    return x;
+}
+__device__ __forceinline__ double hashed(unsigned long long h, int emax) {
+    const double m = 1.0 + (double)(h >> 11) * (1.0 / 9007199254740992.0);    // [1, 2)
+    const int e = (int)((h >> 3) % (unsigned long long)(2 * emax + 1)) - emax;
+    return (h & 1) ? -ldexp(m, e) : ldexp(m, e);
+}
+__global__ void k_check_division(long long n, unsigned long long seed, int emax, unsigned long long* bad) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const unsigned long long h1 = mix64(seed ^ (2 * (unsigned long long)i)), h2 = mix64(seed ^ (2 * (unsigned long long)i + 1));
+        const double a = (h1 % 97 == 0) ? 0.0 : hashed(h1, emax), b = hashed(h2, 40);
+        const double q1 = div_nr(a, b, recip_nr(b)), q2 = a / b;
+        // the two zeros compare equal: a zero quotient may differ in sign only
+        if (q1 != q2 && __double_as_longlong(q1) != __double_as_longlong(q2)) atomicAdd(bad, 1ull);
+    }
+}
+
+int check_division(long long n, unsigned long long seed, int emax, long long* mismatches) {
+    unsigned long long* d = nullptr;
+    if (hipMalloc(&d, sizeof(unsigned long long)) != hipSuccess) return -1;
+    unsigned long long h = 0;
+    hipError_t e = hipMemset(d, 0, sizeof(h));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_check_division, dim3(2048), dim3(256), 0, 0, n, seed, emax, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return -1;
+    *mismatches = (long long)h;
+    return 0;
 }
 
 }  // namespace ocfd

@@ -47,6 +47,7 @@ int halo_points(const HaloMap& h);
 void launch_halo(const Blk& src, const Blk& dst, const Sub& B, int NV, hipStream_t st);
 void launch_lusgs_sweep(const Blk& b, const Par& p, bool forward, hipStream_t st);
 void launch_force_sub(const Blk& b, const Par& p, const Sub& B, double* acc12, hipStream_t st);
+int check_division(long long n, unsigned long long seed, int emax, long long* mismatches);
 std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, int nfields, int tj, long long* total_elements);
 bool lusgs_tile_ok(int tj);
 int lusgs_fields(int NV);

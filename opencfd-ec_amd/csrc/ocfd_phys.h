@@ -42,9 +42,17 @@ __device__ __forceinline__ double div_nr(double a, double b, double r) {
 }
 
 // a / b in the compute type: the shared-reciprocal sequence above for double, the compiler's
-// division for float
+// division for float; rcp_of / dvr share one reciprocal between divisions by b
 template <typename T> __device__ __forceinline__ T dvd(T a, T b) {
     if constexpr (sizeof(T) == 8) return div_nr(a, b, recip_nr(b));
+    else return a / b;
+}
+template <typename T> __device__ __forceinline__ T rcp_of(T b) {
+    if constexpr (sizeof(T) == 8) return recip_nr(b);
+    else return T(0);
+}
+template <typename T> __device__ __forceinline__ T dvr(T a, T b, T r) {
+    if constexpr (sizeof(T) == 8) return div_nr(a, b, r);
     else return a / b;
 }
 
@@ -224,10 +232,11 @@ __device__ __forceinline__ void flux_roe(const T* QL, const T* QR, T* F, T gamma
     T Fl0 = UL1, Fl1 = UL1 * uul + pl, Fl2 = UL2 * uul, Fl3 = UL3 * uul, Fl4 = uul * (UL4 + pl);
     T Fr0 = UR1, Fr1 = UR1 * uur + pr, Fr2 = UR2 * uur, Fr3 = UR3 * uur, Fr4 = uur * (UR4 + pr);
     T tmpp = sqrt(dr / dl);
-    T avu = (uul + tmpp * uur) / (T(1) + tmpp);
-    T avv = (vvl + tmpp * vvr) / (T(1) + tmpp);
-    T avw = (wwl + tmpp * wwr) / (T(1) + tmpp);
-    T avh = (hl + tmpp * hr) / (T(1) + tmpp);
+    const T den = T(1) + tmpp, rden = rcp_of(den);
+    T avu = dvr(uul + tmpp * uur, den, rden);
+    T avv = dvr(vvl + tmpp * vvr, den, rden);
+    T avw = dvr(wwl + tmpp * wwr, den, rden);
+    T avh = dvr(hl + tmpp * hr, den, rden);
     T ava = sqrt((gamma - T(1)) * (avh - T(0.5) * ((avu * avu + avv * avv) + avw * avw)));
     T l0 = fabs(avu - ava), l1 = fabs(avu), l2 = fabs(avu + ava);
     if (l0 < delt) l0 = (sq(l0) + delt * delt) / (T(2) * delt);

@@ -1220,6 +1220,14 @@ int ocfd_batch_set_graph(ocfd_batch* b, int32_t enable) {
     return OCFD_OK;
 }
 
+int ocfd_check_division(int64_t n, uint64_t seed, int32_t numerator_exponent, int64_t* mismatches) {
+    if (n < 0 || !mismatches || numerator_exponent < 0 || numerator_exponent > 1000) return fail(OCFD_EARG, "bad argument");
+    long long m = 0;
+    if (check_division((long long)n, (unsigned long long)seed, numerator_exponent, &m)) return fail(OCFD_EHIP, "division check failed to run");
+    *mismatches = (int64_t)m;
+    return OCFD_OK;
+}
+
 int ocfd_mem_bytes(ocfd_ctx* c, int64_t* bytes) {
     if (!c || !bytes) return fail(OCFD_EARG, "null argument");
     *bytes = (int64_t)c->bytes;

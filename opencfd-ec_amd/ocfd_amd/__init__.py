@@ -59,6 +59,7 @@ EXPORTS = [
     "ocfd_step_timed", "ocfd_set_lu_schedule", "ocfd_set_lu_tile", "ocfd_get_lu_tile", "ocfd_autotune_lu",
     "ocfd_batch_create", "ocfd_batch_destroy", "ocfd_batch_add", "ocfd_batch_remove", "ocfd_batch_step",
     "ocfd_batch_set_graph", "ocfd_batch_set_lu_tile", "ocfd_batch_autotune_lu", "ocfd_batch_step_timed", "ocfd_mem_bytes",
+    "ocfd_check_division",
 ]
 LU_TILES = (4, 8, 16)
 LU_PIPELINED, LU_HYPERPLANE, LU_PIPELINED_FENCED = 0, 1, 2
@@ -101,6 +102,7 @@ def lib():
         L.ocfd_batch_set_lu_tile.argtypes = [vp, i32]
         L.ocfd_batch_autotune_lu.argtypes = [vp, i32, ip, dp]
         L.ocfd_batch_step_timed.argtypes = [vp, i32, dp]
+        L.ocfd_check_division.argtypes = [C.c_int64, C.c_uint64, i32, C.POINTER(C.c_int64)]
         L.ocfd_batch_create.argtypes = [C.POINTER(vp)]
         L.ocfd_batch_destroy.argtypes = [vp]
         L.ocfd_batch_add.argtypes = [vp, vp]
@@ -432,3 +434,10 @@ def solver_from_case(case, precision: int = 0, states: Optional[Sequence[np.ndar
         s.bc()
         s.halo()
     return s
+
+
+def check_division(n: int, seed: int = 1, numerator_exponent: int = 300) -> int:
+    """Quotients of the shared-reciprocal division that differ from IEEE division (ocfd_check_division)."""
+    m = C.c_int64()
+    _check(lib().ocfd_check_division(n, seed, numerator_exponent, C.byref(m)))
+    return int(m.value)

@@ -298,3 +298,12 @@ def test_batch_autotune_lu():
             assert np.array_equal(s.download_U(no), ref.download_U(no))
         assert s.counters()[0] == 3
     bt.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("emax", [40, 300])
+def test_shared_reciprocal_division_is_ieee_division(emax):
+    """The LU-SGS sweep and WENO5 divide by a refined reciprocal shared between divisions (the
+    IEEE sequence without its pre-scaling and fix-up steps): on 2^25 hashed operand pairs, with
+    divisors within 2^+-40 and numerators within 2^+-emax, every quotient equals a / b."""
+    assert A.check_division(1 << 25, seed=11, numerator_exponent=emax) == 0

@@ -178,6 +178,13 @@ int ocfd_set_lu_schedule(ocfd_ctx *ctx, int32_t schedule);
  * 16.  Results do not depend on it (bit for bit); the sweep time does.  No counterpart in the
  * reference. */
 int ocfd_set_lu_tile(ocfd_ctx *ctx, int32_t tj);
+
+/* Failure-path switch of the pipelined LU-SGS sweeps of this context (not of a batch).  A sweep
+ * wait normally gives up after 2^26 polls of an upstream tile (seconds); the step then returns
+ * OCFD_EHIP, the iteration's state is invalid, and the sweep tables are restored for the next
+ * one.  With on = 1 every wait that finds its producer behind gives up at its first poll, which
+ * forces that path; on = 0 restores normal waiting.  No counterpart in the reference. */
+int ocfd_set_lu_fail_waits(ocfd_ctx *ctx, int32_t on);
 int ocfd_get_lu_tile(ocfd_ctx *ctx, int32_t *tj);
 
 /* Tile-shape autotuner of OCFD_LU_PIPELINED.  No counterpart in the reference.  Runs niter

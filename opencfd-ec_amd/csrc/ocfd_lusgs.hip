@@ -27,6 +27,14 @@ namespace ocfd {
 constexpr int LU_TJ = 8, LU_TK = 8, LU_NT = LU_TJ * LU_TK;
 constexpr long LU_POLL_LIMIT = 1L << 26;
 
+// err[0]: the error flag a wait sets when it gives up.  err[2]: nonzero makes every wait that
+// finds its producer behind give up at its first poll (the host's failure-path test).  The poll
+// limit itself stays a compile-time constant: a limit read at run time changes the register
+// allocation of the team sweep and costs it a sixth of its speed.
+__device__ __forceinline__ bool lu_fail_waits(const int* err) {
+    return __hip_atomic_load(err + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
 // Per-lane record shared through LDS with the downstream lanes of the tile.
 template <int NV>
 struct LuSlot {
@@ -75,7 +83,8 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_sweep(const Blk* __restrict__ b
                 if (known_k < need_k) known_k = __hip_atomic_load(prog + T.dep_k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 fresh = true;
                 if (known_j >= need_j && known_k >= need_k) break;
-                if (++polls > LU_POLL_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                if (++polls > LU_POLL_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                    lu_fail_waits(err)) {
                     __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     known_j = known_k = -2;
                     break;
@@ -406,7 +415,8 @@ __device__ __forceinline__ bool team_wait_remote(const double* __restrict__ er, 
         for (int m = 0; m < NV; m++) ready = ready && !is_sentinel(DU[m]);
         if (ready) return false;
         if (++polls > LU_POLL_LIMIT ||
-            ((polls & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+            ((polls & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) ||
+            (polls == 1 && lu_fail_waits(err))) {
             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return true;
         }
@@ -522,8 +532,11 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
     };
 
     // one step: complete the read cell's DU, then this direction's term of the own cell, a
-    // barrier; true when the workgroup gives up
-    auto step = [&](int q, const TeamIn<NV, FWD, D>& in, double* rdu) -> bool {
+    // barrier; with check, true when the workgroup gives up.  Every third step checks (an LDS
+    // read after the barrier that the other steps skip): a wave that gave up has set the error
+    // flag, so the waits of the steps in between give up within 256 polls and their values are
+    // discarded
+    auto step = [&](int q, const TeamIn<NV, FWD, D>& in, double* rdu, bool check) -> bool {
         double DUs[NV];
         complete(q, in, DUs);
         store(q, DUs);
@@ -547,7 +560,7 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
             for (int m = 0; m < NV; m++) sh.dus[q & 1][D][m][lane] = has ? du[m] : 0.0;
         }
         team_barrier();
-        return sh.bail != 0;
+        return check && sh.bail != 0;
     };
 
     TeamIn<NV, FWD, D> iA, iB, iC;
@@ -558,15 +571,15 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
     if constexpr (D > 0) team_load_du<NV, FWD>(rem_p(0), rA);
     for (int q = 0; q < nsteps; q += 3) {
         if constexpr (D > 0) team_load_du<NV, FWD>(rem_p(q + 1), rB);
-        if (step(q, iA, rA)) return;
+        if (step(q, iA, rA, false)) return;
         load(q + 3, iA);
         if (q + 1 >= nsteps) break;
         if constexpr (D > 0) team_load_du<NV, FWD>(rem_p(q + 2), rC);
-        if (step(q + 1, iB, rB)) return;
+        if (step(q + 1, iB, rB, false)) return;
         load(q + 4, iB);
         if (q + 2 >= nsteps) break;
         if constexpr (D > 0) team_load_du<NV, FWD>(rem_p(q + 3), rA);
-        if (step(q + 2, iC, rC)) return;
+        if (step(q + 2, iC, rC, true)) return;
         load(q + 5, iC);
     }
     // the DU of the last step

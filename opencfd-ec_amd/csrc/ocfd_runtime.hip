@@ -55,6 +55,7 @@ struct Engine {
     std::vector<hipEvent_t> events;           // stage boundaries of ocfd_step_timed
     int lu_schedule = OCFD_LU_PIPELINED;     // LU-SGS sweep schedule
     int lu_tj = 8;                            // tile width of the pipelined sweep
+    int lu_fail = 0;                          // 1: every sweep wait that has to poll gives up
     LuPlan lu{};                              // pipelined-sweep tables (built on first use)
     int lu_nv = 0;
     std::vector<void*> lu_allocs;
@@ -382,7 +383,7 @@ int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) {
     long long P = 0;
     const int tj = e.lu_schedule == OCFD_LU_PIPELINED_FENCED ? 8 : e.lu_tj;
     std::vector<LuTask> f = lusgs_tasks(blks, true, lusgs_fields(NV), tj, &P), b = lusgs_tasks(blks, false, lusgs_fields(NV), tj, nullptr);
-    std::vector<int> minus(std::max(f.size(), b.size()), -1), zero(4, 0);
+    std::vector<int> minus(std::max(f.size(), b.size()), -1), zero{0, 0, e.lu_fail, 0};
     std::vector<int> mf(minus.begin(), minus.begin() + f.size()), mb(minus.begin(), minus.begin() + b.size());
     Blk* pb; LuTask *pf, *pw; int *qf, *qb, *pe;
     int r;
@@ -1090,6 +1091,17 @@ int ocfd_set_graph(ocfd_ctx* c, int32_t enable) {
 int ocfd_set_lu_schedule(ocfd_ctx* c, int32_t schedule) {
     if (!c) return fail(OCFD_EARG, "null context");
     return engine_set_lu_schedule(c->e, schedule);
+}
+
+int ocfd_set_lu_fail_waits(ocfd_ctx* c, int32_t on) {
+    if (!c) return fail(OCFD_EARG, "null context");
+    Engine& e = c->e;
+    e.lu_fail = on ? 1 : 0;
+    if (e.lu.err) {     // the sweeps read the word at run time: captured graphs stay valid
+        HIPCHK(hipStreamSynchronize(e.st));
+        HIPCHK(hipMemcpy(e.lu.err + 2, &e.lu_fail, sizeof(int), hipMemcpyHostToDevice));
+    }
+    return OCFD_OK;
 }
 
 int ocfd_set_lu_tile(ocfd_ctx* c, int32_t tj) {

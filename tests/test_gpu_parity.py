@@ -307,3 +307,27 @@ def test_shared_reciprocal_division_is_ieee_division(emax):
     IEEE sequence without its pre-scaling and fix-up steps): on 2^25 hashed operand pairs, with
     divisors within 2^+-40 and numerators within 2^+-emax, every quotient equals a / b."""
     assert A.check_division(1 << 25, seed=11, numerator_exponent=emax) == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("schedule", ["pipelined", "fenced"])
+def test_lu_wait_failure_error_and_recovery(schedule):
+    """A sweep that gives up waiting for an upstream tile (forced: every wait that has to poll gives
+    up at its first poll, as at the poll limit) ends every workgroup, returns OCFD_EHIP and
+    restores the sweep tables; with normal waiting back, the same solver then steps exactly like a
+    fresh one."""
+    case = C.naca_cgrid(65, 25, 9, Iflag_Flux=4, wall_spacing=0.02)
+    U0 = C.perturbed_state(case, seed=21, amp=1e-4)
+    s = A.solver_from_case(case, states=U0)
+    s.set_lu_schedule(A.LU_PIPELINED if schedule == "pipelined" else A.LU_PIPELINED_FENCED)
+    s.set_lu_fail_waits(True)
+    with pytest.raises(A.OcfdError) as e:
+        s.step(1)
+    assert e.value.code == A.OCFD_EHIP
+    s.set_lu_fail_waits(False)
+    s.set_interior(U0); s.bc(); s.halo()
+    s.step(2)
+    ref = A.solver_from_case(case, states=U0)
+    ref.set_lu_schedule(A.LU_HYPERPLANE)
+    ref.step(2)
+    assert np.array_equal(s.download_U(1), ref.download_U(1))

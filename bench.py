@@ -156,7 +156,7 @@ def serve_main(args) -> int:
     advance together through shared launches until their Res_rms(1) has dropped by --drop; the
     baseline runs the plain solver on one request at a time to the same criterion."""
     from ocfd_amd import serve as SV
-    dims = SV.DIMS if args.serve_dims == "small" else SV.DIMS_SURVEY
+    dims = SV.DIMS if args.serve_dims == "small" else SV.DIMS_LARGE
     reqs = SV.sweep_requests(args.serve, lam=args.rate, seed=5, dims=dims)
     srv = SV.Server(mem_budget=args.serve_budget * 1e9, round_iters=args.round_iters, drop=args.drop,
                     max_iters=args.max_iters, graph=bool(args.graph))
@@ -271,8 +271,8 @@ def main() -> int:
     ap.add_argument("--max-iters", type=int, default=6000, help="serving: iteration cap per request")
     ap.add_argument("--serve-budget", type=float, default=200.0, help="serving: HBM budget of admitted requests, GB")
     ap.add_argument("--seq-sample", type=int, default=32, help="serving: requests the sequential baseline runs")
-    ap.add_argument("--serve-dims", choices=("small", "survey"), default="small",
-                    help="serving: request grids 97x33x9/129x49x13/193x65x17 (small) or 129x49x17/193x65x25/257x97x33")
+    ap.add_argument("--serve-dims", choices=("small", "large"), default="small",
+                    help="serving: request grids 97x33x9/129x49x13/193x65x17 (small) or 129x49x17/193x65x25/257x97x33 (large)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--precision-check", type=int, default=0, metavar="N",
                     help="only run N iterations of the workload in FP64 and in mixed precision from freestream "
@@ -283,8 +283,8 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: the hot path runs on a single MI355X in this build; "
-              "multi-GPU sharding by block is a design note, not an implemented path.", file=sys.stderr)
+        print("bench.py measures one GPU: the hot path runs on a single MI355X; "
+              "sharding by block over several GPUs is designed, not implemented.", file=sys.stderr)
         return 2
 
     if args.copy_probe:

@@ -54,7 +54,7 @@ class Result:
 
 
 DIMS = ((97, 33, 9), (129, 49, 13), (193, 65, 17))
-DIMS_SURVEY = ((129, 49, 17), (193, 65, 25), (257, 97, 33))
+DIMS_LARGE = ((129, 49, 17), (193, 65, 25), (257, 97, 33))
 
 
 def sweep_requests(n: int = 256, lam: float = 2.0, seed: int = 5, dims=DIMS, max_blocks: int = 8,

@@ -169,6 +169,33 @@ module ocfd_ec
       integer(c_int32_t), intent(out) :: best_tj
       real(c_double), intent(out) :: ms(3)
     end function
+    integer(c_int) function ocfd_get_lu_tile(ctx, tj) bind(C, name="ocfd_get_lu_tile")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: tj
+    end function
+    integer(c_int) function ocfd_set_lu_schedule(ctx, schedule) bind(C, name="ocfd_set_lu_schedule")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: schedule
+    end function
+    integer(c_int) function ocfd_set_lu_fail_waits(ctx, on) bind(C, name="ocfd_set_lu_fail_waits")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
+    integer(c_int) function ocfd_mem_bytes(ctx, bytes) bind(C, name="ocfd_mem_bytes")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: bytes
+    end function
+    integer(c_int) function ocfd_check_division(n, seed, numerator_exponent, mismatches) &
+        bind(C, name="ocfd_check_division")
+      import :: c_int, c_int32_t, c_int64_t
+      integer(c_int64_t), value :: n, seed
+      integer(c_int32_t), value :: numerator_exponent
+      integer(c_int64_t), intent(out) :: mismatches
+    end function
   end interface
 
 contains

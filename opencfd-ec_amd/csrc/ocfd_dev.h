@@ -34,6 +34,7 @@ struct Blk {
     // the memory of their FP64 counterparts
     float *primf[6], *Ff[3][5], *Fvf[3][4], *Fsaf[3][2];
     double *aux;                         // turbulence scratch (SST f1)
+    double *kt, *wt;                     // SST: k and omega per unit mass, max(U, 0) / rho, on the box
     uint8_t *bcf[3];                     // face-row flags: bit0 lower face physical, bit1 upper face physical
     double *surf[6][3];                  // Surf1..6 (wall viscous force of the i-/j-/k-/i+/j+/k+ faces)
     int *flags;                          // [0] IF_OverLimit latch, [1] limited cells this iteration, [2] SA cells above LSAmax

@@ -21,6 +21,12 @@ __global__ void __launch_bounds__(64) k_prims(BSet S) {
     double T = (b.U[4][s] - (0.5 * d) * ((u * u + v * v) + w * w)) / (p.Cv * d);
     b.prim[0][s] = d; b.prim[1][s] = u; b.prim[2][s] = v; b.prim[3][s] = w;
     b.prim[4][s] = (p00 * d) * T; b.prim[5][s] = T; b.prim[6][s] = sqrt(T) / p.Ma;
+    if (p.turb == 3) {
+        // SST: k and omega per unit mass, read by the turbulence stencils (sub_turbulence_SST.f90)
+        const double kv = b.U[5][s], wv = b.U[6][s];
+        b.kt[s] = (kv < 0.0 ? 0.0 : kv) / d;
+        b.wt[s] = (wv < 0.0 ? 0.0 : wv) / d;
+    }
     if (p.mixed) {
         b.primf[0][s] = (float)d; b.primf[1][s] = (float)u; b.primf[2][s] = (float)v; b.primf[3][s] = (float)w;
         b.primf[4][s] = (float)((p00 * d) * T); b.primf[5][s] = (float)T;
@@ -209,8 +215,9 @@ __device__ __forceinline__ void d_amut(const Blk& b, const Sub& B, int a, int c)
 // ---------------------------------------------------------------------------------------------
 // Menter SST (sub_turbulence_SST.f90:7-348).  f1 on the last face of each direction reads the
 // never-written cell (nx,.,.)/(.,ny,.)/(.,.,nz); here that storage holds zero.
-__device__ __forceinline__ double kt_of(const Blk& b, size_t s) { double v = b.U[5][s]; v = v < 0.0 ? 0.0 : v; return v / b.U[0][s]; }
-__device__ __forceinline__ double wt_of(const Blk& b, size_t s) { double v = b.U[6][s]; v = v < 0.0 ? 0.0 : v; return v / b.U[0][s]; }
+// k and omega per unit mass, max(U, 0) / rho, as k_prims stores them for the iteration
+__device__ __forceinline__ double kt_of(const Blk& b, size_t s) { return b.kt[s]; }
+__device__ __forceinline__ double wt_of(const Blk& b, size_t s) { return b.wt[s]; }
 
 // cell pass: mu_t, f1, sources into Res(6:7)
 __global__ void __launch_bounds__(64) k_sst_cell(BSet S) {

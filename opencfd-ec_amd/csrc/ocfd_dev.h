@@ -29,7 +29,7 @@ struct Blk {
     double *vol, *S[3], *nv[9], *met[4][9], *dw, *dtm, *x, *y, *z, *xc, *yc, *zc;
     double *F[3][5];                     // inviscid face flux, m = 1..5
     double *Fv[3][4];                    // viscous face flux, m = 2..5
-    double *Fsa[3][2];                   // SA face terms: convection+diffusion flux, grad(vt).n S
+    double *Fsa[3][2];                   // SA face terms: convection+diffusion flux, grad(vt).n S; SST: k and omega fluxes
     // mixed precision: FP32 primitives (rho u v w p T) and FP32 face terms; the face arrays share
     // the memory of their FP64 counterparts
     float *primf[6], *Ff[3][5], *Fvf[3][4], *Fsaf[3][2];

@@ -278,64 +278,71 @@ __global__ void k_sst_ghost(BSet S, int f) {
     else { if (a > nx + 1 || c > ny + 1) return; b.mut[IX(b, a, c, f == 4 ? 0 : nz)] = b.mut[IX(b, a, c, f == 4 ? 1 : nz - 1)]; }
 }
 
-// k and omega face fluxes, accumulated into Res(6:7) per direction in i, j, k order
-__global__ void __launch_bounds__(64) k_sst_res(BSet S) {
-    OCFD_LOCATE(SP_CELL)
+// k and omega flux through face (i,j,k) of direction DIR, between cells (i,j,k) - e_DIR and
+// (i,j,k): convection upwinded on the normal velocity, diffusion with the face gradient of k and
+// omega; one thread per face, each face evaluated once for both of its cells
+template <int DIR>
+__global__ void __launch_bounds__(64) k_sst_face(BSet S) {
+    OCFD_LOCATE(SP_FI + DIR)
     const double sigma_k1 = 0.85, sigma_w1 = 0.5, sigma_k2 = 1.0, sigma_w2 = 0.856;
     const double Re = p.Re;
     const double *uu = b.prim[1], *vv = b.prim[2], *ww = b.prim[3];
+    const int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
+    const int fi = i, fj = j, fk_ = k;
+    size_t f = IX(b, fi, fj, fk_), fm = IX(b, fi - di, fj - dj, fk_ - dk);
+    double f1 = b.aux[f];
+    double skk = f1 * sigma_k1 + (1.0 - f1) * sigma_k2, sww = f1 * sigma_w1 + (1.0 - f1) * sigma_w2;
+    double mua = b.mu[fm] * Re, mub = b.mu[f] * Re, mta = b.mut[fm], mtb = b.mut[f];
+    double muk = (((mua + mub) + skk * (mta + mtb)) * 0.5) / Re;
+    double muw = (((mua + mub) + sww * (mta + mtb)) * 0.5) / Re;
+    auto K = [&](int a, int bb, int cc) { return kt_of(b, IX(b, a, bb, cc)); };
+    auto W = [&](int a, int bb, int cc) { return wt_of(b, IX(b, a, bb, cc)); };
+    double kti, ktj, ktk, wti, wtj, wtk;
+    if constexpr (DIR == 0) {
+        kti = K(fi, fj, fk_) - K(fi - 1, fj, fk_); wti = W(fi, fj, fk_) - W(fi - 1, fj, fk_);
+        ktj = 0.25 * (((K(fi, fj + 1, fk_) - K(fi, fj - 1, fk_)) + K(fi - 1, fj + 1, fk_)) - K(fi - 1, fj - 1, fk_));
+        wtj = 0.25 * (((W(fi, fj + 1, fk_) - W(fi, fj - 1, fk_)) + W(fi - 1, fj + 1, fk_)) - W(fi - 1, fj - 1, fk_));
+        ktk = 0.25 * (((K(fi, fj, fk_ + 1) - K(fi, fj, fk_ - 1)) + K(fi - 1, fj, fk_ + 1)) - K(fi - 1, fj, fk_ - 1));
+        wtk = 0.25 * (((W(fi, fj, fk_ + 1) - W(fi, fj, fk_ - 1)) + W(fi - 1, fj, fk_ + 1)) - W(fi - 1, fj, fk_ - 1));
+    } else if constexpr (DIR == 1) {
+        kti = 0.25 * (((K(fi + 1, fj, fk_) - K(fi - 1, fj, fk_)) + K(fi + 1, fj - 1, fk_)) - K(fi - 1, fj - 1, fk_));
+        ktj = K(fi, fj, fk_) - K(fi, fj - 1, fk_);
+        ktk = 0.25 * (((K(fi, fj, fk_ + 1) - K(fi, fj, fk_ - 1)) + K(fi, fj - 1, fk_ + 1)) - K(fi, fj - 1, fk_ - 1));
+        wti = 0.25 * (((W(fi + 1, fj, fk_) - W(fi - 1, fj, fk_)) + W(fi + 1, fj - 1, fk_)) - W(fi - 1, fj - 1, fk_));
+        wtj = W(fi, fj, fk_) - W(fi, fj - 1, fk_);
+        wtk = 0.25 * (((W(fi, fj, fk_ + 1) - W(fi, fj, fk_ - 1)) + W(fi, fj - 1, fk_ + 1)) - W(fi, fj - 1, fk_ - 1));
+    } else {
+        kti = 0.25 * (((K(fi + 1, fj, fk_) - K(fi - 1, fj, fk_)) + K(fi + 1, fj, fk_ - 1)) - K(fi - 1, fj, fk_ - 1));
+        ktj = 0.25 * (((K(fi, fj + 1, fk_) - K(fi, fj - 1, fk_)) + K(fi, fj + 1, fk_ - 1)) - K(fi, fj - 1, fk_ - 1));
+        ktk = K(fi, fj, fk_) - K(fi, fj, fk_ - 1);
+        wti = 0.25 * (((W(fi + 1, fj, fk_) - W(fi - 1, fj, fk_)) + W(fi + 1, fj, fk_ - 1)) - W(fi - 1, fj, fk_ - 1));
+        wtj = 0.25 * (((W(fi, fj + 1, fk_) - W(fi, fj - 1, fk_)) + W(fi, fj + 1, fk_ - 1)) - W(fi, fj - 1, fk_ - 1));
+        wtk = W(fi, fj, fk_) - W(fi, fj, fk_ - 1);
+    }
+    double* const* mt = b.met[DIR + 1];
+    double ktx = (kti * mt[0][f] + ktj * mt[3][f]) + ktk * mt[6][f], wtx = (wti * mt[0][f] + wtj * mt[3][f]) + wtk * mt[6][f];
+    double kty = (kti * mt[1][f] + ktj * mt[4][f]) + ktk * mt[7][f], wty = (wti * mt[1][f] + wtj * mt[4][f]) + wtk * mt[7][f];
+    double ktz = (kti * mt[2][f] + ktj * mt[5][f]) + ktk * mt[8][f], wtz = (wti * mt[2][f] + wtj * mt[5][f]) + wtk * mt[8][f];
+    double s1x = b.nv[3 * DIR][f], s1y = b.nv[3 * DIR + 1][f], s1z = b.nv[3 * DIR + 2][f];
+    double vn1 = (uu[fm] * s1x + vv[fm] * s1y) + ww[fm] * s1z;
+    double vn2 = (uu[f] * s1x + vv[f] * s1y) + ww[f] * s1z;
+    double kfi = 0.5 * ((vn1 + fabs(vn1)) * kt_of(b, fm) + (vn2 - fabs(vn2)) * kt_of(b, f));
+    double wfi = 0.5 * ((vn1 + fabs(vn1)) * wt_of(b, fm) + (vn2 - fabs(vn2)) * wt_of(b, f));
+    double Sf = b.S[DIR][f];
+    b.Fsa[DIR][0][f] = (-kfi + muk * ((ktx * s1x + kty * s1y) + ktz * s1z)) * Sf;
+    b.Fsa[DIR][1][f] = (-wfi + muw * ((wtx * s1x + wty * s1y) + wtz * s1z)) * Sf;
+}
+
+// the face fluxes accumulated into Res(6:7) per direction in i, j, k order
+__global__ void __launch_bounds__(64) k_sst_res(BSet S) {
+    OCFD_LOCATE(SP_CELL)
     size_t s = IX(b, i, j, k);
+    const size_t sp[3] = {s + 1, s + (size_t)b.NI, s + (size_t)b.NI * b.NJ};
     double r6 = b.Res[5][s], r7 = b.Res[6][s];
+#pragma unroll
     for (int dir = 0; dir < 3; dir++) {
-        int di = dir == 0, dj = dir == 1, dk = dir == 2;
-        double fk[2], fw[2];
-        for (int side = 0; side < 2; side++) {
-            int fi = i + side * di, fj = j + side * dj, fk_ = k + side * dk;
-            size_t f = IX(b, fi, fj, fk_), fm = IX(b, fi - di, fj - dj, fk_ - dk);
-            double f1 = b.aux[f];
-            double skk = f1 * sigma_k1 + (1.0 - f1) * sigma_k2, sww = f1 * sigma_w1 + (1.0 - f1) * sigma_w2;
-            double mua = b.mu[fm] * Re, mub = b.mu[f] * Re, mta = b.mut[fm], mtb = b.mut[f];
-            double muk = (((mua + mub) + skk * (mta + mtb)) * 0.5) / Re;
-            double muw = (((mua + mub) + sww * (mta + mtb)) * 0.5) / Re;
-            auto K = [&](int a, int bb, int cc) { return kt_of(b, IX(b, a, bb, cc)); };
-            auto W = [&](int a, int bb, int cc) { return wt_of(b, IX(b, a, bb, cc)); };
-            double kti, ktj, ktk, wti, wtj, wtk;
-            if (dir == 0) {
-                kti = K(fi, fj, fk_) - K(fi - 1, fj, fk_); wti = W(fi, fj, fk_) - W(fi - 1, fj, fk_);
-                ktj = 0.25 * (((K(fi, fj + 1, fk_) - K(fi, fj - 1, fk_)) + K(fi - 1, fj + 1, fk_)) - K(fi - 1, fj - 1, fk_));
-                wtj = 0.25 * (((W(fi, fj + 1, fk_) - W(fi, fj - 1, fk_)) + W(fi - 1, fj + 1, fk_)) - W(fi - 1, fj - 1, fk_));
-                ktk = 0.25 * (((K(fi, fj, fk_ + 1) - K(fi, fj, fk_ - 1)) + K(fi - 1, fj, fk_ + 1)) - K(fi - 1, fj, fk_ - 1));
-                wtk = 0.25 * (((W(fi, fj, fk_ + 1) - W(fi, fj, fk_ - 1)) + W(fi - 1, fj, fk_ + 1)) - W(fi - 1, fj, fk_ - 1));
-            } else if (dir == 1) {
-                kti = 0.25 * (((K(fi + 1, fj, fk_) - K(fi - 1, fj, fk_)) + K(fi + 1, fj - 1, fk_)) - K(fi - 1, fj - 1, fk_));
-                ktj = K(fi, fj, fk_) - K(fi, fj - 1, fk_);
-                ktk = 0.25 * (((K(fi, fj, fk_ + 1) - K(fi, fj, fk_ - 1)) + K(fi, fj - 1, fk_ + 1)) - K(fi, fj - 1, fk_ - 1));
-                wti = 0.25 * (((W(fi + 1, fj, fk_) - W(fi - 1, fj, fk_)) + W(fi + 1, fj - 1, fk_)) - W(fi - 1, fj - 1, fk_));
-                wtj = W(fi, fj, fk_) - W(fi, fj - 1, fk_);
-                wtk = 0.25 * (((W(fi, fj, fk_ + 1) - W(fi, fj, fk_ - 1)) + W(fi, fj - 1, fk_ + 1)) - W(fi, fj - 1, fk_ - 1));
-            } else {
-                kti = 0.25 * (((K(fi + 1, fj, fk_) - K(fi - 1, fj, fk_)) + K(fi + 1, fj, fk_ - 1)) - K(fi - 1, fj, fk_ - 1));
-                ktj = 0.25 * (((K(fi, fj + 1, fk_) - K(fi, fj - 1, fk_)) + K(fi, fj + 1, fk_ - 1)) - K(fi, fj - 1, fk_ - 1));
-                ktk = K(fi, fj, fk_) - K(fi, fj, fk_ - 1);
-                wti = 0.25 * (((W(fi + 1, fj, fk_) - W(fi - 1, fj, fk_)) + W(fi + 1, fj, fk_ - 1)) - W(fi - 1, fj, fk_ - 1));
-                wtj = 0.25 * (((W(fi, fj + 1, fk_) - W(fi, fj - 1, fk_)) + W(fi, fj + 1, fk_ - 1)) - W(fi, fj - 1, fk_ - 1));
-                wtk = W(fi, fj, fk_) - W(fi, fj, fk_ - 1);
-            }
-            double* const* mt = b.met[dir + 1];
-            double ktx = (kti * mt[0][f] + ktj * mt[3][f]) + ktk * mt[6][f], wtx = (wti * mt[0][f] + wtj * mt[3][f]) + wtk * mt[6][f];
-            double kty = (kti * mt[1][f] + ktj * mt[4][f]) + ktk * mt[7][f], wty = (wti * mt[1][f] + wtj * mt[4][f]) + wtk * mt[7][f];
-            double ktz = (kti * mt[2][f] + ktj * mt[5][f]) + ktk * mt[8][f], wtz = (wti * mt[2][f] + wtj * mt[5][f]) + wtk * mt[8][f];
-            double s1x = b.nv[3 * dir][f], s1y = b.nv[3 * dir + 1][f], s1z = b.nv[3 * dir + 2][f];
-            double vn1 = (uu[fm] * s1x + vv[fm] * s1y) + ww[fm] * s1z;
-            double vn2 = (uu[f] * s1x + vv[f] * s1y) + ww[f] * s1z;
-            double kfi = 0.5 * ((vn1 + fabs(vn1)) * kt_of(b, fm) + (vn2 - fabs(vn2)) * kt_of(b, f));
-            double wfi = 0.5 * ((vn1 + fabs(vn1)) * wt_of(b, fm) + (vn2 - fabs(vn2)) * wt_of(b, f));
-            double Sf = b.S[dir][f];
-            fk[side] = (-kfi + muk * ((ktx * s1x + kty * s1y) + ktz * s1z)) * Sf;
-            fw[side] = (-wfi + muw * ((wtx * s1x + wty * s1y) + wtz * s1z)) * Sf;
-        }
-        r6 = (r6 + fk[1]) - fk[0];
-        r7 = (r7 + fw[1]) - fw[0];
+        r6 = (r6 + b.Fsa[dir][0][sp[dir]]) - b.Fsa[dir][0][s];
+        r7 = (r7 + b.Fsa[dir][1][sp[dir]]) - b.Fsa[dir][1][s];
     }
     b.Res[5][s] = r6;
     b.Res[6][s] = r7;
@@ -1162,6 +1169,9 @@ void set_pre(const SetPlan& P, hipStream_t st) {
         for (int f = 0; f < 6; f++)
             hipLaunchKernelGGL(k_sst_ghost, dim3((unsigned)((m3 + 63) / 64), (unsigned)m3, (unsigned)P.nb), dim3(64), 0, st, bset(P, SP_CELL), f);
         launch_subs(k_amut, P, P.amut, st);
+        launch_space<SP_FI>(k_sst_face<0>, P, st);
+        launch_space<SP_FJ>(k_sst_face<1>, P, st);
+        launch_space<SP_FK>(k_sst_face<2>, P, st);
         launch_space<SP_CELL>(k_sst_res, P, st);
         launch_space<SP_N0>(k_sst_finish, P, st);
         launch_subs(k_amut, P, P.amut, st);

@@ -146,7 +146,8 @@ int carve_block(ocfd_ctx* c, HostBlock& hb) {
     if (c->p.turb == 3 && ((r = dalloc(hb, &b.kt, n)) || (r = dalloc(hb, &b.wt, n)))) return r;
     for (int dd = 0; dd < 3; dd++) for (int m = 0; m < 5; m++) if ((r = dalloc(hb, &b.F[dd][m], n))) return r;
     if (visc) for (int dd = 0; dd < 3; dd++) for (int m = 0; m < 4; m++) if ((r = dalloc(hb, &b.Fv[dd][m], n))) return r;
-    if (visc && c->p.turb == 2) for (int dd = 0; dd < 3; dd++) for (int m = 0; m < 2; m++) if ((r = dalloc(hb, &b.Fsa[dd][m], n))) return r;
+    if (visc && (c->p.turb == 2 || c->p.turb == 3))
+        for (int dd = 0; dd < 3; dd++) for (int m = 0; m < 2; m++) if ((r = dalloc(hb, &b.Fsa[dd][m], n))) return r;
     if (c->p.mixed) {
         // FP32 primitives in half-size slots; FP32 face terms in the first half of the FP64 arrays
         for (int q = 0; q < 6; q++) {

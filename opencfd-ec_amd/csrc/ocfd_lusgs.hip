@@ -279,15 +279,18 @@ constexpr int PK_T = 256;
 // box -> sweep order for fields [f0,f1) and [f2,f3), or (TO_PACKED false) field range [f0,f1) of
 // the sweep order (the backward DU) back into the box arrays of DU, writing the sentinel back
 // into it and into the forward DU; one tile, one i-window and one field per workgroup, through
-// LDS so both sides move in contiguous runs
+// LDS so both sides move in contiguous runs.  Neighbouring i-windows of a tile share the step
+// rows at their boundary (each writes some lanes of them), so a tile's windows run back to back
+// on one XCD, where the L2 merges those partial lines before they go to memory.
 template <int NV, bool TO_PACKED, int TJ>
 __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
-                                                       double* __restrict__ pk, long long P, Par p, int f0, int f1, int f2, int f3) {
+                                                       double* __restrict__ pk, long long P, Par p, int nwin, int f0, int f1, int f2, int f3) {
     __shared__ double buf[LU_NT][PK_W + 1];
-    const LuTask T = tiles[blockIdx.x];
+    const int tw = xcd_tile((int)blockIdx.x, (int)gridDim.x);     // (tile, i-window), windows fastest
+    const LuTask T = tiles[tw / nwin];
     const Blk& b = blks[T.blk];
     const int nxm = b.nx - 1;
-    const int i0 = 1 + (int)blockIdx.y * PK_W;
+    const int i0 = 1 + (tw % nwin) * PK_W;
     if (i0 > nxm) return;
     const int W = min(PK_W, nxm - i0 + 1);
     const int nsteps = nxm + T.tj + T.tk - 2;
@@ -670,7 +673,7 @@ template <int NV, int TJ>
 static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st) {
     if (!fenced) {
         using F = Pk<NV>;
-        dim3 tgrid((unsigned)L.n_fwd, (unsigned)L.nwin, (unsigned)NV);
+        dim3 tgrid((unsigned)(L.n_fwd * L.nwin), 1u, (unsigned)NV);
         if (forward) {
             hipLaunchKernelGGL((k_lusgs_team<NV, true, TJ>), dim3((unsigned)L.n_fwd), dim3(LT_NT), 0, st,
                                L.blks, L.fwd, L.pk, L.err, p);
@@ -679,7 +682,7 @@ static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, h
                                L.blks, L.bwd, L.pk, L.err, p);
             // DU back into the block layout (and the sentinel back into DUB and DUF)
             hipLaunchKernelGGL((k_lu_transpose<NV, false, TJ>), tgrid, dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p,
-                               F::DUB, F::DUB + NV, F::DUB + NV, F::DUB + NV);
+                               L.nwin, F::DUB, F::DUB + NV, F::DUB + NV, F::DUB + NV);
         }
         return;
     }
@@ -696,16 +699,16 @@ static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, h
 template <int NV, int TJ>
 static void pack_nv(const LuPlan& L, const Par& p, hipStream_t st) {
     using F = Pk<NV>;
-    hipLaunchKernelGGL((k_lu_transpose<NV, true, TJ>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin, (unsigned)(F::DUF + (F::S - F::LC))),
-                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p, F::U, F::DUF, F::LC, F::S);
+    hipLaunchKernelGGL((k_lu_transpose<NV, true, TJ>), dim3((unsigned)(L.n_fwd * L.nwin), 1u, (unsigned)(F::DUF + (F::S - F::LC))),
+                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p, L.nwin, F::U, F::DUF, F::LC, F::S);
 }
 
 // the geometry the sweeps read (S, n), into sweep order once per plan
 template <int NV, int TJ>
 static void pack_geometry_nv(const LuPlan& L, const Par& p, hipStream_t st) {
     using F = Pk<NV>;
-    hipLaunchKernelGGL((k_lu_transpose<NV, true, TJ>), dim3((unsigned)L.n_fwd, (unsigned)L.nwin, (unsigned)(F::NF - F::S)),
-                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p, F::S, F::NF, F::NF, F::NF);
+    hipLaunchKernelGGL((k_lu_transpose<NV, true, TJ>), dim3((unsigned)(L.n_fwd * L.nwin), 1u, (unsigned)(F::NF - F::S)),
+                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p, L.nwin, F::S, F::NF, F::NF, F::NF);
 }
 
 // NVAR and the plan's tile width as template arguments

@@ -46,8 +46,8 @@ PASS_STAGES = (("residual", ("pre", "inviscid", "viscous_dt")), ("lu_forward", (
 # HBM bytes per cell of each pass, measured with rocprofv3 FETCH_SIZE and WRITE_SIZE on the MI355X
 # for the single-block C-grid in FP64 and scaled by a plain device copy of known size (the summary
 # and its commands: profiles/iteration_kernels_naca0012_cgrid_257x97x33.md)
-MEASURED_TRAFFIC = {("naca0012_cgrid_257x97x33", "fp64"): {"residual": 3165.3, "lu_forward": 502.7,
-                                                           "lu_backward": 733.9, "update": 204.2}}
+MEASURED_TRAFFIC = {("naca0012_cgrid_257x97x33", "fp64"): {"residual": 3163.2, "lu_forward": 502.8,
+                                                           "lu_backward": 734.0, "update": 204.2}}
 FLUX_NAMES = {1: "Steger-Warming", 2: "HLL", 3: "HLLC", 4: "Roe", 5: "Van Leer", 6: "AUSMPW"}
 SCHEME_NAMES = {5: "MUSCL3", 7: "WENO5"}
 

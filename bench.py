@@ -8,7 +8,7 @@ region starts; the timed region is exactly --steps iterations bracketed by devic
 synchronisation, and HIP events at the stage boundaries (on the library's own stream) give the
 device time of each stage over the same iterations.
 
-Workloads:
+Workloads (the default is the largest single-GPU configuration, swept_wing_24block):
   naca0012_cgrid_257x97x33   single-block C-grid around NACA 0012, 257x97x33 nodes (786,432
                              cells), 2e-6 wall spacing, 20-chord far field; SA RANS, Re 6e6,
                              Mach 0.8, AoA 1.25 deg, WENO5 reconstruction, FP64
@@ -19,9 +19,16 @@ Workloads:
                              blocks (60,817,408 cells); SA, Re 1.2e7, Mach 0.84, AoA 3, WENO5 + Roe
 
 cpu_baseline: the compiled reference (oracle/_ref/ref_driver_omp, the reference's Fortran built
-from its own sources with amdflang) on a bounded number of iterations of the same workload,
-timed around its NS_Time_advance loop with every host core the process may use; when that binary
-is not present, the oracle's C restatement is timed instead (kind "port").
+from its own sources with amdflang) on a bounded number of iterations of the same workload (for
+the two large multi-block grids the same generator at half resolution per direction, with the
+wall distance the device computed for it handed over as wall_dist.dat, which the reference reads
+instead of its brute-force search), timed around its NS_Time_advance loop with every host core the
+process may use; when that binary is not present, the oracle's C restatement is timed instead
+(kind "port").
+
+roofline: the largest kernel of the iteration that runs alone in its stage (so the stage's HIP
+events time exactly that kernel), its algorithmic bytes per launch (the per-cell words of its pass
+times the cells of the launch) over its event-timed duration.
 """
 from __future__ import annotations
 
@@ -47,7 +54,12 @@ PASS_STAGES = (("residual", ("pre", "inviscid", "viscous_dt")), ("lu_forward", (
 # for the single-block C-grid in FP64 and scaled by a plain device copy of known size (the summary
 # and its commands: profiles/iteration_kernels_naca0012_cgrid_257x97x33.md)
 MEASURED_TRAFFIC = {("naca0012_cgrid_257x97x33", "fp64"): {"residual": 3163.2, "lu_forward": 502.8,
-                                                           "lu_backward": 734.0, "update": 204.2}}
+                                                           "lu_backward": 734.0, "update": 204.2},
+                    # profiles/iteration_kernels_swept_wing_24block.md (k_lusgs_team<6, true, 8>)
+                    ("swept_wing_24block", "fp64"): {"lu_forward": 546.4}}
+# stages that hold exactly one kernel launch per iteration: (kernel, the pass whose per-cell words
+# are its algorithmic bytes)
+SINGLE_KERNEL_STAGES = {"lu_forward": ("k_lusgs_team<{nv}, true, {tj}>", 1)}
 FLUX_NAMES = {1: "Steger-Warming", 2: "HLL", 3: "HLLC", 4: "Roe", 5: "Van Leer", 6: "AUSMPW"}
 SCHEME_NAMES = {5: "MUSCL3", 7: "WENO5"}
 
@@ -126,18 +138,41 @@ def host_cores() -> int:
     return max(1, n)
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(case, what: str, iters: int):
     """Reference CPU throughput on `iters` iterations of the workload sample (rank 0, N=1 only)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     cores = host_cores()
     cells = ncells(case)
     import refrun  # noqa: E402
+    if iters <= 0:
+        # about 75 M cell-iterations (10-30 s of the reference on the host), 3 to 20 iterations
+        iters = max(3, min(20, int(round(75e6 / cells))))
     sample = f"{iters} iterations of {what} ({cells} cells) from freestream"
     if refrun.available(omp=True):
         try:
-            r = refrun.run_case(case, nsteps=iters, kshow=10 ** 8, omp_threads=cores, dump=False, timeout=900)
+            dw = None
+            if C.nvar_of(case.params) > 5 and cells > 2_000_000:
+                # the wall distance of the sample from the device, as wall_dist.dat: the reference
+                # reads it instead of its O(cells x wall nodes) search
+                g = A.solver_from_case(case)
+                dw = g.wall_distance()
+                g.close()
+                sample += ", wall_dist.dat from the device's wall distance"
+            r = refrun.run_case(case, nsteps=iters, kshow=10 ** 8, omp_threads=cores, dump=False, timeout=900, wall_dist=dw)
             return dict(value=cells * iters / r["loop_seconds"] / 1e6, unit="Mcell-iter/s", cores=cores,
-                        kind="reference", sample=sample + " (reference Fortran, MPI 1 rank x OpenMP)")
+                        kind="reference", cpu=cpu_model(),
+                        sample=sample + " (reference Fortran, MPI 1 rank x OpenMP)")
         except Exception as e:  # the reference binary could not run on this host
             print(f"# reference CPU build failed to run: {e}", file=sys.stderr)
     import oracle  # noqa: E402
@@ -147,7 +182,7 @@ def cpu_baseline(case, what: str, iters: int):
     t0 = time.perf_counter()
     o.step(iters)
     dt = time.perf_counter() - t0
-    return dict(value=cells * iters / dt / 1e6, unit="Mcell-iter/s", cores=1, kind="port",
+    return dict(value=cells * iters / dt / 1e6, unit="Mcell-iter/s", cores=1, kind="port", cpu=cpu_model(),
                 sample=sample + " (oracle C restatement, serial loops)")
 
 
@@ -246,7 +281,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="naca0012_cgrid_257x97x33")
+    ap.add_argument("--workload", default="swept_wing_24block")
     ap.add_argument("--flux", type=int, default=4,
                     help="Iflag_Flux (default 4 Roe; 3 = the reference's HLLC, which diverges on "
                          "the C-grid within a few iterations)")
@@ -256,14 +291,16 @@ def main() -> int:
                     help="iterations per LU-SGS tile shape of the autotuner before the warm-up (0: no tuning)")
     ap.add_argument("--lu-tile", type=int, choices=(4, 8, 16), default=None,
                     help="fixed LU-SGS tile width instead of autotuning (profiling runs)")
-    ap.add_argument("--cpu-iters", type=int, default=20, help="iterations of the CPU baseline sample")
+    ap.add_argument("--cpu-iters", type=int, default=0,
+                    help="iterations of the CPU baseline sample (0: about 75 M cell-iterations, 3 to 20)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="precision of the headline run (fp64: the drop-in; mixed: FP32 reconstruction, "
                          "fluxes and SA terms)")
     ap.add_argument("--variants", default=None,
                     help="extra runs reported beside the headline, precision:batch pairs ('' for none; "
-                         "default mixed:1,fp64:4,mixed:4 on the single-block C-grid, none elsewhere)")
+                         "default mixed:1,fp64:4,mixed:4 on the single-block C-grid, mixed:1 on the swept wing, "
+                         "none elsewhere)")
     ap.add_argument("--serve", type=int, default=0, help="run the serving sweep with this many requests instead")
     ap.add_argument("--rate", type=float, default=2.0, help="serving: request arrivals per second")
     ap.add_argument("--drop", type=float, default=5e-3, help="serving: Res_rms(1) drop that ends a solve")
@@ -330,19 +367,25 @@ def main() -> int:
         t = sum(stage_ms[x] for x in stages) / args.steps
         alg = w * 8.0 * cells
         passes[pname] = dict(ms=t, alg_bytes=alg, gbs=alg / (t * 1e-3) / 1e9)
-    # the roofline's kernel: the forward LU-SGS sweep, one launch per iteration
-    # (k_lusgs_team<NV,true>), so its stage events time exactly that kernel.  On the 257x97x33
-    # C-grid it is the largest single kernel of the iteration (the residual pass is about twenty
-    # kernels, the largest of them a fifth of the sweep's time); on the large multi-block grids the
-    # residual kernels dominate, and `passes` reports every pass.
-    dom = "lu_forward" if args.lu != "hyperplane" else max(passes, key=lambda k: passes[k]["ms"])
-    d = passes[dom]
+    # the roofline's kernel: the largest kernel that runs alone in its stage, so the stage's
+    # HIP events (on the library's stream) time exactly that kernel over the timed launches
+    singles = {k: v for k, v in SINGLE_KERNEL_STAGES.items() if not (k.startswith("lu_") and args.lu == "hyperplane")}
+    stage_t = {k: stage_ms[k] / args.steps for k in singles}
+    if stage_t:
+        dom = max(stage_t, key=stage_t.get)
+        kname, pidx = singles[dom]
+        kernel = kname.format(nv=nv, tj=s_tile)
+        t = stage_t[dom]
+        alg = words[pidx] * 8.0 * cells
+    else:
+        dom = max(passes, key=lambda k: passes[k]["ms"])
+        kernel, t, alg = dom, passes[dom]["ms"], passes[dom]["alg_bytes"]
+    gbs = alg / (t * 1e-3) / 1e9
     measured = MEASURED_TRAFFIC.get((args.workload, args.precision), {}).get(dom)
-    roofline = dict(bound="hbm", achieved=round(d["gbs"], 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(d["gbs"] / HBM_PEAK_GBS, 5),
+    roofline = dict(bound="hbm", achieved=round(gbs, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(gbs / HBM_PEAK_GBS, 5),
                     traffic=round(measured * cells) if measured else None,
-                    kernel=f"k_lusgs_team<{nv}, true>" if dom == "lu_forward" else dom, stage=dom,
-                    ms_per_launch=round(d["ms"], 4), alg_bytes_per_launch=d["alg_bytes"])
+                    kernel=kernel, stage=dom, ms_per_launch=round(t, 4), alg_bytes_per_launch=alg)
     iter_bytes = sum(words) * 8.0 * cells
     out = {
         "metric": "Mcell-iter/s/GPU + %HBM roofline",
@@ -379,7 +422,7 @@ def main() -> int:
     }
     s.close()
     if args.variants is None:
-        args.variants = "mixed:1,fp64:4,mixed:4" if args.workload == "naca0012_cgrid_257x97x33" else ""
+        args.variants = {"naca0012_cgrid_257x97x33": "mixed:1,fp64:4,mixed:4", "swept_wing_24block": "mixed:1"}.get(args.workload, "")
     if args.variants:
         out["variants"] = []
         for v in args.variants.split(","):

@@ -48,6 +48,13 @@ typedef struct ocfd_params {
                                 viscous) and the SA face and source terms in FP32; metrics, LU-SGS
                                 (Jacobians, diagonal), update, limiter and norms in FP64.  Laminar
                                 and SA only (SST is refused with OCFD_EUNSUPPORTED). */
+    int32_t IF_InnerFlow;      /* IF_InnerFlow (sub_modules.f90:163).  1 turns bc 5/6 into the total-
+                                  pressure inlet and the outlet of the inner-flow mode and changes
+                                  the initial field (sub_boundary.f90:28-40; sub_init.f90:322): such
+                                  a case is refused with OCFD_EUNSUPPORTED when it has a bc 5 or 6
+                                  subface, and ocfd_init_flow refuses it. */
+    int32_t IF_TurboMachinary; /* IF_TurboMachinary (sub_modules.f90:162).  1 (the turbomachinery
+                                  solver) is outside this path: ocfd_create refuses it. */
 } ocfd_params;
 
 /* Geometry of one block as Comput_Goemetric_var leaves it (sub_geometry.f90:6-346). */
@@ -95,6 +102,8 @@ int ocfd_download_U(ocfd_ctx *ctx, int32_t block_no, double *U);
 #define OCFD_F_DT 3
 #define OCFD_F_MU 4
 #define OCFD_F_MUT 5
+#define OCFD_F_DW 6      /* wall distance dw (cells 1..n-1; zero elsewhere), e.g. for write_dw
+                            (sub_IO.f90:573-644) after ocfd_setup_geometry computed it */
 int ocfd_download_field(ocfd_ctx *ctx, int32_t block_no, int32_t field, double *out);
 
 /* Init_flow, freestream branch (sub_init.f90:264-410): freestream on every cell, then BC and halo */
@@ -120,6 +129,7 @@ int ocfd_force_vt_kw(ocfd_ctx *ctx);
 /* comput_max_Res_onemesh (sub_time_advance.f90:330-380): Res_max(NVAR), Res_rms(NVAR).
  * Returns OCFD_ENAN when the residual holds a NaN. */
 int ocfd_res_norms(ocfd_ctx *ctx, double *res_max, double *res_rms);
+
 
 /* comput_force (sub_Post.f90:4-265): out[0..8] = Ft0(1:9) after scaling, out[9..11] = CL CD CS */
 int ocfd_forces(ocfd_ctx *ctx, double *out12);
@@ -181,8 +191,9 @@ int ocfd_set_lu_tile(ocfd_ctx *ctx, int32_t tj);
 
 /* Failure-path switch of the pipelined LU-SGS sweeps of this context (not of a batch).  A sweep
  * wait normally gives up after 2^26 polls of an upstream tile (seconds); the step then returns
- * OCFD_EHIP, the iteration's state is invalid, and the sweep tables are restored for the next
- * one.  With on = 1 every wait that finds its producer behind gives up at its first poll, which
+ * OCFD_EHIP, U is left at the last iteration completed before the failure (the update, limiter
+ * and force_vt_kw kernels of that and every later iteration of the call leave it as it is, and
+ * the later sweeps return at once), and the sweep tables are restored for the next call.  With on = 1 every wait that finds its producer behind gives up at its first poll, which
  * forces that path; on = 0 restores normal waiting.  No counterpart in the reference. */
 int ocfd_set_lu_fail_waits(ocfd_ctx *ctx, int32_t on);
 int ocfd_get_lu_tile(ocfd_ctx *ctx, int32_t *tj);
@@ -200,7 +211,12 @@ int ocfd_autotune_lu(ocfd_ctx *ctx, int32_t niter, int32_t *best_tj, double *ms)
  * every kernel of the iteration covering the blocks of all of them in one launch; each context
  * keeps its own parameters (Mach, AoA, Re, ...), state, counters, norms and forces.  A context is
  * added after its geometry and state are set; while it is in a batch it is stepped through the
- * batch only.  Results are those of the context stepped alone, bit for bit. */
+ * batch only: ocfd_step_lusgs, ocfd_step_timed, ocfd_residual, ocfd_update_limit, ocfd_bc,
+ * ocfd_halo, ocfd_force_vt_kw, ocfd_init_flow and ocfd_autotune_lu refuse a member with OCFD_EARG
+ * (ocfd_res_norms, ocfd_forces and the downloads work).  A context is in at most one batch.
+ * ocfd_destroy of a member first removes it from its batch; ocfd_batch_destroy releases its
+ * members, which may then be stepped alone.  Results are those of the context stepped alone, bit
+ * for bit. */
 typedef struct ocfd_batch ocfd_batch;
 int ocfd_batch_create(ocfd_batch **out);
 int ocfd_batch_destroy(ocfd_batch *batch);

@@ -48,6 +48,12 @@ __device__ __host__ __forceinline__ size_t JK(const Blk& b, int j, int k) { retu
 __device__ __host__ __forceinline__ size_t IK(const Blk& b, int i, int k) { return (size_t)(i + 3) + (size_t)b.NI * (size_t)(k + 3); }
 __device__ __host__ __forceinline__ size_t IJ(const Blk& b, int i, int j) { return (size_t)(i + 3) + (size_t)b.NI * (size_t)(j + 3); }
 
+// what the residual norms read of a block
+struct NormBlk {
+    const double* res[MAXV];
+    int nx, ny, nz, NI, NJ;
+};
+
 // one subface of the .inc table (sub_modules.f90:63-68)
 struct Sub {
     int ib, ie, jb, je, kb, ke, bc, face, f_no;
@@ -79,7 +85,11 @@ struct BSet {
     const Par* par;
     const int* tile_blk;     // for the launch's space
     const int* tile0;
+    const int* halt;         // nonzero *halt: an LU-SGS sweep of this iteration gave up; the update
+                             // kernels then leave U as it is (may be null)
 };
+
+__device__ __forceinline__ bool halted(const BSet& S) { return S.halt && *(volatile const int*)S.halt != 0; }
 
 // Workgroups of a one-dimensional grid are handed to the 8 XCDs in turn (workgroup t to XCD
 // t mod 8).  Renumbering so that XCD x runs the x-th eighth of the tiles in order keeps the

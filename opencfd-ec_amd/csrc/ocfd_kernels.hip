@@ -360,10 +360,21 @@ __global__ void __launch_bounds__(64) k_sst_finish(BSet S) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// flux_inviscous_i/j/k (sub_Residual.f90:182-705): one thread per face, face flux stored
-template <int DIR, typename T, int RECON>
-__global__ void __launch_bounds__(64) k_inv_flux(BSet S) {
-    OCFD_LOCATE(SP_FI + DIR)
+// the 1-D flux of the face frame: FLUX fixes it at compile time (the register budget of a face
+// kernel is then that of one solver); FLUX = 0 selects it at run time from iflux
+template <int FLUX, typename T>
+__device__ __forceinline__ void riemann_t(const T* QL, const T* QR, T* F, T gamma, int iflux) {
+    if constexpr (FLUX == 0) riemann(QL, QR, F, gamma, iflux);
+    else if constexpr (FLUX == F_SW) flux_sw(QL, QR, F, gamma);
+    else if constexpr (FLUX == F_ROE) flux_roe(QL, QR, F, gamma);
+    else if constexpr (FLUX == F_VL) flux_vl(QL, QR, F, gamma);
+    else if constexpr (FLUX == F_AUSM) flux_ausmpw(QL, QR, F, gamma);
+    else flux_hll_hllc(QL, QR, F, gamma, FLUX);
+}
+
+// flux_inviscous_i/j/k (sub_Residual.f90:182-705) of face (i,j,k) of direction DIR, stored
+template <int DIR, typename T, int RECON, int FLUX>
+__device__ __forceinline__ void inv_face(const Blk& b, const Par& p, int i, int j, int k) {
     size_t s = IX(b, i, j, k);
     int pos = DIR == 0 ? i : DIR == 1 ? j : k;
     int nd = DIR == 0 ? b.nx : DIR == 1 ? b.ny : b.nz;
@@ -447,19 +458,13 @@ __global__ void __launch_bounds__(64) k_inv_flux(BSet S) {
     QL[3] = (UL[1] * s3x + UL[2] * s3y) + UL[3] * s3z; QL[4] = UL[4];
     QR[0] = UR[0]; QR[1] = (UR[1] * s1x + UR[2] * s1y) + UR[3] * s1z; QR[2] = (UR[1] * s2x + UR[2] * s2y) + UR[3] * s2z;
     QR[3] = (UR[1] * s3x + UR[2] * s3y) + UR[3] * s3z; QR[4] = UR[4];
-    riemann(QL, QR, F0, T(p.gamma), p.flux);
+    riemann_t<FLUX, T>(QL, QR, F0, T(p.gamma), p.flux);
     T Sf = T(b.S[DIR][s]);
     put_F(b, DIR, 0, s, -F0[0] * Sf);
     put_F(b, DIR, 1, s, -((F0[1] * s1x + F0[2] * s2x) + F0[3] * s3x) * Sf);
     put_F(b, DIR, 2, s, -((F0[1] * s1y + F0[2] * s2y) + F0[3] * s3y) * Sf);
     put_F(b, DIR, 3, s, -((F0[1] * s1z + F0[2] * s2z) + F0[3] * s3z) * Sf);
     put_F(b, DIR, 4, s, -F0[4] * Sf);
-    if (p.turb == 2) {
-        T fv, f2;
-        sa_face<DIR, T>(b, p, i, j, k, fv, f2);
-        put_Fsa(b, DIR, 0, s, fv);
-        put_Fsa(b, DIR, 1, s, f2);
-    }
 }
 
 // explicit average-state flux on a physical k subface (sub_Residual.f90:667-688)
@@ -483,11 +488,10 @@ __global__ void __launch_bounds__(64) k_inv_kbc(SubSet S) {
     put_F(b, 2, 4, s, T(-((E0 + p0) * un) * Sk));
 }
 
-// flux_viscous_i/j/k (sub_Residual.f90:712-1247)
+// flux_viscous_i/j/k (sub_Residual.f90:712-1247) of face (i,j,k) of direction DIR, stored
 template <int DIR, typename T>
-__global__ void __launch_bounds__(64) k_visc_flux(BSet S) {
+__device__ __forceinline__ void visc_face(const Blk& b, const Par& p, int i, int j, int k) {
     constexpr int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
-    OCFD_LOCATE(SP_FI + DIR)
     const T tmp1 = T(4.0 / 3.0), tmp2 = T(2.0 / 3.0);
     size_t s = IX(b, i, j, k);
     if (b.flags[0] == 1) {      // over-limit block: no viscous flux, zero wall forces (:734-750)
@@ -561,6 +565,25 @@ __global__ void __launch_bounds__(64) k_visc_flux(BSet S) {
         size_t m = DIR == 0 ? JK(b, j, k) : DIR == 1 ? IK(b, i, k) : IJ(b, i, j);
         b.surf[f][0][m] = f1; b.surf[f][1][m] = f2; b.surf[f][2][m] = f3;
     }
+}
+
+// Every flux through face (i,j,k) of direction DIR, one thread per face: the SA face terms
+// (sub_turbulence_SA.f90:60-174), the inviscid flux and the viscous flux, each stored for the cell
+// pass.  The SA terms go first, so their registers are free again for the reconstruction; the
+// viscous flux, mostly loads of the face's gradient stencil and metrics, overlaps the inviscid
+// flux's arithmetic in the same kernel.
+template <int DIR, typename T, int RECON, int FLUX>
+__global__ void __launch_bounds__(64) k_face(BSet S) {
+    OCFD_LOCATE(SP_FI + DIR)
+    if (p.turb == 2) {
+        const size_t s = IX(b, i, j, k);
+        T fv, f2;
+        sa_face<DIR, T>(b, p, i, j, k, fv, f2);
+        put_Fsa(b, DIR, 0, s, fv);
+        put_Fsa(b, DIR, 1, s, f2);
+    }
+    inv_face<DIR, T, RECON, FLUX>(b, p, i, j, k);
+    if (p.visc == 1) visc_face<DIR, T>(b, p, i, j, k);
 }
 
 // Res(1:5): inviscid i, j, k then viscous i, j, k face differences, in the reference order
@@ -690,6 +713,7 @@ __global__ void k_lusgs_plane(Blk b, Par p, int plane) {
 // U = Un + DU (sub_time_advance.f90:92-101) and the limit_flow_U snapshot/detection
 // (sub_limitflow.f90:46-94).  The snapshot (rho,u,v,w,p) goes to prim[0..4].
 __global__ void __launch_bounds__(64) k_update_detect(BSet S) {
+    if (halted(S)) return;
     OCFD_LOCATE(SP_CELL)
     size_t s = IX(b, i, j, k);
     double U[MAXV];
@@ -710,6 +734,7 @@ __device__ __forceinline__ bool lim_bad(const Blk& b, const Par& p, size_t s) {
 // limit_flow_U replacement step (sub_limitflow.f90:84-150): bad cells take the mean of the valid
 // 3x3x3 neighbours of the snapshot, or their own clamped values; sets the IF_OverLimit latch
 __global__ void __launch_bounds__(64) k_limit_apply(BSet S) {
+    if (halted(S)) return;
     OCFD_LOCATE(SP_CELL)
     if (b.flags[1] == 0) return;
     if (i == 1 && j == 1 && k == 1) b.flags[0] = 1;
@@ -775,6 +800,7 @@ __device__ __forceinline__ void sa_replace(const Blk& b, const Par& p, int i, in
 }
 
 __global__ void __launch_bounds__(64) k_limit_sa_detect(BSet S) {
+    if (halted(S)) return;
     OCFD_LOCATE(SP_CELL)
     size_t s = IX(b, i, j, k);
     double v = b.U[5][s];
@@ -789,6 +815,7 @@ __global__ void __launch_bounds__(64) k_limit_sa_detect(BSet S) {
 }
 
 __global__ void __launch_bounds__(64) k_limit_sa_isolated(BSet S) {
+    if (halted(S)) return;
     OCFD_LOCATE(SP_CELL)
     if (b.flags[2] == 0) return;
     long c = sa_cell_index(b, i, j, k);
@@ -806,6 +833,7 @@ __global__ void __launch_bounds__(64) k_limit_sa_isolated(BSet S) {
 
 // one workgroup per block: the remaining (clustered) cells in cell order; clears the mask
 __global__ void __launch_bounds__(64) k_limit_sa_ordered(BSet S) {
+    if (halted(S)) return;
     const Blk& b = S.blk[blockIdx.x];
     const Par& p = S.par[blockIdx.x];
     if (b.flags[2] == 0) return;
@@ -842,6 +870,7 @@ __global__ void k_reset_flags(BSet S) {
 
 // force_vt_kw (sub_time_advance.f90:27-61)
 __global__ void __launch_bounds__(64) k_force_vt_kw(BSet S) {
+    if (halted(S)) return;
     OCFD_LOCATE(SP_CELL)
     size_t s = IX(b, i, j, k);
     for (int m = 5; m < p.NV; m++) if (b.U[m][s] < 0) b.U[m][s] = 1.0e-10;
@@ -1070,10 +1099,11 @@ __global__ void k_corner_edges1(Blk b, int NV) {
 __global__ void k_corner_vertices1(Blk b) { d_corner_vertices(b, threadIdx.x, blockIdx.x); }
 
 // ---------------------------------------------------------------------------------------------
-// comput_max_Res_onemesh (sub_time_advance.f90:330-380): per-block partial max and sum of
-// squares in a fixed tree order (deterministic, independent of co-scheduled work)
-__global__ void k_norms(BSet S, int NV, double* part) {
-    const Blk& b = S.blk[blockIdx.z];
+// comput_max_Res_onemesh (sub_time_advance.f90:330-380): per block, component and part (a fixed
+// slice of the block's cells) the max |Res|, the sum of squares and a NaN flag, each part reduced
+// in a fixed tree order (deterministic, independent of co-scheduled work)
+__global__ void k_norms(const NormBlk* tab, int NV, double* part) {
+    const NormBlk& b = tab[blockIdx.z];
     __shared__ double smax[256], ssum[256];
     __shared__ int snan[256];
     int m = blockIdx.y;
@@ -1081,7 +1111,7 @@ __global__ void k_norms(BSet S, int NV, double* part) {
     double mx = 0.0, sm = 0.0; int nan = 0;
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
         int i = (int)(c % (b.nx - 1)) + 1, j = (int)((c / (b.nx - 1)) % (b.ny - 1)) + 1, k = (int)(c / ((long)(b.nx - 1) * (b.ny - 1))) + 1;
-        double r = b.Res[m][IX(b, i, j, k)];
+        double r = b.res[m][(size_t)(i + 3) + (size_t)b.NI * ((size_t)(j + 3) + (size_t)b.NJ * (size_t)(k + 3))];
         if (r != r) nan = 1;
         mx = fmax(mx, fabs(r));
         sm = sm + r * r;
@@ -1103,46 +1133,84 @@ __global__ void k_norms(BSet S, int NV, double* part) {
     (void)NV;
 }
 
-// comput_force (sub_Post.f90:4-265): one wall subface, per-face contributions summed in the
-// reference loop order by a single work-group tree is not required for the metric, so a
-// fixed-order serial sum per subface is used (surface work, a few thousand faces).
-__global__ void k_force_sub(Blk b, Par p, Sub B, double* acc9) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int f = B.face, d = (f - 1) % 3;
+// the parts of every block combined in the order of the reference's block loop: per component,
+// blocks in order and each block's parts in order; out[v] = max |Res_v|, out[NV + v] = sum of
+// squares, out[2 NV + v] = 1 when a NaN was seen
+__global__ void k_norms_final(int nb, int NV, int P, const double* part, double* out) {
+    const int v = (int)threadIdx.x;
+    if (v >= NV) return;
+    double mx = 0.0, sm = 0.0, nan = 0.0;
+    for (int b = 0; b < nb; b++)
+        for (int q = 0; q < P; q++) {
+            const double* o = part + 3 * (((size_t)b * NV + v) * P + q);
+            mx = mx < o[0] ? o[0] : mx;
+            sm = sm + o[1];
+            if (o[2] != 0.0) nan = 1.0;
+        }
+    out[v] = mx; out[NV + v] = sm; out[2 * NV + v] = nan;
+}
+
+// comput_force (sub_Post.f90:4-265): the pressure force, the viscous force (Surf1..6) and their
+// moments about the centroid, one thread per wall face.  Every workgroup of 64 faces sums its
+// faces in a fixed tree order into one part; k_force_final adds the parts in tile order (blocks,
+// subfaces, face rows): deterministic, though not the reference's serial order.
+__global__ void __launch_bounds__(64) k_force_parts(SubSet S, double cx, double cy, double cz, double* part) {
+    OCFD_SUB_LOCATE
+    const int f = B.face, d = (f - 1) % 3;
     int lo1, hi1, lo2, hi2;
     if (d == 0) { lo1 = B.kb; hi1 = B.ke - 1; lo2 = B.jb; hi2 = B.je - 1; }
     else if (d == 1) { lo1 = B.kb; hi1 = B.ke - 1; lo2 = B.ib; hi2 = B.ie - 1; }
     else { lo1 = B.jb; hi1 = B.je - 1; lo2 = B.ib; hi2 = B.ie - 1; }
-    double g = p.gamma, p_inf = 1.0 / (g * p.Ma * p.Ma);
-    double A[9];
-    for (int q = 0; q < 9; q++) A[q] = acc9[q];
-    for (int o = lo1; o <= hi1; o++) for (int in = lo2; in <= hi2; in++) {
+    const int o = lo1 + c, in = lo2 + a;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (o <= hi1 && in <= hi2) {
         size_t s1, s0, sf, sm;
         if (d == 0) { int j = in, k = o; s1 = IX(b, f == 1 ? 1 : b.nx - 1, j, k); s0 = IX(b, f == 1 ? 0 : b.nx, j, k); sf = IX(b, f == 1 ? 1 : b.nx, j, k); sm = JK(b, j, k); }
         else if (d == 1) { int i = in, k = o; s1 = IX(b, i, f == 2 ? 1 : b.ny - 1, k); s0 = IX(b, i, f == 2 ? 0 : b.ny, k); sf = IX(b, i, f == 2 ? 1 : b.ny, k); sm = IK(b, i, k); }
         else { int i = in, j = o; s1 = IX(b, i, j, f == 3 ? 1 : b.nz - 1); s0 = IX(b, i, j, f == 3 ? 0 : b.nz); sf = IX(b, i, j, f == 3 ? 1 : b.nz); sm = IJ(b, i, j); }
+        const double g = p.gamma, p_inf = 1.0 / (g * p.Ma * p.Ma);
         double p1 = (g - 1.0) * (b.U[4][s1] - ((b.U[1][s1] * b.U[1][s1] + b.U[2][s1] * b.U[2][s1]) + b.U[3][s1] * b.U[3][s1]) / b.U[0][s1]);
         double p2 = (g - 1.0) * (b.U[4][s0] - ((b.U[1][s0] * b.U[1][s0] + b.U[2][s0] * b.U[2][s0]) + b.U[3][s0] * b.U[3][s0]) / b.U[0][s0]);
         double Pw = (f <= 3) ? -(0.5 * (p1 + p2) - p_inf) * b.S[d][sf] : (0.5 * (p1 + p2) - p_inf) * b.S[d][sf];
         double n1 = b.nv[3 * d][sf], n2 = b.nv[3 * d + 1][sf], n3 = b.nv[3 * d + 2][sf];
-        double sv0 = b.surf[f - 1][0][sm], sv1 = b.surf[f - 1][1][sm], sv2 = b.surf[f - 1][2][sm];
-        A[0] = A[0] + Pw * n1; A[1] = A[1] + Pw * n2; A[2] = A[2] + Pw * n3;
-        double fx0, fy0, fz0;
-        if (f <= 3) { A[3] = A[3] + sv0; A[4] = A[4] + sv1; A[5] = A[5] + sv2; fx0 = Pw * n1 + sv0; fy0 = Pw * n2 + sv1; fz0 = Pw * n3 + sv2; }
-        else { A[3] = A[3] - sv0; A[4] = A[4] - sv1; A[5] = A[5] - sv2; fx0 = Pw * n1 - sv0; fy0 = Pw * n2 - sv1; fz0 = Pw * n3 - sv2; }
-        double xc = (b.xc[s1] + b.xc[s0]) * 0.5 - acc9[9], yc = (b.yc[s1] + b.yc[s0]) * 0.5 - acc9[10], zc = (b.zc[s1] + b.zc[s0]) * 0.5 - acc9[11];
-        A[6] = A[6] + (yc * fz0 - zc * fy0);
-        A[7] = A[7] + (zc * fx0 - xc * fz0);
-        A[8] = A[8] + (xc * fy0 - yc * fx0);
+        double sg = f <= 3 ? 1.0 : -1.0;
+        double sv0 = sg * b.surf[f - 1][0][sm], sv1 = sg * b.surf[f - 1][1][sm], sv2 = sg * b.surf[f - 1][2][sm];
+        v[0] = Pw * n1; v[1] = Pw * n2; v[2] = Pw * n3;
+        v[3] = sv0; v[4] = sv1; v[5] = sv2;
+        double fx0 = Pw * n1 + sv0, fy0 = Pw * n2 + sv1, fz0 = Pw * n3 + sv2;
+        double xc = (b.xc[s1] + b.xc[s0]) * 0.5 - cx, yc = (b.yc[s1] + b.yc[s0]) * 0.5 - cy, zc = (b.zc[s1] + b.zc[s0]) * 0.5 - cz;
+        v[6] = yc * fz0 - zc * fy0;
+        v[7] = zc * fx0 - xc * fz0;
+        v[8] = xc * fy0 - yc * fx0;
     }
-    for (int q = 0; q < 9; q++) acc9[q] = A[q];
+    __shared__ double sh[9][64];
+    const int t = (int)threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 9; q++) sh[q][t] = v[q];
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+        if (t < w) {
+#pragma unroll
+            for (int q = 0; q < 9; q++) sh[q][t] = sh[q][t] + sh[q][t + w];
+        }
+        __syncthreads();
+    }
+    if (t < 9) part[(size_t)blockIdx.x * 9 + t] = sh[t][0];
+}
+
+__global__ void k_force_final(int nparts, const double* part, double* out9) {
+    const int q = (int)threadIdx.x;
+    if (q >= 9) return;
+    double acc = 0.0;
+    for (int w = 0; w < nparts; w++) acc = acc + part[(size_t)w * 9 + q];
+    out9[q] = acc;
 }
 
 // ---------------------------------------------------------------------------------------------
 // host launchers.  Every launch covers all blocks of the set; the per-block conditions of the
 // reference (turbulence model, limiter, NVAR) are uniform over a set.
 static inline dim3 g3(int n0, int n1, int n2, int bs) { return dim3((unsigned)((n0 + bs - 1) / bs), (unsigned)n1, (unsigned)n2); }
-static inline BSet bset(const SetPlan& P, int sp) { return BSet{P.blk, P.par, P.tile_blk[sp], P.tile0[sp]}; }
+static inline BSet bset(const SetPlan& P, int sp) { return BSet{P.blk, P.par, P.tile_blk[sp], P.tile0[sp], P.halt}; }
 
 template <int SP>
 static void launch_space(void (*kern)(BSet), const SetPlan& P, hipStream_t st) {
@@ -1180,33 +1248,40 @@ void set_pre(const SetPlan& P, hipStream_t st) {
     }
 }
 
-template <typename T, int R>
-static void inviscid_r(const SetPlan& P, hipStream_t st) {
-    launch_space<SP_FI>(k_inv_flux<0, T, R>, P, st);
-    launch_space<SP_FJ>(k_inv_flux<1, T, R>, P, st);
-    launch_space<SP_FK>(k_inv_flux<2, T, R>, P, st);
+template <typename T, int R, int FL>
+static void faces_rf(const SetPlan& P, hipStream_t st) {
+    launch_space<SP_FI>(k_face<0, T, R, FL>, P, st);
+    launch_space<SP_FJ>(k_face<1, T, R, FL>, P, st);
+    launch_space<SP_FK>(k_face<2, T, R, FL>, P, st);
 }
-// the reconstruction variable set (IFlag_Reconstruction) as a template argument: each kernel
-// holds only its own reconstruction
+// the reconstruction variable set (IFlag_Reconstruction) and, for the primitive-variable
+// reconstruction, the 1-D flux as template arguments: each kernel holds only its own
+// reconstruction and Riemann solver
 template <typename T>
-static void inviscid_t(const SetPlan& P, hipStream_t st) {
-    if (P.p0.recon == 0) inviscid_r<T, 0>(P, st);
-    else if (P.p0.recon == 1) inviscid_r<T, 1>(P, st);
-    else inviscid_r<T, 2>(P, st);
+static void faces_t(const SetPlan& P, hipStream_t st) {
+    if (P.p0.recon == 0) {
+        switch (P.p0.flux) {
+        case F_SW: faces_rf<T, 0, F_SW>(P, st); break;
+        case F_HLL: faces_rf<T, 0, F_HLL>(P, st); break;
+        case F_HLLC: faces_rf<T, 0, F_HLLC>(P, st); break;
+        case F_ROE: faces_rf<T, 0, F_ROE>(P, st); break;
+        case F_VL: faces_rf<T, 0, F_VL>(P, st); break;
+        default: faces_rf<T, 0, F_AUSM>(P, st); break;
+        }
+    } else if (P.p0.recon == 1) {
+        faces_rf<T, 1, 0>(P, st);
+    } else {
+        faces_rf<T, 2, 0>(P, st);
+    }
     launch_subs(k_inv_kbc<T>, P, P.kbc, st);
 }
 void set_inviscid(const SetPlan& P, hipStream_t st) {
-    if (P.p0.mixed) inviscid_t<float>(P, st);
-    else inviscid_t<double>(P, st);
+    if (P.p0.mixed) faces_t<float>(P, st);
+    else faces_t<double>(P, st);
 }
 
 template <typename T>
 static void viscous_dt_t(const SetPlan& P, hipStream_t st) {
-    if (P.p0.visc == 1) {
-        launch_space<SP_FI>(k_visc_flux<0, T>, P, st);
-        launch_space<SP_FJ>(k_visc_flux<1, T>, P, st);
-        launch_space<SP_FK>(k_visc_flux<2, T>, P, st);
-    }
     launch_space<SP_CELL>(k_cell<T>, P, st);
 }
 void set_viscous_dt(const SetPlan& P, hipStream_t st) {
@@ -1240,8 +1315,9 @@ void set_force_vt_kw(const SetPlan& P, hipStream_t st) {
 
 int norms_parts() { return 64; }
 
-void set_norms(const SetPlan& P, double* part, hipStream_t st) {
-    hipLaunchKernelGGL(k_norms, dim3((unsigned)norms_parts(), (unsigned)P.p0.NV, (unsigned)P.nb), dim3(256), 0, st, bset(P, SP_CELL), P.p0.NV, part);
+void launch_norms(const NormBlk* tab, int nb, int NV, double* part, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_norms, dim3((unsigned)norms_parts(), (unsigned)NV, (unsigned)nb), dim3(256), 0, st, tab, NV, part);
+    hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(64), 0, st, nb, NV, norms_parts(), (const double*)part, out);
 }
 
 // ---- per-block launches: the hyperplane LU-SGS schedule, the setup-time halo, forces ----
@@ -1290,8 +1366,11 @@ void launch_halo(const Blk& src, const Blk& dst, const Sub& B, int NV, hipStream
     hipLaunchKernelGGL(k_halo, GRID1(n, 256), 0, st, src, dst, h, NV);
 }
 
-void launch_force_sub(const Blk& b, const Par& p, const Sub& B, double* acc12, hipStream_t st) {
-    hipLaunchKernelGGL(k_force_sub, dim3(1), dim3(64), 0, st, b, p, B, acc12);
+void launch_forces(const SetPlan& P, const double cen[3], double* part, double* out9, hipStream_t st) {
+    if (P.force.ntiles > 0)
+        hipLaunchKernelGGL(k_force_parts, dim3((unsigned)P.force.ntiles), dim3(64), 0, st,
+                           SubSet{P.blk, P.par, P.force.t, P.force.tile_task}, cen[0], cen[1], cen[2], part);
+    hipLaunchKernelGGL(k_force_final, dim3(1), dim3(64), 0, st, P.force.ntiles, (const double*)part, out9);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -3,6 +3,7 @@
 #include <vector>
 
 #include "ocfd_dev.h"
+#include "../../include/ocfd_ec.h"
 
 namespace ocfd {
 
@@ -27,9 +28,10 @@ struct SetPlan {
     int* tile_blk[SP_COUNT] = {};
     int* tile0[SP_COUNT] = {};
     int ntiles[SP_COUNT] = {};
-    SubList bc, amut, kbc;
+    SubList bc, amut, kbc, force;   // force: the wall (bc 2) subfaces, one thread per face
     HaloList halo;
     int mx = 0;                      // max over blocks of max(nx, ny, nz)
+    const int* halt = nullptr;       // the LU-SGS error word of the engine's sweeps (or null)
 };
 
 void set_pre(const SetPlan& P, hipStream_t st);
@@ -39,14 +41,22 @@ void set_update_limit(const SetPlan& P, hipStream_t st);
 void set_bc(const SetPlan& P, hipStream_t st);
 void set_halo(const SetPlan& P, hipStream_t st);
 void set_force_vt_kw(const SetPlan& P, hipStream_t st);
+// the residual norms of a table of blocks: parts (nb x NV x norms_parts() x 3 doubles), then
+// out (3 NV doubles: max, sum of squares, NaN flag per component)
 int norms_parts();
-void set_norms(const SetPlan& P, double* part, hipStream_t st);
+void launch_norms(const NormBlk* tab, int nb, int NV, double* part, double* out, hipStream_t st);
+// the residual norms of a context in a caller-provided device buffer (ocfd_runtime.hip)
+constexpr int NORMS_INTO_MAX_BLOCKS = 256;
+int norms_into(ocfd_ctx* c, char* dev, size_t cap, double* res_max, double* res_rms);
+int ctx_device(const ocfd_ctx* c);
+int set_error(int code, const char* msg);
 
 HaloMap halo_map(const Sub& B);
 int halo_points(const HaloMap& h);
 void launch_halo(const Blk& src, const Blk& dst, const Sub& B, int NV, hipStream_t st);
 void launch_lusgs_sweep(const Blk& b, const Par& p, bool forward, hipStream_t st);
-void launch_force_sub(const Blk& b, const Par& p, const Sub& B, double* acc12, hipStream_t st);
+// comput_force over the set's wall faces: parts (ntiles x 9 doubles), then out9 (Ft(1:9))
+void launch_forces(const SetPlan& P, const double cen[3], double* part, double* out9, hipStream_t st);
 int check_division(long long n, unsigned long long seed, int emax, long long* mismatches);
 std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, int nfields, int tj, long long* total_elements);
 bool lusgs_tile_ok(int tj);

@@ -47,9 +47,14 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_sweep(const Blk* __restrict__ b
                                                        int* __restrict__ prog, int* __restrict__ prog_other,
                                                        int n_other, int* __restrict__ err, Par p) {
     __shared__ LuSlot<NV> slot[2][LU_NT];
+    __shared__ int gave_up;
     const int w = blockIdx.x, lane = threadIdx.x;
     // this launch re-arms the counters of the other sweep (same tile count) for its next launch
     if (lane == 0 && w < n_other) prog_other[w] = -1;
+    // a sweep of this iteration (or an earlier one of the same launch sequence) gave up: nothing to do
+    if (lane == 0) gave_up = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (gave_up != 0) return;
 
     const LuTask T = tasks[w];
     const Blk& b = blks[T.blk];
@@ -604,8 +609,11 @@ __global__ void __launch_bounds__(LT_NT) k_lusgs_team(const Blk* __restrict__ bl
                                                       double* __restrict__ pk, int* __restrict__ err, Par p) {
     __shared__ TeamShared<NV> sh;
     const int wave = threadIdx.x / LU_NT, lane = threadIdx.x % LU_NT;
-    if (threadIdx.x == 0) sh.bail = 0;
+    // a sweep earlier in the launch sequence gave up waiting: this one does nothing either (its
+    // DU stays unused, the update kernels leave U as it is)
+    if (threadIdx.x == 0) sh.bail = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
+    if (sh.bail != 0) return;
     const LuTask T = tasks[blockIdx.x];
     const Blk& b = blks[T.blk];
     if (wave == 0) team_sweep<NV, FWD, 0, TJ>(sh, b, T, pk, err, p, lane);

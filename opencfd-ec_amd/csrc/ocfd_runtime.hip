@@ -32,7 +32,6 @@ struct HostBlock {
     std::vector<Sub> subs;
     Blk d{};
     std::vector<void*> allocs;
-    double* force_acc = nullptr;     // 12 doubles: 9 accumulators + centroid
     std::vector<double> nodes[3];    // node coordinates awaiting ocfd_setup_geometry
     size_t bytes = 0;                // device memory of the block's arrays
     // the block's arrays are carved out of one device allocation: a sizing pass measures it
@@ -75,13 +74,16 @@ struct ocfd_ctx {
     ocfd_params hp{};
     Par p{};
     std::vector<HostBlock> blocks;   // index block_no - 1
-    double* norm_part = nullptr;
-    size_t norm_part_n = 0;
+    char* norm_buf = nullptr;        // residual norms: block table, parts and result (ocfd_res_norms)
+    size_t norm_bytes = 0;
+    double* force_buf = nullptr;     // forces: per-tile parts and the 9 sums (ocfd_forces)
+    size_t force_n = 0;
     int kstep = 0;
     double tt = 0.0;
     int device = 0;
     size_t bytes = 0;                // device memory of the block arrays
     Engine e;
+    ocfd_batch* batch = nullptr;     // the batch the context steps through, if any
     std::vector<BlockRef> refs() {
         std::vector<BlockRef> r;
         for (HostBlock& hb : blocks) r.push_back(BlockRef{&hb, &p, 0});
@@ -166,7 +168,6 @@ int carve_block(ocfd_ctx* c, HostBlock& hb) {
     size_t fsz[6] = {jk, ik, ij, jk, ik, ij};
     for (int f = 0; f < 6; f++) for (int q = 0; q < 3; q++) if ((r = dalloc(hb, &b.surf[f][q], fsz[f]))) return r;
     b.flags = (int*)carve(hb, 16 * sizeof(int));
-    if ((r = dalloc(hb, &hb.force_acc, 12))) return r;
     if (c->p.NV == 6) {
         size_t ncell = (size_t)(hb.nx - 1) * (hb.ny - 1) * (hb.nz - 1);
         double* m = nullptr;
@@ -268,6 +269,10 @@ int check_subs(ocfd_ctx* c, const HostBlock& hb) {
             if (B.face1 < 1 || B.face1 > 6) return fail(OCFD_EARG, "inner subface with linked face id outside 1..6");
             int La[3] = {std::abs(B.L1), std::abs(B.L2), std::abs(B.L3)};
             for (int q = 0; q < 3; q++) if (La[q] < 1 || La[q] > 3) return fail(OCFD_EARG, "inner subface with L outside +-1..3");
+        } else if ((B.bc == 5 || B.bc == 6) && c->hp.IF_InnerFlow == 1) {
+            // inner flow: boundary_BC_Inflow_Turbo / boundary_BC_Outflow_Turbo (sub_boundary.f90:28-40)
+            return fail(OCFD_EUNSUPPORTED, "bc " + std::to_string(B.bc) + " in the inner-flow mode (IF_InnerFlow = 1) is outside this path; block " +
+                                               std::to_string(hb.no));
         } else if (!(B.bc == 2 || B.bc == 3 || B.bc == 4 || B.bc == 5 || B.bc == 6 || B.bc == 401)) {
             return fail(OCFD_EUNSUPPORTED, "The boundary condition is not supported!!! Block_no is " + std::to_string(hb.no) +
                                                " bc=" + std::to_string(B.bc));
@@ -348,6 +353,7 @@ void drop_lu_plan(Engine& e) {
     for (void* q : e.lu_allocs) (void)hipFree(q);
     e.lu_allocs.clear();
     e.lu = LuPlan{};
+    e.set.halt = nullptr;
 }
 
 void drop_set_plan(Engine& e) {
@@ -480,7 +486,7 @@ int build_set_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) 
         P.ntiles[sp] = (int)tb.size();
         if ((r = upload_table(e.set_allocs, tb, &P.tile_blk[sp])) || (r = upload_table(e.set_allocs, t0, &P.tile0[sp]))) return r;
     }
-    std::vector<SubTask> bc, amut, kbc;
+    std::vector<SubTask> bc, amut, kbc, force;
     std::vector<HaloTask> halo;
     std::vector<int> halo_tile;
     for (int bi = 0; bi < P.nb; bi++) {
@@ -499,7 +505,15 @@ int build_set_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) 
                 else if (B.bc == 3 || B.bc == 2) t.kind = SUB_SYM;
                 else t.kind = SUB_EXT;
                 bc.push_back(t);
-                if (B.bc == 2) { SubTask a = t; a.kind = SUB_AMUT; amut.push_back(a); }
+                if (B.bc == 2) {
+                    SubTask a = t; a.kind = SUB_AMUT; amut.push_back(a);
+                    // comput_force's face loop: cells of the two tangential directions
+                    SubTask fo = t;
+                    const int d = (B.face - 1) % 3;
+                    fo.n0 = d == 0 ? B.je - B.jb : B.ie - B.ib;
+                    fo.n1 = d == 2 ? B.je - B.jb : B.ke - B.kb;
+                    if (fo.n0 > 0 && fo.n1 > 0) force.push_back(fo);
+                }
                 if (B.face == 3 || B.face == 6) {
                     SubTask k = t;
                     k.kind = SUB_KBC; k.n0 = B.ie - B.ib; k.n1 = std::max(1, B.je - B.jb);
@@ -515,7 +529,7 @@ int build_set_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) 
         }
     }
     if ((r = build_sub_list(e.set_allocs, bc, P.bc)) || (r = build_sub_list(e.set_allocs, amut, P.amut)) ||
-        (r = build_sub_list(e.set_allocs, kbc, P.kbc)))
+        (r = build_sub_list(e.set_allocs, kbc, P.kbc)) || (r = build_sub_list(e.set_allocs, force, P.force)))
         return r;
     P.halo.ntiles = (int)halo_tile.size();
     if ((r = upload_table(e.set_allocs, halo, &P.halo.t)) || (r = upload_table(e.set_allocs, halo_tile, &P.halo.tile_task))) return r;
@@ -526,7 +540,11 @@ int build_set_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) 
 int engine_prepare(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, bool lu) {
     int r = build_set_plan(e, refs, p0);
     if (r) return r;
-    if (lu && e.lu_schedule != OCFD_LU_HYPERPLANE) return build_lu_plan(e, refs, p0);
+    if (lu && e.lu_schedule != OCFD_LU_HYPERPLANE) {
+        if ((r = build_lu_plan(e, refs, p0))) return r;
+        // once a sweep gives up, the update kernels of that and later iterations leave U as it is
+        e.set.halt = e.lu.err;
+    }
     return OCFD_OK;
 }
 
@@ -626,9 +644,16 @@ int ready(ocfd_ctx* c) {
     return build_set_plan(c->e, c->refs(), c->p);
 }
 
-// ready() plus the LU-SGS tables
+// ready() for the entry points that advance or change the state: a context in a batch is stepped
+// through the batch only
+int ready_solo(ocfd_ctx* c) {
+    if (c && c->batch) return fail(OCFD_EARG, "the context is in a batch and is stepped through it (ocfd_batch_remove first)");
+    return ready(c);
+}
+
+// ready_solo() plus the LU-SGS tables
 int prepare(ocfd_ctx* c) {
-    int r = ready(c);
+    int r = ready_solo(c);
     if (r) return r;
     return engine_prepare(c->e, c->refs(), c->p, true);
 }
@@ -636,6 +661,50 @@ int prepare(ocfd_ctx* c) {
 int finish(ocfd_ctx* c) { return engine_finish(c->e); }
 
 void enqueue_stage(ocfd_ctx* c, int stage) { enqueue_stage(c->e, c->refs(), c->p, stage); }
+
+// device buffer of the residual norms: the block table, the parts, the result (3 NV doubles)
+struct NormLayout { size_t tab, part, out, bytes; };
+NormLayout norm_layout(int nb, int NV) {
+    NormLayout L;
+    L.tab = 0;
+    L.part = ((size_t)nb * sizeof(NormBlk) + 255) & ~(size_t)255;
+    L.out = L.part + (((size_t)nb * NV * norms_parts() * 3 * sizeof(double) + 255) & ~(size_t)255);
+    L.bytes = L.out + 3 * MAXV * sizeof(double);
+    return L;
+}
+
+void fill_norm_table(const ocfd_ctx* c, NormBlk* tab) {
+    for (size_t m = 0; m < c->blocks.size(); m++) {
+        const Blk& b = c->blocks[m].d;
+        NormBlk t{};
+        for (int v = 0; v < MAXV; v++) t.res[v] = v < c->p.NV ? b.Res[v] : nullptr;
+        t.nx = b.nx; t.ny = b.ny; t.nz = b.nz; t.NI = b.NI; t.NJ = b.NJ;
+        tab[m] = t;
+    }
+}
+
+// the norms' launches on the context's stream over the buffer `dev` (table already in place),
+// then Res_max and Res_rms = sqrt(sum / Num_Cell) on the host (comput_max_Res_onemesh,
+// sub_time_advance.f90:330-380)
+int run_norms(ocfd_ctx* c, char* dev, const NormLayout& L, double* res_max, double* res_rms) {
+    const int NV = c->p.NV, nb = (int)c->blocks.size();
+    launch_norms(reinterpret_cast<const NormBlk*>(dev + L.tab), nb, NV, reinterpret_cast<double*>(dev + L.part),
+                 reinterpret_cast<double*>(dev + L.out), c->e.st);
+    HIPCHK(hipGetLastError());
+    double o[3 * MAXV];
+    HIPCHK(hipMemcpyAsync(o, dev + L.out, 3 * (size_t)NV * sizeof(double), hipMemcpyDeviceToHost, c->e.st));
+    HIPCHK(hipStreamSynchronize(c->e.st));
+    long ncell = 0;
+    for (const HostBlock& hb : c->blocks) ncell += (long)(hb.nx - 1) * (hb.ny - 1) * (hb.nz - 1);
+    bool nan = false;
+    for (int v = 0; v < NV; v++) {
+        res_max[v] = o[v];
+        res_rms[v] = std::sqrt(o[NV + v] / (double)ncell);
+        nan = nan || o[2 * NV + v] != 0.0;
+    }
+    if (nan) return fail(OCFD_ENAN, "NaN in Residual is found !");
+    return OCFD_OK;
+}
 void enqueue_residual(ocfd_ctx* c) {
     for (int s = OCFD_STAGE_PRE; s <= OCFD_STAGE_LU_BACKWARD; s++) enqueue_stage(c, s);
 }
@@ -644,6 +713,34 @@ void enqueue_halo(ocfd_ctx* c) { set_halo(c->e.set, c->e.st); }
 void enqueue_force_vt_kw(ocfd_ctx* c) { set_force_vt_kw(c->e.set, c->e.st); }
 
 }  // namespace
+
+namespace ocfd {
+
+// The residual norms of a context computed in a caller-provided device buffer of cap bytes (the
+// block table is copied in from the host stack, so the buffer may hold anything before): no
+// allocation of any kind on the device.
+int norms_into(ocfd_ctx* c, char* dev, size_t cap, double* res_max, double* res_rms) {
+    if (!c) return fail(OCFD_EARG, "null context");
+    if (!res_max || !res_rms) return fail(OCFD_EARG, "null output");
+    const int nb = (int)c->blocks.size();
+    if (nb < 1) return fail(OCFD_EARG, "no blocks added");
+    for (int q = 0; q < nb; q++)
+        if (c->blocks[q].no != q + 1) return fail(OCFD_EARG, "block " + std::to_string(q + 1) + " missing");
+    if (nb > NORMS_INTO_MAX_BLOCKS)
+        return fail(OCFD_EARG, "at most " + std::to_string(NORMS_INTO_MAX_BLOCKS) + " blocks; this context has " + std::to_string(nb));
+    const NormLayout L = norm_layout(nb, c->p.NV);
+    if (L.bytes > cap) return fail(OCFD_EARG, "the residual norms of this context exceed the buffer");
+    HIPCHK(hipSetDevice(c->device));
+    NormBlk tab[NORMS_INTO_MAX_BLOCKS];
+    fill_norm_table(c, tab);
+    HIPCHK(hipMemcpyAsync(dev + L.tab, tab, (size_t)nb * sizeof(NormBlk), hipMemcpyHostToDevice, c->e.st));
+    return run_norms(c, dev, L, res_max, res_rms);
+}
+
+int ctx_device(const ocfd_ctx* c) { return c ? c->device : -1; }
+int set_error(int code, const char* msg) { return fail(code, msg); }
+
+}  // namespace ocfd
 
 // ================================ C ABI ========================================================
 extern "C" {
@@ -658,15 +755,19 @@ int ocfd_create(const ocfd_params* p, ocfd_ctx** out) {
     if (p->Iflag_Scheme == 9 || p->Bound_Scheme == 9) return fail(OCFD_EUNSUPPORTED, "WENO7 reconstruction is not on this path");
     if (p->Iflag_Flux < 1 || p->Iflag_Flux > 6) return fail(OCFD_EARG, "Iflag_Flux outside 1..6");
     if (p->precision != 0 && p->precision != 1) return fail(OCFD_EARG, "precision must be 0 (all FP64) or 1 (FP32 reconstruction, fluxes and SA terms)");
+    if (p->IF_TurboMachinary != 0) return fail(OCFD_EUNSUPPORTED, "the turbomachinery mode (IF_TurboMachinary = 1) is outside this path");
+    if (p->IF_InnerFlow != 0 && p->IF_InnerFlow != 1) return fail(OCFD_EARG, "IF_InnerFlow must be 0 or 1");
     if (p->precision == 1 && p->If_viscous == 1 && p->Iflag_turbulence_model == 3)
         return fail(OCFD_EUNSUPPORTED, "precision 1 covers the laminar and SA paths; SST runs in FP64 (precision 0)");
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (ndev < 1) return fail(OCFD_EHIP, "no HIP device");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
     ocfd_ctx* c = new ocfd_ctx();
     c->hp = *p;
     c->p = to_par(*p);
-    HIPCHK(hipGetDevice(&c->device));
+    c->device = dev;
     HIPCHK(hipStreamCreateWithFlags(&c->e.st, hipStreamNonBlocking));
     *out = c;
     return OCFD_OK;
@@ -674,9 +775,20 @@ int ocfd_create(const ocfd_params* p, ocfd_ctx** out) {
 
 int ocfd_destroy(ocfd_ctx* c) {
     if (!c) return OCFD_OK;
+    // a context destroyed while in a batch leaves it first: the batch's tables and graphs hold
+    // its block arrays
+    if (c->batch) {
+        ocfd_batch* b = c->batch;
+        (void)hipStreamSynchronize(b->e.st);
+        for (size_t q = 0; q < b->members.size(); q++)
+            if (b->members[q] == c) { b->members.erase(b->members.begin() + q); break; }
+        drop_plans(b->e);
+        c->batch = nullptr;
+    }
     engine_destroy(c->e);
     for (HostBlock& hb : c->blocks) for (void* q : hb.allocs) (void)hipFree(q);
-    if (c->norm_part) (void)hipFree(c->norm_part);
+    if (c->norm_buf) (void)hipFree(c->norm_buf);
+    if (c->force_buf) (void)hipFree(c->force_buf);
     delete c;
     return OCFD_OK;
 }
@@ -693,6 +805,7 @@ static int add_block_common(ocfd_ctx* c, int32_t no, int32_t nx, int32_t ny, int
     if (r) return r;
     c->bytes += hb.bytes;
     drop_plans(c->e);
+    if (c->norm_buf) { (void)hipFree(c->norm_buf); c->norm_buf = nullptr; c->norm_bytes = 0; }
     *out = &hb;
     return OCFD_OK;
 }
@@ -952,6 +1065,9 @@ int ocfd_download_field(ocfd_ctx* c, int32_t no, int32_t field, double* out) {
         case OCFD_F_DT: src = b.dt; break;
         case OCFD_F_MU: src = b.muf; break;
         case OCFD_F_MUT: src = b.mut; break;
+        case OCFD_F_DW:
+            if (!b.dw) return fail(OCFD_EARG, "no wall distance without a turbulence model");
+            src = b.dw; break;
         default: return fail(OCFD_EARG, "unknown field id");
         }
         if ((r = get_box(*hb, src, h))) return r;
@@ -963,8 +1079,10 @@ int ocfd_download_field(ocfd_ctx* c, int32_t no, int32_t field, double* out) {
 }
 
 int ocfd_init_flow(ocfd_ctx* c) {
-    int r = ready(c);
+    int r = ready_solo(c);
     if (r) return r;
+    // init_flow_zero's external-flow branch; the inner-flow initial field is outside this path
+    if (c->hp.IF_InnerFlow == 1) return fail(OCFD_EUNSUPPORTED, "ocfd_init_flow: the inner-flow initial field (IF_InnerFlow = 1) is outside this path");
     double g = c->p.gamma, Ma = c->p.Ma;
     double d0 = 1.0, p0 = 1.0 / (g * Ma * Ma);
     double u0 = std::cos(c->p.A_alfa) * std::cos(c->p.A_beta), v0 = std::sin(c->p.A_alfa) * std::cos(c->p.A_beta), w0 = std::sin(c->p.A_beta);
@@ -1004,60 +1122,52 @@ int ocfd_step_timed(ocfd_ctx* c, int32_t niter, double* stage_ms) {
 }
 
 int ocfd_residual(ocfd_ctx* c) { int r = prepare(c); if (r) return r; enqueue_residual(c); return finish(c); }
-int ocfd_update_limit(ocfd_ctx* c) { int r = ready(c); if (r) return r; enqueue_stage(c, OCFD_STAGE_UPDATE); return finish(c); }
+int ocfd_update_limit(ocfd_ctx* c) { int r = ready_solo(c); if (r) return r; enqueue_stage(c, OCFD_STAGE_UPDATE); return finish(c); }
 int ocfd_bc(ocfd_ctx* c) {
-    int r = ready(c); if (r) return r;
+    int r = ready_solo(c); if (r) return r;
     for (HostBlock& hb : c->blocks) if ((r = check_subs(c, hb))) return r;
     enqueue_bc(c); return finish(c);
 }
-int ocfd_halo(ocfd_ctx* c) { int r = ready(c); if (r) return r; enqueue_halo(c); return finish(c); }
-int ocfd_force_vt_kw(ocfd_ctx* c) { int r = ready(c); if (r) return r; enqueue_force_vt_kw(c); return finish(c); }
+int ocfd_halo(ocfd_ctx* c) { int r = ready_solo(c); if (r) return r; enqueue_halo(c); return finish(c); }
+int ocfd_force_vt_kw(ocfd_ctx* c) { int r = ready_solo(c); if (r) return r; enqueue_force_vt_kw(c); return finish(c); }
 
 int ocfd_res_norms(ocfd_ctx* c, double* res_max, double* res_rms) {
     int r = ready(c);
     if (r) return r;
     if (!res_max || !res_rms) return fail(OCFD_EARG, "null output");
-    int NV = c->p.NV, P = norms_parts();
-    size_t need = (size_t)c->blocks.size() * NV * P * 3;
-    if (c->norm_part_n < need) {
-        if (c->norm_part) HIPCHK(hipFree(c->norm_part));
-        HIPCHK(hipMalloc(&c->norm_part, need * sizeof(double)));
-        c->norm_part_n = need;
+    const int nb = (int)c->blocks.size();
+    const NormLayout L = norm_layout(nb, c->p.NV);
+    if (c->norm_bytes < L.bytes) {
+        if (c->norm_buf) HIPCHK(hipFree(c->norm_buf));
+        c->norm_buf = nullptr; c->norm_bytes = 0;
+        HIPCHK(hipMalloc((void**)&c->norm_buf, L.bytes));
+        c->norm_bytes = L.bytes;
+        std::vector<NormBlk> tab((size_t)nb);
+        fill_norm_table(c, tab.data());
+        HIPCHK(hipMemcpyAsync(c->norm_buf + L.tab, tab.data(), (size_t)nb * sizeof(NormBlk), hipMemcpyHostToDevice, c->e.st));
     }
-    set_norms(c->e.set, c->norm_part, c->e.st);
-    std::vector<double> h(need);
-    HIPCHK(hipMemcpyAsync(h.data(), c->norm_part, need * sizeof(double), hipMemcpyDeviceToHost, c->e.st));
-    HIPCHK(hipStreamSynchronize(c->e.st));
-    double mx[MAXV] = {0}, sm[MAXV] = {0};
-    bool nan = false;
-    long ncell = 0;
-    for (size_t m = 0; m < c->blocks.size(); m++) {
-        const HostBlock& hb = c->blocks[m];
-        ncell += (long)(hb.nx - 1) * (hb.ny - 1) * (hb.nz - 1);
-        for (int v = 0; v < NV; v++) for (int q = 0; q < P; q++) {
-            const double* o = h.data() + 3 * ((m * NV + v) * P + q);
-            mx[v] = std::max(mx[v], o[0]); sm[v] = sm[v] + o[1]; nan = nan || o[2] != 0.0;
-        }
-    }
-    for (int v = 0; v < NV; v++) { res_max[v] = mx[v]; res_rms[v] = std::sqrt(sm[v] / (double)ncell); }
-    if (nan) return fail(OCFD_ENAN, "NaN in Residual is found !");
-    return OCFD_OK;
+    return run_norms(c, c->norm_buf, L, res_max, res_rms);
 }
 
 int ocfd_forces(ocfd_ctx* c, double* out) {
     int r = ready(c);
     if (r) return r;
     if (!out) return fail(OCFD_EARG, "null output");
-    std::vector<double> tot(9, 0.0);
-    for (HostBlock& hb : c->blocks) {
-        double init[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, c->hp.Centroid[0], c->hp.Centroid[1], c->hp.Centroid[2]};
-        HIPCHK(hipMemcpyAsync(hb.force_acc, init, sizeof(init), hipMemcpyHostToDevice, c->e.st));
-        for (const Sub& B : hb.subs) if (B.bc == 2) launch_force_sub(hb.d, c->p, B, hb.force_acc, c->e.st);
-        double acc[12];
-        HIPCHK(hipMemcpyAsync(acc, hb.force_acc, sizeof(acc), hipMemcpyDeviceToHost, c->e.st));
-        HIPCHK(hipStreamSynchronize(c->e.st));
-        for (int q = 0; q < 9; q++) tot[q] = tot[q] + acc[q];
+    const SetPlan& P = c->e.set;
+    const size_t need = ((size_t)P.force.ntiles + 1) * 9;
+    if (c->force_n < need) {
+        if (c->force_buf) HIPCHK(hipFree(c->force_buf));
+        c->force_buf = nullptr; c->force_n = 0;
+        HIPCHK(hipMalloc((void**)&c->force_buf, need * sizeof(double)));
+        c->force_n = need;
     }
+    const double cen[3] = {c->hp.Centroid[0], c->hp.Centroid[1], c->hp.Centroid[2]};
+    double* out9 = c->force_buf + (size_t)P.force.ntiles * 9;
+    launch_forces(P, cen, c->force_buf, out9, c->e.st);
+    HIPCHK(hipGetLastError());
+    std::vector<double> tot(9, 0.0);
+    HIPCHK(hipMemcpyAsync(tot.data(), out9, 9 * sizeof(double), hipMemcpyDeviceToHost, c->e.st));
+    HIPCHK(hipStreamSynchronize(c->e.st));
     for (int q = 0; q < 6; q++) tot[q] = 2.0 * tot[q] / c->hp.Ref_S;
     for (int q = 6; q < 9; q++) tot[q] = 2.0 * tot[q] / (c->hp.Ref_S * c->hp.Ref_L);
     double Fx = tot[0] + tot[3], Fy = tot[1] + tot[4], Fz = tot[2] + tot[5], AoA = c->hp.AoA;
@@ -1155,6 +1265,7 @@ int ocfd_batch_create(ocfd_batch** out) {
 int ocfd_batch_destroy(ocfd_batch* b) {
     if (!b) return OCFD_OK;
     engine_destroy(b->e);
+    for (ocfd_ctx* c : b->members) c->batch = nullptr;     // the members may be stepped alone again
     delete b;
     return OCFD_OK;
 }
@@ -1164,11 +1275,13 @@ int ocfd_batch_add(ocfd_batch* b, ocfd_ctx* c) {
     int r = ready(c);
     if (r) return r;
     for (HostBlock& hb : c->blocks) if ((r = check_subs(c, hb))) return r;
-    for (ocfd_ctx* m : b->members) if (m == c) return fail(OCFD_EARG, "context already in the batch");
+    if (c->batch == b) return fail(OCFD_EARG, "context already in the batch");
+    if (c->batch) return fail(OCFD_EARG, "context already in another batch");
     if (!b->members.empty() && !same_switches(b->members[0]->p, c->p))
         return fail(OCFD_EARG, "a batch needs the same NVAR, schemes, flux, turbulence model, limiter, gamma, w_LU and precision in every member");
     HIPCHK(hipStreamSynchronize(c->e.st));
     b->members.push_back(c);
+    c->batch = b;
     drop_plans(b->e);
     // the member steps through the batch's launches now: its own tables (the sweep storage is
     // about 380 B per cell) are rebuilt only if it is stepped alone again
@@ -1182,6 +1295,7 @@ int ocfd_batch_remove(ocfd_batch* b, ocfd_ctx* c) {
         if (b->members[q] != c) continue;
         HIPCHK(hipStreamSynchronize(b->e.st));
         b->members.erase(b->members.begin() + q);
+        c->batch = nullptr;
         drop_plans(b->e);
         return OCFD_OK;
     }

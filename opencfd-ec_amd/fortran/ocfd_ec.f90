@@ -17,7 +17,7 @@ module ocfd_ec
   implicit none
 
   integer(c_int), parameter :: OCFD_OK = 0, OCFD_EHIP = 1, OCFD_EARG = 2, OCFD_ENAN = 3, OCFD_EUNSUPPORTED = 4
-  integer(c_int), parameter :: OCFD_F_RES = 1, OCFD_F_DU = 2, OCFD_F_DT = 3, OCFD_F_MU = 4, OCFD_F_MUT = 5
+  integer(c_int), parameter :: OCFD_F_RES = 1, OCFD_F_DU = 2, OCFD_F_DT = 3, OCFD_F_MU = 4, OCFD_F_MUT = 5, OCFD_F_DW = 6
   integer(c_int), parameter :: OCFD_NSTAGES = 7
 
   type, bind(C) :: ocfd_params
@@ -32,6 +32,7 @@ module ocfd_ec
     integer(c_int32_t) :: IFlag_Reconstruction, Bound_Scheme, Iflag_local_dt, If_dtime_mesh
     integer(c_int32_t) :: IF_Scheme_Positivity, IFLAG_LIMIT_FLOW, Cood_Y_UP
     integer(c_int32_t) :: precision
+    integer(c_int32_t) :: IF_InnerFlow, IF_TurboMachinary
   end type ocfd_params
 
   ! met(q, s) holds the C member met[s][q]
@@ -251,6 +252,7 @@ contains
     p%Bound_Scheme = Bound_Scheme; p%Iflag_local_dt = Iflag_local_dt; p%If_dtime_mesh = If_dtime_mesh
     p%IF_Scheme_Positivity = IF_Scheme_Positivity; p%IFLAG_LIMIT_FLOW = IFLAG_LIMIT_FLOW
     p%Cood_Y_UP = Cood_Y_UP; p%precision = 0
+    p%IF_InnerFlow = IF_InnerFlow; p%IF_TurboMachinary = IF_TurboMachinary
     call ocfd_check(ocfd_create(p, ctx), 'ocfd_create')
 
     do m = 1, Mesh(1)%Num_Block
@@ -321,4 +323,27 @@ contains
     type(c_ptr), intent(in) :: ctx
     call ocfd_check(ocfd_res_norms(ctx, Mesh(1)%Res_max, Mesh(1)%Res_rms), 'ocfd_res_norms')
   end subroutine ocfd_host_norms
+
+  ! comput_force on the device (ocfd_forces): the console lines and the force.log and
+  ! force-invis-vis.log records comput_force writes (sub_Post.f90:224-246), with no download of
+  ! the state; only 12 doubles cross the bus
+  subroutine ocfd_host_forces(ctx)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double) :: f(12)
+    integer :: u
+    call ocfd_check(ocfd_forces(ctx, f), 'ocfd_forces')
+    if (my_id == 0) then
+      print*, "--------------Force and Moment -----------------------------"
+      print*, "Total Force Coefficient (CL, CD, CS) ="
+      print*, f(10), f(11), f(12)
+      print*, "Total Moment Coefficient(CMx, CMy, CMz)="
+      print*, f(7), f(8), f(9)
+      open(newunit=u, file="force.log", position="append")
+      write(u, "(I7,6E20.8)") Mesh(1)%Kstep, f(10), f(11), f(12), f(7), f(8), f(9)
+      close(u)
+      open(newunit=u, file="force-invis-vis.log", position="append")
+      write(u, "(I7,6E20.8)") Mesh(1)%Kstep, f(1), f(2), f(3), f(4), f(5), f(6)
+      close(u)
+    end if
+  end subroutine ocfd_host_forces
 end module ocfd_ec_host

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libocfd_ec.so")
 LAP = 4
 
 OCFD_OK, OCFD_EHIP, OCFD_EARG, OCFD_ENAN, OCFD_EUNSUPPORTED = 0, 1, 2, 3, 4
-F_RES, F_DU, F_DT, F_MU, F_MUT = 1, 2, 3, 4, 5
+F_RES, F_DU, F_DT, F_MU, F_MUT, F_DW = 1, 2, 3, 4, 5, 6
 
 
 class OcfdError(RuntimeError):
@@ -40,7 +40,7 @@ class Params(C.Structure):
         ("Centroid", C.c_double * 3), ("Periodic_d", C.c_double * 3)] + [(n, C.c_int32) for n in (
         "NVAR", "If_viscous", "Iflag_turbulence_model", "Iflag_Scheme", "Iflag_Flux",
         "IFlag_Reconstruction", "Bound_Scheme", "Iflag_local_dt", "If_dtime_mesh",
-        "IF_Scheme_Positivity", "IFLAG_LIMIT_FLOW", "Cood_Y_UP", "precision")]
+        "IF_Scheme_Positivity", "IFLAG_LIMIT_FLOW", "Cood_Y_UP", "precision", "IF_InnerFlow", "IF_TurboMachinary")]
 
 
 class BlockGeom(C.Structure):
@@ -163,6 +163,8 @@ def const_params(control: Dict[str, object], precision: int = 0) -> Params:
     P.IF_Scheme_Positivity, P.IFLAG_LIMIT_FLOW = int(control["IF_Scheme_Positivity"]), int(control["IFLAG_LIMIT_FLOW"])
     P.Cood_Y_UP = int(control["Cood_Y_UP"])
     P.precision = precision
+    P.IF_InnerFlow = int(control.get("IF_InnerFlow", 0))
+    P.IF_TurboMachinary = int(control.get("IF_TurboMachinary", 0))
     return P
 
 
@@ -241,6 +243,11 @@ class Solver:
 
     def close(self):
         if getattr(self, "_h", None):
+            # ocfd_destroy takes a member out of its batch first
+            bt = getattr(self, "_batch", None)
+            if bt is not None and self in bt.members:
+                bt.members.remove(self)
+            self._batch = None
             lib().ocfd_destroy(self._h)
             self._h = None
 
@@ -300,6 +307,14 @@ class Solver:
         a = np.zeros(self.ushape(no), order="F")
         _check(lib().ocfd_download_U(self._h, no, _dp(a)))
         return a
+
+    def wall_distance(self) -> List[np.ndarray]:
+        """dw(1:nx-1, 1:ny-1, 1:nz-1) of every block (numpy [i-1, j-1, k-1]), as computed on the device."""
+        out = []
+        for no in sorted(self.dims):
+            nx, ny, nz = self.dims[no]
+            out.append(np.asfortranarray(self.field(no, F_DW)[2:nx + 1, 2:ny + 1, 2:nz + 1]))
+        return out
 
     def field(self, no: int, fid: int) -> np.ndarray:
         nx, ny, nz = self.dims[no]
@@ -388,6 +403,9 @@ class Batch:
         if getattr(self, "_h", None):
             lib().ocfd_batch_destroy(self._h)
             self._h = None
+            for s in self.members:
+                s._batch = None
+            self.members = []
 
     def __del__(self):
         try:
@@ -398,10 +416,12 @@ class Batch:
     def add(self, s: "Solver"):
         _check(lib().ocfd_batch_add(self._h, s._h))
         self.members.append(s)
+        s._batch = self
 
     def remove(self, s: "Solver"):
         _check(lib().ocfd_batch_remove(self._h, s._h))
         self.members.remove(s)
+        s._batch = None
 
     def step(self, n: int = 1):
         _check(lib().ocfd_batch_step(self._h, n))

@@ -217,12 +217,13 @@ def sub_range(dims, face, lo_hi: Dict[int, Tuple[int, int]]):
 # C1: channel with a circular-arc bump (97x33x9 nodes by default)
 
 def channel_bump(ni: int = 97, nj: int = 33, nk: int = 9, Lx: float = 3.0, Lz: float = 0.3,
-                 bump: float = 0.1, **params) -> Case:
+                 bump: float = 0.1, inlet_bc: int = BC_FARFIELD, outlet_bc: int = BC_FARFIELD, **params) -> Case:
     """Single-block 3D inviscid channel, 10%-chord circular-arc bump on x in [1,2] of the lower wall.
 
     Lower and upper walls are slip walls (bc=2 with If_viscous=0), z is periodic through a
     self-link (bc -2/-3) with Periodic_dZ=Lz, inflow/outflow are far-field (bc=4) faces driven by
-    P_outlet = 1/(gamma Ma^2).
+    P_outlet = 1/(gamma Ma^2); ``inlet_bc``/``outlet_bc`` select another condition for the i=1 and
+    i=ni faces (5 inflow, 6 outflow, 401 extrapolation).
     """
     R = (0.25 + bump * bump) / (2 * bump)          # circle through (1,0),(2,0),(1.5,bump)
     x1 = np.linspace(0.0, Lx, ni)
@@ -238,8 +239,8 @@ def channel_bump(ni: int = 97, nj: int = 33, nk: int = 9, Lx: float = 3.0, Lz: f
     z[:] = zz[None, None, :]
     dims = (ni, nj, nk)
     subs = [
-        Subface(full_face(dims, 1), BC_FARFIELD),
-        Subface(full_face(dims, 4), BC_FARFIELD),
+        Subface(full_face(dims, 1), inlet_bc),
+        Subface(full_face(dims, 4), outlet_bc),
         Subface(full_face(dims, 2), BC_WALL),
         Subface(full_face(dims, 5), BC_WALL),
         Subface(full_face(dims, 3), BC_PERIODIC_L, full_face(dims, 6) + (1,)),

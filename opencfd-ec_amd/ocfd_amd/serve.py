@@ -18,12 +18,14 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import queue
+import threading
 import time
 from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-from . import Batch, Solver, solver_from_case
+from . import OCFD_ENAN, Batch, OcfdError, Solver, solver_from_case
 from . import cases as C
 
 
@@ -47,6 +49,7 @@ class Result:
     CL: float
     CD: float
     cells: int
+    failed: bool = False           # the residual held a NaN (OCFD_ENAN): the solve diverged
 
     @property
     def latency(self) -> float:
@@ -82,17 +85,24 @@ def sweep_requests(n: int = 256, lam: float = 2.0, seed: int = 5, dims=DIMS, max
 class Server:
     """Continuous batching of steady solves on one GPU.
 
-    mem_budget: bytes of device memory the admitted requests may hold together.
+    mem_budget: bytes of device memory the admitted and prepared requests may hold together.
     round_iters: iterations per batched step between admission/convergence checks.
+    A preparer thread builds each arrived request (grid, device geometry and wall distance, its
+    first iteration on its own stream) while the batch runs; the serving loop only admits the
+    prepared requests into the batch between rounds.  make_solver / make_batch build the solver of
+    a case and the batch (the library's by default).
     """
 
     def __init__(self, mem_budget: float = 200e9, round_iters: int = 20, drop: float = 5e-3,
-                 max_iters: int = 6000, graph: bool = True):
+                 max_iters: int = 6000, graph: bool = True, make_solver: Callable = solver_from_case,
+                 make_batch: Callable = Batch):
         self.mem_budget = mem_budget
         self.round_iters = round_iters
         self.drop = drop
         self.max_iters = max_iters
         self.graph = graph
+        self.make_solver = make_solver
+        self.make_batch = make_batch
 
     @staticmethod
     def lu_bytes(case) -> int:
@@ -107,61 +117,151 @@ class Server:
             steps += max(tiles, -(-(ny - 1) // 16) * -(-(nz - 1) // 4)) * (nx - 1 + 18)
         return int(steps * 64 * (4 * nv + 21) * 8 + 16e6)
 
+    @staticmethod
+    def rms1(s) -> float:
+        """Res_rms(1) of a solver's last iteration; NaN when its residual holds a NaN (OCFD_ENAN)."""
+        try:
+            return float(s.norms()[1][0])
+        except OcfdError as e:
+            if e.code == OCFD_ENAN:
+                return math.nan
+            raise
+
+    def _result(self, a, tnow, ratio, conv, failed):
+        if failed:
+            cl = cd = math.nan
+        else:
+            f = a["s"].forces()
+            cl, cd = float(f[9]), float(f[10])
+        return Result(a["req"].rid, a["req"].arrival, a["admitted"], tnow, a["iters"], conv, ratio, cl, cd,
+                      a["req"].cells, failed)
+
+    def _preparer(self, pending, ready, st, t0, clock):
+        """Thread body: for each request in arrival order, wait for its arrival, build its solver
+        and run its first iteration once the memory budget has room, then hand it over."""
+        try:
+            for r in pending:
+                while clock() - t0 < r.arrival:
+                    if st["stop"].is_set():
+                        return
+                    time.sleep(min(0.02, max(0.0, r.arrival - (clock() - t0))))
+                s = self.make_solver(r.case)
+                need = s.mem_bytes() + self.lu_bytes(r.case)
+                while True:
+                    with st["lock"]:
+                        held = st["used"] + st["reserved"]
+                        if held == 0 or held + need <= self.mem_budget:
+                            st["reserved"] += need
+                            break
+                    if st["stop"].is_set():
+                        s.close()
+                        return
+                    time.sleep(0.005)
+                s.set_graph(self.graph)
+                s.step(1)
+                ready.put((r, s, need, self.rms1(s), clock() - t0))
+        except BaseException as e:          # handed to the serving loop, which raises it
+            ready.put(e)
+        finally:
+            ready.put(None)
+
     def run(self, requests: List[Request], clock: Callable[[], float] = time.monotonic,
             log: Optional[Callable[[str], None]] = None) -> List[Result]:
         pending = sorted(requests, key=lambda r: r.arrival)
         active: Dict[int, dict] = {}
         results: List[Result] = []
-        batch = Batch()
-        batch.set_graph(self.graph)
-        used = 0
+        ready: "queue.Queue" = queue.Queue()
+        st = dict(lock=threading.Lock(), used=0, reserved=0, stop=threading.Event())
         self.peak_active, self.peak_used = 0, 0
         t0 = clock()
-        while pending or active:
-            now = clock() - t0
-            # admit what has arrived and fits
-            while pending and pending[0].arrival <= now:
-                r = pending[0]
-                s = solver_from_case(r.case)
-                need = s.mem_bytes() + self.lu_bytes(r.case)
-                if active and used + need > self.mem_budget:
+        batch = self.make_batch()
+        prep = threading.Thread(target=self._preparer, args=(pending, ready, st, t0, clock), daemon=True)
+        waiting = []                         # prepared solvers not yet admitted (closed on exit)
+        try:
+            batch.set_graph(self.graph)
+            prep.start()
+            preparing = True
+
+            def admit(item):
+                r, s, need, rms0, tprep = item
+                with st["lock"]:
+                    st["reserved"] -= need
+                    st["used"] += need
+                    used = st["used"]
+                a = dict(req=r, s=s, rms0=rms0, iters=1, admitted=clock() - t0, need=need)
+                if not math.isfinite(rms0):          # diverged in its first iteration
+                    results.append(self._result(a, clock() - t0, math.nan, False, True))
                     s.close()
-                    break
-                pending.pop(0)
-                s.step(1)
-                rms0 = float(s.norms()[1][0])
+                    with st["lock"]:
+                        st["used"] -= need
+                    return
                 batch.add(s)
-                used += need
-                active[r.rid] = dict(req=r, s=s, rms0=rms0, iters=1, admitted=clock() - t0, need=need)
+                active[r.rid] = a
                 self.peak_active = max(self.peak_active, len(active))
                 self.peak_used = max(self.peak_used, used)
                 if log:
-                    log(f"t={now:7.2f}s admit {r.rid} cells={r.cells} active={len(active)} mem={used / 1e9:.1f}GB")
-            if not active:
-                if pending:
-                    wait = pending[0].arrival - (clock() - t0)
-                    if wait > 0:
-                        time.sleep(min(wait, 0.05))
-                continue
-            batch.step(self.round_iters)
-            tnow = clock() - t0
-            for rid in list(active):
-                a = active[rid]
-                a["iters"] += self.round_iters
-                rms = float(a["s"].norms()[1][0])
-                ratio = rms / a["rms0"] if a["rms0"] > 0 else 0.0
-                conv = ratio <= self.drop
-                if conv or a["iters"] >= self.max_iters or not math.isfinite(ratio):
-                    f = a["s"].forces()
-                    results.append(Result(rid, a["req"].arrival, a["admitted"], tnow, a["iters"], conv, ratio,
-                                          float(f[9]), float(f[10]), a["req"].cells))
+                    log(f"t={a['admitted']:7.2f}s admit {r.rid} cells={r.cells} active={len(active)} mem={used / 1e9:.1f}GB")
+
+            while preparing or active:
+                # admit every prepared request; with nothing active, wait for the next one
+                block = not active
+                while True:
+                    try:
+                        item = ready.get(timeout=0.05) if block else ready.get_nowait()
+                    except queue.Empty:
+                        break
+                    if item is None:
+                        preparing = False
+                        break
+                    if isinstance(item, BaseException):
+                        raise item
+                    waiting.append(item[1])
+                    admit(item)
+                    waiting.remove(item[1])
+                    block = False
+                if not active:
+                    continue
+                batch.step(self.round_iters)
+                tnow = clock() - t0
+                for rid in list(active):
+                    a = active[rid]
+                    a["iters"] += self.round_iters
+                    rms = self.rms1(a["s"])
+                    failed = not math.isfinite(rms)
+                    ratio = rms / a["rms0"] if a["rms0"] > 0 else 0.0
+                    conv = (not failed) and ratio <= self.drop
+                    if conv or failed or a["iters"] >= self.max_iters:
+                        results.append(self._result(a, tnow, ratio, conv, failed))
+                        batch.remove(a["s"])
+                        a["s"].close()
+                        with st["lock"]:
+                            st["used"] -= a["need"]
+                        del active[rid]
+                        if log:
+                            log(f"t={tnow:7.2f}s done {rid} iters={a['iters']} drop={ratio:.2e} active={len(active)}"
+                                + (" FAILED (NaN)" if failed else ""))
+        finally:
+            st["stop"].set()
+            while prep.is_alive():
+                try:
+                    item = ready.get(timeout=0.05)
+                    if isinstance(item, tuple):
+                        item[1].close()
+                except queue.Empty:
+                    pass
+            while not ready.empty():
+                item = ready.get_nowait()
+                if isinstance(item, tuple):
+                    item[1].close()
+            for s in waiting:
+                s.close()
+            for a in active.values():
+                try:
                     batch.remove(a["s"])
-                    a["s"].close()
-                    used -= a["need"]
-                    del active[rid]
-                    if log:
-                        log(f"t={tnow:7.2f}s done {rid} iters={a['iters']} drop={ratio:.2e} active={len(active)}")
-        batch.close()
+                except OcfdError:
+                    pass
+                a["s"].close()
+            batch.close()
         return results
 
     def run_sequential(self, requests: List[Request], clock: Callable[[], float] = time.monotonic,
@@ -174,21 +274,25 @@ class Server:
             if r.arrival > now:
                 time.sleep(r.arrival - now)
             adm = clock() - t0
-            s = solver_from_case(r.case)
-            s.set_graph(self.graph)
-            s.step(1)
-            rms0 = float(s.norms()[1][0])
-            iters, ratio = 1, 1.0
-            while True:
-                s.step(self.round_iters)
-                iters += self.round_iters
-                ratio = float(s.norms()[1][0]) / rms0 if rms0 > 0 else 0.0
-                if ratio <= self.drop or iters >= self.max_iters or not math.isfinite(ratio):
-                    break
-            f = s.forces()
-            results.append(Result(r.rid, r.arrival, adm, clock() - t0, iters, ratio <= self.drop, ratio,
-                                  float(f[9]), float(f[10]), r.cells))
-            s.close()
+            s = self.make_solver(r.case)
+            try:
+                s.set_graph(self.graph)
+                s.step(1)
+                rms0 = self.rms1(s)
+                iters, ratio, failed = 1, 1.0, not math.isfinite(rms0)
+                while not failed:
+                    s.step(self.round_iters)
+                    iters += self.round_iters
+                    rms = self.rms1(s)
+                    failed = not math.isfinite(rms)
+                    ratio = rms / rms0 if rms0 > 0 else 0.0
+                    if ratio <= self.drop or iters >= self.max_iters:
+                        break
+                a = dict(req=r, s=s, iters=iters, admitted=adm)
+                conv = (not failed) and ratio <= self.drop
+                results.append(self._result(a, clock() - t0, math.nan if failed else ratio, conv, failed))
+            finally:
+                s.close()
             if log:
                 log(f"t={clock() - t0:7.2f}s sequential done {r.rid} cells={r.cells} iters={iters} drop={ratio:.2e}")
         return results
@@ -197,6 +301,7 @@ class Server:
 def summarize(results: List[Result], wall: float) -> Dict[str, float]:
     lat = np.array([r.latency for r in results]) if results else np.zeros(1)
     return dict(requests=len(results), converged=int(sum(r.converged for r in results)),
+                failed=int(sum(r.failed for r in results)),
                 p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
                 solves_per_hour=3600.0 * len(results) / wall if wall > 0 else 0.0,
                 mean_iters=float(np.mean([r.iterations for r in results])) if results else 0.0,

@@ -93,6 +93,7 @@ program dropin_driver
   gmax = Mesh(1)%Res_max(1:NVAR)
   grms = Mesh(1)%Res_rms(1:NVAR)
   call ocfd_check(ocfd_forces(ctx, f12), 'ocfd_forces')
+  call ocfd_host_forces(ctx)                  ! force.log line from the device (written first)
   do mb = 1, Mesh(1)%Num_Block
     call ocfd_check(ocfd_download_U(ctx, int(mb, c_int32_t), Ug(mb)%a), 'ocfd_download_U')
   end do

@@ -89,11 +89,14 @@ def read_forcelog(path: str):
 
 
 def run_case(case, nsteps: int = 1, kshow: int = 1, U_init=None, omp_threads: int = 0,
-             keep: Optional[str] = None, timeout: float = 1800.0, dump: bool = True) -> Dict[str, object]:
+             keep: Optional[str] = None, timeout: float = 1800.0, dump: bool = True,
+             wall_dist=None) -> Dict[str, object]:
     """Run the reference driver on ``case`` (an ocfd_amd.cases.Case) and return its dumps.
 
     With ``dump=False`` the driver writes no arrays; the result then holds the wall-clock seconds
-    of the iteration loop (``loop_seconds``) and the norms/force logs only."""
+    of the iteration loop (``loop_seconds``) and the norms/force logs only.  ``wall_dist`` (dw per
+    block, numpy [i-1, j-1, k-1]) is written as wall_dist.dat, which the reference then reads
+    instead of computing the wall distance by brute force (Comput_dist_wall, sub_comput_dw.f90:12-41)."""
     import sys
     sys.path.insert(0, os.path.join(os.path.dirname(HERE), "opencfd-ec_amd"))
     from ocfd_amd import cases as C  # noqa: E402
@@ -106,6 +109,9 @@ def run_case(case, nsteps: int = 1, kshow: int = 1, U_init=None, omp_threads: in
         C.write_case(C.Case(case.name, case.blocks, params), d)
         if U_init is not None:
             C.write_initial_state(os.path.join(d, "U_init.bin"), U_init)
+        if wall_dist is not None:
+            from ocfd_amd import restart as R  # noqa: E402
+            R.write_wall_dist(os.path.join(d, "wall_dist.dat"), wall_dist)
         exe = REF_DRIVER_OMP if omp_threads > 0 else REF_DRIVER
         env = dict(os.environ)
         if omp_threads > 0:

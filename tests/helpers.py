@@ -39,6 +39,15 @@ CASES = {
     "cgrid_sa_iso_roe": Case(lambda: C.naca_cgrid(65, 25, 9, Twall=400.0, Iflag_Flux=4, **_CG), 6, 1e-5, False, 2),
     "cgrid_sa_cut12": Case(lambda: C.cut_blocks(C.naca_cgrid(65, 25, 9, Iflag_Flux=4, **_CG), [[[17, 40], [12], [5]]]),
                            7, 1e-5, False, 2),
+    # inflow (bc 5) and outflow (bc 6) faces of the external-flow mode, and an extrapolated outlet
+    # (bc 401): sub_boundary.f90:28-40,152-283,458-504
+    "channel_inout_bc56": Case(lambda: C.channel_bump(25, 13, 7, inlet_bc=C.BC_INFLOW, outlet_bc=C.BC_OUTFLOW), 13, 1e-3, True, 2),
+    "channel_extrap_bc401": Case(lambda: C.channel_bump(25, 13, 7, outlet_bc=C.BC_EXTRAPOLATE, Iflag_Flux=5), 14, 1e-3, True, 2),
+    # MUT_MAX >= 0: mu_t clamped to MUT_MAX mu (limit_mut, sub_Residual.f90:1387-1408).  The values sit at
+    # the median of mu_t / mu of these states (1.29 for SA, 0.01 for SST), so about half the cells clamp
+    "cgrid_sa_mutmax": Case(lambda: C.naca_cgrid(65, 25, 9, Iflag_Flux=4, MUT_MAX=1.294, **_CG), 11, 1e-3, False, 2),
+    "cgrid_sst_mutmax": Case(lambda: C.naca_cgrid(65, 25, 9, Iflag_turbulence_model=3, Iflag_Scheme=5, Iflag_Flux=6,
+                                                  Re=3e6, Ma=0.7, MUT_MAX=0.01, **_CG), 12, 1e-5, False, 2),
 }
 
 MET = ("ix", "iy", "iz", "jx", "jy", "jz", "kx", "ky", "kz")

@@ -47,3 +47,38 @@ def test_batch_members_equal_alone(graph):
         assert np.array_equal(s.norms()[1], ref_rms)
         assert s.counters()[0] == iters[q]
     bt.close()
+
+
+@pytest.mark.gpu
+def test_member_refused_alone_and_destroy_leaves_batch():
+    """A batch member cannot be stepped alone (OCFD_EARG); destroying a member takes it out of
+    the batch, whose other members then step exactly as alone; after the batch is destroyed its
+    members can be stepped alone again."""
+    cases = _cases()
+    S = [A.solver_from_case(cs, states=C.perturbed_state(cs, seed=30 + q, amp=1e-4)) for q, cs in enumerate(cases)]
+    bt = A.Batch()
+    bt.add(S[0]); bt.add(S[1]); bt.add(S[2])
+    for call in (lambda: S[0].step(1), lambda: S[0].residual(), lambda: S[0].update_limit(), lambda: S[0].bc(),
+                 lambda: S[0].halo(), lambda: S[0].init_flow(), lambda: S[0].autotune_lu(1)):
+        with pytest.raises(A.OcfdError) as e:
+            call()
+        assert e.value.code == A.OCFD_EARG
+    with pytest.raises(A.OcfdError):
+        bt.add(S[1])                                   # already in the batch
+    other = A.Batch()
+    with pytest.raises(A.OcfdError):
+        other.add(S[1])                                # a context is in one batch at most
+    other.close()
+    bt.step(1)
+    S[0].norms(); S[0].forces()                        # monitors work on members
+    S[0].close()                                       # destroyed while in the batch
+    assert S[0] not in bt.members and len(bt.members) == 2
+    bt.step(2)
+    for q in (1, 2):
+        ref, ref_rms = _alone(cases[q], 30 + q, 3)
+        for no in range(1, len(cases[q].blocks) + 1):
+            assert np.array_equal(S[q].download_U(no), ref[no - 1]), f"member {q} block {no}"
+    bt.close()
+    S[1].step(1)                                       # released by the batch's destruction
+    ref, _ = _alone(cases[1], 31, 4)
+    assert np.array_equal(S[1].download_U(1), ref[0])

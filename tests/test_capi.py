@@ -2,6 +2,7 @@
 import ctypes
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -26,6 +27,18 @@ def test_library_exports_every_symbol():
     assert not missing, missing
 
 
-def test_params_layout_matches_header():
-    # 31 doubles + 3 + 3 then 13 int32
-    assert ctypes.sizeof(A.Params) == 8 * (29 + 3 + 3) + 4 * 13 + 4  # trailing pad to 8
+def test_params_layout_matches_header(tmp_path):
+    """The ctypes mirror of ocfd_params has the C struct's size and every member's offset (a C
+    program compiled against include/ocfd_ec.h prints them)."""
+    names = [f[0] for f in A.Params._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"ocfd_ec.h\"\nint main(void) {\n"
+                   '  printf("size %zu\\n", sizeof(ocfd_params));\n'
+                   + "".join(f'  printf("{n} %zu\\n", offsetof(ocfd_params, {n}));\n' for n in names)
+                   + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines())
+    assert int(out["size"]) == ctypes.sizeof(A.Params)
+    for n in names:
+        assert int(out[n]) == getattr(A.Params, n).offset, n

@@ -56,3 +56,8 @@ def test_fortran_host_dropin(name):
     fr = np.array([float(v) for v in force[-1][1:7]])
     scale = max(np.max(np.abs(fr)), 1e-300)
     assert np.all(np.abs(fg - fr) <= 5.1e-8 * np.maximum(np.abs(fr), 1e-6 * scale)), (fg, fr)
+    # the glue's force.log record (ocfd_host_forces, written before the reference's own) at the
+    # same step and to the 8 digits of the format
+    assert len(force) == 2 and force[0][0] == force[1][0]
+    fl = np.array([float(v) for v in force[0][1:7]])
+    assert np.all(np.abs(fl - fr) <= 1.1e-7 * np.maximum(np.abs(fr), 1e-6 * scale)), (fl, fr)

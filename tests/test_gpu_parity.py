@@ -321,9 +321,26 @@ def test_lu_wait_failure_error_and_recovery(schedule):
     s = A.solver_from_case(case, states=U0)
     s.set_lu_schedule(A.LU_PIPELINED if schedule == "pipelined" else A.LU_PIPELINED_FENCED)
     s.set_lu_fail_waits(True)
+    before = s.download_U(1)
     with pytest.raises(A.OcfdError) as e:
         s.step(1)
     assert e.value.code == A.OCFD_EHIP
+    # the update of the failed iteration left U as it was
+    assert np.array_equal(s.download_U(1), before)
+    # several iterations in one call: U stays at the last iteration completed before the failure
+    # (every later sweep returns at once and every later update leaves U alone)
+    refs = [before]
+    r = A.solver_from_case(case, states=U0)
+    r.set_lu_schedule(A.LU_HYPERPLANE)
+    for _ in range(3):
+        r.step(1)
+        refs.append(r.download_U(1))
+    s.set_interior(U0); s.bc(); s.halo()
+    with pytest.raises(A.OcfdError) as e:
+        s.step(3)
+    assert e.value.code == A.OCFD_EHIP
+    after = s.download_U(1)
+    assert any(np.array_equal(after, q) for q in refs[:3]), "U is not the state of a completed iteration"
     s.set_lu_fail_waits(False)
     s.set_interior(U0); s.bc(); s.halo()
     s.step(2)
@@ -331,3 +348,56 @@ def test_lu_wait_failure_error_and_recovery(schedule):
     ref.set_lu_schedule(A.LU_HYPERPLANE)
     ref.step(2)
     assert np.array_equal(s.download_U(1), ref.download_U(1))
+
+
+@pytest.mark.gpu
+def test_swept_wing_reduced_matches_oracle():
+    """The 24-block swept, tapered wing generator (1025x257x233 parent cut 4x3x2; SA, WENO5 + Roe,
+    Mach 0.84, AoA 3, Re 1.2e7) at 1/8 of its resolution per direction (129x33x33 parent, 24
+    blocks), from freestream, on the device-geometry path: the residual after one iteration
+    within 1e-11 per cell and the state after 3 iterations within 1e-9 of the oracle."""
+    case = C.swept_wing(scale=0.125)
+    print(f"\n[swept wing 1/8: {len(case.blocks)} blocks, {case.ncell} cells]", flush=True)
+    assert len(case.blocks) == 24
+    o = oracle.Oracle(case)
+    o.init_flow()
+    s = A.solver_from_case(case)
+    nv = s.nv
+    o.residual(); s.residual()
+    for no, b in enumerate(case.blocks, start=1):
+        nx, ny, nz = b.dims
+        I = (slice(None), slice(2, nx + 1), slice(2, ny + 1), slice(2, nz + 1))
+        _cmp(f"Res b{no}", s.field(no, A.F_RES)[I], oracle_field(o, no, "Res", nv)[I], False)
+    o.update_limit(); s.update_limit()
+    o.bc(); s.bc(); o.halo(); s.halo(); o.force_vt_kw(); s.force_vt_kw()
+    o.step(2); s.step(2)
+    for no, b in enumerate(case.blocks, start=1):
+        nx, ny, nz = b.dims
+        _cmp(f"U3 b{no}", s.download_U(no)[:, 2:nx + 5, 2:ny + 5, 2:nz + 5], oracle_U(o, no, nv)[:, 2:nx + 5, 2:ny + 5, 2:nz + 5],
+             False, tol=1e-9)
+    s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.slow
+def test_swept_wing_full_size_schedules_agree():
+    """The full 60,817,408-cell swept wing (24 blocks, about 100 GB on the device): two iterations
+    from freestream with the pipelined LU-SGS sweeps and with the reference's hyperplane order
+    give the same state bit for bit in every block, and the residual norms agree."""
+    case = C.swept_wing()
+    print(f"\n[swept wing full size: {case.ncell} cells]", flush=True)
+    s = A.solver_from_case(case)
+    s.set_graph(True)
+    s.step(2)
+    U = [s.download_U(no) for no in range(1, 25)]
+    rms = s.norms()[1]
+    print("  pipelined done", flush=True)
+    s.init_flow()
+    s.set_lu_schedule(A.LU_HYPERPLANE)
+    s.step(2)
+    print("  hyperplane done", flush=True)
+    for no in range(1, 25):
+        assert np.array_equal(s.download_U(no), U[no - 1]), f"block {no}"
+    assert np.array_equal(s.norms()[1], rms)
+    assert np.all(np.isfinite(rms))
+    s.close()

@@ -138,15 +138,21 @@ class Server:
 
     def _preparer(self, pending, ready, st, t0, clock):
         """Thread body: for each request in arrival order, wait for its arrival, build its solver
-        and run its first iteration once the memory budget has room, then hand it over."""
+        and run its first iteration once the memory budget has room, then hand it over.
+
+        The device work of this thread holds st["dev"], which the serving loop also holds while
+        the batch may capture its graphs (the first round after a change of membership): a graph
+        capture in progress on one thread must not see device calls from another.  The first
+        iteration here runs without graphs, so this thread never captures."""
         try:
             for r in pending:
                 while clock() - t0 < r.arrival:
                     if st["stop"].is_set():
                         return
                     time.sleep(min(0.02, max(0.0, r.arrival - (clock() - t0))))
-                s = self.make_solver(r.case)
-                need = s.mem_bytes() + self.lu_bytes(r.case)
+                with st["dev"]:
+                    s = self.make_solver(r.case)
+                    need = s.mem_bytes() + self.lu_bytes(r.case)
                 while True:
                     with st["lock"]:
                         held = st["used"] + st["reserved"]
@@ -154,12 +160,15 @@ class Server:
                             st["reserved"] += need
                             break
                     if st["stop"].is_set():
-                        s.close()
+                        with st["dev"]:
+                            s.close()
                         return
                     time.sleep(0.005)
-                s.set_graph(self.graph)
-                s.step(1)
-                ready.put((r, s, need, self.rms1(s), clock() - t0))
+                with st["dev"]:
+                    s.set_graph(False)
+                    s.step(1)
+                    rms0 = self.rms1(s)
+                ready.put((r, s, need, rms0, clock() - t0))
         except BaseException as e:          # handed to the serving loop, which raises it
             ready.put(e)
         finally:
@@ -171,7 +180,8 @@ class Server:
         active: Dict[int, dict] = {}
         results: List[Result] = []
         ready: "queue.Queue" = queue.Queue()
-        st = dict(lock=threading.Lock(), used=0, reserved=0, stop=threading.Event())
+        st = dict(lock=threading.Lock(), dev=threading.Lock(), used=0, reserved=0, stop=threading.Event())
+        flags = dict(recapture=True)
         self.peak_active, self.peak_used = 0, 0
         t0 = clock()
         batch = self.make_batch()
@@ -196,6 +206,7 @@ class Server:
                         st["used"] -= need
                     return
                 batch.add(s)
+                flags["recapture"] = True
                 active[r.rid] = a
                 self.peak_active = max(self.peak_active, len(active))
                 self.peak_used = max(self.peak_used, used)
@@ -221,7 +232,14 @@ class Server:
                     block = False
                 if not active:
                     continue
-                batch.step(self.round_iters)
+                if flags["recapture"]:
+                    # the batch captures its stage graphs in this round: no device calls from the
+                    # preparing thread meanwhile
+                    with st["dev"]:
+                        batch.step(self.round_iters)
+                    flags["recapture"] = False
+                else:
+                    batch.step(self.round_iters)
                 tnow = clock() - t0
                 for rid in list(active):
                     a = active[rid]
@@ -233,6 +251,7 @@ class Server:
                     if conv or failed or a["iters"] >= self.max_iters:
                         results.append(self._result(a, tnow, ratio, conv, failed))
                         batch.remove(a["s"])
+                        flags["recapture"] = True
                         a["s"].close()
                         with st["lock"]:
                             st["used"] -= a["need"]

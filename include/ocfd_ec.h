@@ -130,6 +130,13 @@ int ocfd_force_vt_kw(ocfd_ctx *ctx);
  * Returns OCFD_ENAN when the residual holds a NaN. */
 int ocfd_res_norms(ocfd_ctx *ctx, double *res_max, double *res_rms);
 
+/* ocfd_res_norms without any allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The norms' scratch is a fixed device array of the library, about
+ * 0.7 MB on every GPU it runs on, held until the process ends and shared by every call on that
+ * GPU: calls that overlap on one GPU (several threads or contexts at once) race on it.  At most
+ * 64 blocks per context (OCFD_EARG beyond).  Same results as ocfd_res_norms. */
+int ocfd_res_norms_noalloc(ocfd_ctx *ctx, double *res_max, double *res_rms);
 
 /* comput_force (sub_Post.f90:4-265): out[0..8] = Ft0(1:9) after scaling, out[9..11] = CL CD CS */
 int ocfd_forces(ocfd_ctx *ctx, double *out12);

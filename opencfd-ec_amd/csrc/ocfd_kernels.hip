@@ -1313,8 +1313,6 @@ void set_force_vt_kw(const SetPlan& P, hipStream_t st) {
     if (P.p0.NV >= 6) launch_space<SP_CELL>(k_force_vt_kw, P, st);
 }
 
-int norms_parts() { return 64; }
-
 void launch_norms(const NormBlk* tab, int nb, int NV, double* part, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_norms, dim3((unsigned)norms_parts(), (unsigned)NV, (unsigned)nb), dim3(256), 0, st, tab, NV, part);
     hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(64), 0, st, nb, NV, norms_parts(), (const double*)part, out);

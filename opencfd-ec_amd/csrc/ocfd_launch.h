@@ -43,10 +43,11 @@ void set_halo(const SetPlan& P, hipStream_t st);
 void set_force_vt_kw(const SetPlan& P, hipStream_t st);
 // the residual norms of a table of blocks: parts (nb x NV x norms_parts() x 3 doubles), then
 // out (3 NV doubles: max, sum of squares, NaN flag per component)
-int norms_parts();
+constexpr int NORMS_PARTS = 64;
+inline int norms_parts() { return NORMS_PARTS; }
 void launch_norms(const NormBlk* tab, int nb, int NV, double* part, double* out, hipStream_t st);
 // the residual norms of a context in a caller-provided device buffer (ocfd_runtime.hip)
-constexpr int NORMS_INTO_MAX_BLOCKS = 256;
+constexpr int NORMS_INTO_MAX_BLOCKS = 64;
 int norms_into(ocfd_ctx* c, char* dev, size_t cap, double* res_max, double* res_rms);
 int ctx_device(const ocfd_ctx* c);
 int set_error(int code, const char* msg);

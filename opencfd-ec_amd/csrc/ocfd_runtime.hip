@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -737,6 +738,38 @@ int norms_into(ocfd_ctx* c, char* dev, size_t cap, double* res_max, double* res_
     return run_norms(c, dev, L, res_max, res_rms);
 }
 
+// ---- the device scratch of ocfd_res_norms_noalloc ----
+// The norms' block table, per-workgroup parts and sums for a context of at most
+// NORMS_INTO_MAX_BLOCKS blocks, as a file-scope device array: every GPU the library runs on holds
+// its own copy (about 0.7 MB) until the process ends.  In host code the array's name is not a
+// device address; its address on a device comes from hipGetSymbolAddress, taken once per device
+// with that device current (ocfd_create) and kept in g_scratch.
+constexpr size_t SCRATCH_BYTES = (((size_t)NORMS_INTO_MAX_BLOCKS * sizeof(NormBlk) + 255) & ~(size_t)255) +
+                                 (((size_t)NORMS_INTO_MAX_BLOCKS * MAXV * NORMS_PARTS * 3 * sizeof(double) + 255) & ~(size_t)255) +
+                                 3 * MAXV * sizeof(double);
+static __device__ __align__(256) char scratch[SCRATCH_BYTES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static char* g_scratch[SCRATCH_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// the scratch address on the current device `dev` (once per device)
+int scratch_setup(int dev) {
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return fail(OCFD_EHIP, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[dev]) return OCFD_OK;
+    void* q = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !q) return fail(OCFD_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = static_cast<char*>(q);
+    return OCFD_OK;
+}
+
+char* scratch_of(int dev) {
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch[dev];
+}
+
 int ctx_device(const ocfd_ctx* c) { return c ? c->device : -1; }
 int set_error(int code, const char* msg) { return fail(code, msg); }
 
@@ -764,6 +797,8 @@ int ocfd_create(const ocfd_params* p, ocfd_ctx** out) {
     if (ndev < 1) return fail(OCFD_EHIP, "no HIP device");
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
+    int r = scratch_setup(dev);
+    if (r) return r;
     ocfd_ctx* c = new ocfd_ctx();
     c->hp = *p;
     c->p = to_par(*p);
@@ -1147,6 +1182,13 @@ int ocfd_res_norms(ocfd_ctx* c, double* res_max, double* res_rms) {
         HIPCHK(hipMemcpyAsync(c->norm_buf + L.tab, tab.data(), (size_t)nb * sizeof(NormBlk), hipMemcpyHostToDevice, c->e.st));
     }
     return run_norms(c, c->norm_buf, L, res_max, res_rms);
+}
+
+int ocfd_res_norms_noalloc(ocfd_ctx* c, double* res_max, double* res_rms) {
+    if (!c) return fail(OCFD_EARG, "null context");
+    char* dev = scratch_of(c->device);
+    if (!dev) return fail(OCFD_EHIP, "no scratch address for the context's device");
+    return norms_into(c, dev, SCRATCH_BYTES, res_max, res_rms);
 }
 
 int ocfd_forces(ocfd_ctx* c, double* out) {

@@ -55,7 +55,7 @@ EXPORTS = [
     "ocfd_create", "ocfd_destroy", "ocfd_last_error", "ocfd_add_block", "ocfd_add_block_nodes",
     "ocfd_setup_geometry", "ocfd_upload_U", "ocfd_download_U", "ocfd_download_field", "ocfd_init_flow",
     "ocfd_step_lusgs", "ocfd_residual", "ocfd_update_limit", "ocfd_bc", "ocfd_halo", "ocfd_force_vt_kw",
-    "ocfd_res_norms", "ocfd_forces", "ocfd_block_status", "ocfd_get_counters", "ocfd_set_graph", "ocfd_sync",
+    "ocfd_res_norms", "ocfd_res_norms_noalloc", "ocfd_forces", "ocfd_block_status", "ocfd_get_counters", "ocfd_set_graph", "ocfd_sync",
     "ocfd_step_timed", "ocfd_set_lu_schedule", "ocfd_set_lu_tile", "ocfd_get_lu_tile", "ocfd_autotune_lu",
     "ocfd_set_lu_fail_waits",
     "ocfd_batch_create", "ocfd_batch_destroy", "ocfd_batch_add", "ocfd_batch_remove", "ocfd_batch_step",
@@ -91,6 +91,7 @@ def lib():
         L.ocfd_download_field.argtypes = [vp, i32, i32, dp]
         L.ocfd_step_lusgs.argtypes = [vp, i32]
         L.ocfd_res_norms.argtypes = [vp, dp, dp]
+        L.ocfd_res_norms_noalloc.argtypes = [vp, dp, dp]
         L.ocfd_forces.argtypes = [vp, dp]
         L.ocfd_block_status.argtypes = [vp, i32, ip, ip]
         L.ocfd_get_counters.argtypes = [vp, ip, dp]
@@ -365,6 +366,13 @@ class Solver:
     def norms(self):
         mx = np.zeros(self.nv); rms = np.zeros(self.nv)
         _check(lib().ocfd_res_norms(self._h, _dp(mx), _dp(rms)))
+        return mx, rms
+
+    def norms_noalloc(self):
+        """norms() through the entry point that allocates nothing (the library's fixed device
+        scratch, at most 64 blocks per context)."""
+        mx = np.zeros(self.nv); rms = np.zeros(self.nv)
+        _check(lib().ocfd_res_norms_noalloc(self._h, _dp(mx), _dp(rms)))
         return mx, rms
 
     def forces(self):

@@ -5,6 +5,7 @@ output.  GPU: the HIP path against the same fixtures, through the C ABI.
 """
 import glob
 import json
+import math
 import os
 
 import numpy as np
@@ -142,3 +143,23 @@ def test_hip_path_matches_reference_fixture(name):
     np.testing.assert_allclose(rms, z["norms_rms"][-1], rtol=1e-9)
     f = s.forces()
     assert _force_ok([f[9], f[10], f[11], f[6], f[7], f[8]], z["force_log"][-1][1:7])
+
+
+def test_converged_sa_fixture_oracle_matches_reference():
+    """The converged SA C-grid fixture (tests/golden/make_converged.py, 30000 iterations): the
+    oracle's CL, CD and CS equal the compiled reference's full-precision print within 1e-12
+    relative, its moments Ft0(7:9) the reference's force.log to the log's 8 digits, and both ran
+    to a residual drop below 1e-10."""
+    fx = json.load(open(os.path.join(HERE, "golden", "converged_sa_cgrid_65x25x9.json")))
+    ref, orc = fx["reference"], fx["oracle"]
+    for k in ("CL", "CD"):
+        assert abs(ref[k] - orc[k]) <= 1e-12 * abs(ref[k]), (k, ref[k], orc[k])
+    assert abs(ref["CS"] - orc["CS"]) <= 1e-12 * abs(ref["CL"])
+    log = ref["forcelog_last"]
+    assert int(log[0]) == fx["iterations"]
+    for q, m in enumerate(orc["Ft"][6:9]):
+        # force.log's E20.8: an 8-digit mantissa in [0.1, 1), so half a unit of its last digit
+        half_unit = 0.5 * 10.0 ** (math.floor(math.log10(abs(m))) + 1 - 8)
+        assert abs(log[4 + q] - m) <= half_unit, (q, log[4 + q], m)
+    assert ref["res_rms1"][-1][1] / orc["res_rms1_first"] < 1e-10
+    assert orc["res_rms1_last"] / orc["res_rms1_first"] < 1e-10

@@ -14,7 +14,7 @@ Workloads (the default is the largest single-GPU configuration, swept_wing_24blo
                              Mach 0.8, AoA 1.25 deg, WENO5 reconstruction, FP64
   channel_bump_97x33x9       inviscid channel with a 10% bump, 97x33x9 nodes, MUSCL3 + Roe
   naca4412_48block           577x193x129 C-grid around NACA 4412 cut 6x4x2 at seeded planes into 48
-                             uneven blocks (14,155,776 cells); SST, Re 3e6, Mach 0.7, MUSCL3 + AUSMPW
+                             uneven blocks (14,155,776 cells); SST, Re 3e6, Mach 0.7, MUSCL3 + AUSMPW, CFL 0.1
   swept_wing_24block         1025x257x233 swept (30 deg), tapered (0.5) wing grid cut 4x3x2 into 24
                              blocks (60,817,408 cells); SA, Re 1.2e7, Mach 0.84, AoA 3, WENO5 + Roe
 
@@ -95,20 +95,27 @@ def batch_member(name: str, flux: int, q: int):
     return case
 
 
-def variant_rate(name: str, flux: int, precision: int, batch: int, steps: int, warmup: int, graph: bool):
+def variant_rate(name: str, flux: int, precision: int, batch: int, steps: int, warmup: int, graph: bool,
+                 plain: bool = False):
     """Mcell-iter/s of `batch` copies of the workload stepped together (one launch per kernel over
-    all their blocks), in the given precision mode; inputs resident before the timed region."""
+    all their blocks), in the given precision mode; inputs resident before the timed region.  With
+    `plain`, one copy in the per-block launch mode instead: every kernel launched once per block,
+    eagerly (no graphs), the LU-SGS sweeps on 8x8 tiles (no autotuning)."""
     members = [A.solver_from_case(batch_member(name, flux, q), precision=precision) for q in range(batch)]
     cells = sum(ncells(batch_member(name, flux, 0)) for _ in range(batch))
     if batch == 1:
         run = members[0]
-        run.set_graph(graph)
+        run.set_graph(graph and not plain)
     else:
         run = A.Batch()
         run.set_graph(graph)
         for m in members:
             run.add(m)
-    tj, _ = run.autotune_lu(1)
+    if plain:
+        run.set_per_block(True)
+        tj = 8
+    else:
+        tj, _ = run.autotune_lu(1)
     run.step(warmup)
     t0 = time.perf_counter()
     run.step(steps)               # returns after the device has finished
@@ -119,6 +126,7 @@ def variant_rate(name: str, flux: int, precision: int, batch: int, steps: int, w
     for m in members:
         m.close()
     return dict(precision="fp64" if precision == 0 else "mixed", batch=batch, cells=cells, lu_tile=f"{tj}x{64 // tj}",
+                launches="per block, eager" if plain else "batched" + (", graphs" if graph else ""),
                 value=round(cells * steps / wall / 1e6, 3), ms_per_step=round(wall * 1e3 / steps, 4),
                 res_rms1_max=max(rms))
 
@@ -298,9 +306,9 @@ def main() -> int:
                     help="precision of the headline run (fp64: the drop-in; mixed: FP32 reconstruction, "
                          "fluxes and SA terms)")
     ap.add_argument("--variants", default=None,
-                    help="extra runs reported beside the headline, precision:batch pairs ('' for none; "
-                         "default mixed:1,fp64:4,mixed:4 on the single-block C-grid, mixed:1 on the swept wing, "
-                         "none elsewhere)")
+                    help="extra runs reported beside the headline, precision:batch pairs, or precision:plain for the "
+                         "per-block eager launches ('' for none; default mixed:1,fp64:4,mixed:4 on the single-block "
+                         "C-grid, mixed:1,fp64:plain on the swept wing, fp64:plain,mixed:1 on the 48-block grid, none elsewhere)")
     ap.add_argument("--serve", type=int, default=0, help="run the serving sweep with this many requests instead")
     ap.add_argument("--rate", type=float, default=2.0, help="serving: request arrivals per second")
     ap.add_argument("--drop", type=float, default=5e-3, help="serving: Res_rms(1) drop that ends a solve")
@@ -422,12 +430,15 @@ def main() -> int:
     }
     s.close()
     if args.variants is None:
-        args.variants = {"naca0012_cgrid_257x97x33": "mixed:1,fp64:4,mixed:4", "swept_wing_24block": "mixed:1"}.get(args.workload, "")
+        args.variants = {"naca0012_cgrid_257x97x33": "mixed:1,fp64:4,mixed:4", "swept_wing_24block": "mixed:1,fp64:plain",
+                         "naca4412_48block": "fp64:plain,mixed:1"}.get(args.workload, "")
     if args.variants:
         out["variants"] = []
         for v in args.variants.split(","):
             pr, bt = v.split(":")
-            r = variant_rate(args.workload, args.flux, 0 if pr == "fp64" else 1, int(bt), args.steps, args.warmup, bool(args.graph))
+            plain = bt == "plain"
+            r = variant_rate(args.workload, args.flux, 0 if pr == "fp64" else 1, 1 if plain else int(bt), args.steps,
+                             args.warmup, bool(args.graph), plain=plain)
             r["vs_headline"] = round(r["value"] / value, 3)
             out["variants"].append(r)
     if not args.no_cpu:

@@ -45,9 +45,8 @@ typedef struct ocfd_params {
     int32_t IFlag_Reconstruction, Bound_Scheme, Iflag_local_dt, If_dtime_mesh;
     int32_t IF_Scheme_Positivity, IFLAG_LIMIT_FLOW, Cood_Y_UP;
     int32_t precision;       /* 0: all FP64 (the drop-in).  1: reconstruction, face fluxes (inviscid,
-                                viscous) and the SA face and source terms in FP32; metrics, LU-SGS
-                                (Jacobians, diagonal), update, limiter and norms in FP64.  Laminar
-                                and SA only (SST is refused with OCFD_EUNSUPPORTED). */
+                                viscous) and the SA or SST face and source terms in FP32; metrics,
+                                LU-SGS (Jacobians, diagonal), update, limiter and norms in FP64. */
     int32_t IF_InnerFlow;      /* IF_InnerFlow (sub_modules.f90:163).  1 turns bc 5/6 into the total-
                                   pressure inlet and the outlet of the inner-flow mode and changes
                                   the initial field (sub_boundary.f90:28-40; sub_init.f90:322): such
@@ -203,6 +202,13 @@ int ocfd_set_lu_tile(ocfd_ctx *ctx, int32_t tj);
  * the later sweeps return at once), and the sweep tables are restored for the next call.  With on = 1 every wait that finds its producer behind gives up at its first poll, which
  * forces that path; on = 0 restores normal waiting.  No counterpart in the reference. */
 int ocfd_set_lu_fail_waits(ocfd_ctx *ctx, int32_t on);
+
+/* Launch mode.  on = 1: every kernel of the iteration is launched once per block, eagerly (no
+ * graphs), and the LU-SGS sweeps keep 8x8 tiles (the autotuner refuses); on = 0 (the default):
+ * one launch per kernel over all blocks, graphs per ocfd_set_graph, any tile shape.  The results
+ * are the same bit for bit.  No counterpart in the reference (the per-block form is its block
+ * loop, for comparison with the batched launches). */
+int ocfd_set_per_block(ocfd_ctx *ctx, int32_t on);
 int ocfd_get_lu_tile(ocfd_ctx *ctx, int32_t *tj);
 
 /* Tile-shape autotuner of OCFD_LU_PIPELINED.  No counterpart in the reference.  Runs niter

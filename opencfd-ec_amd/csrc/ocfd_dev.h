@@ -100,6 +100,8 @@ __device__ __forceinline__ int xcd_tile(int t, int n) {
     return t < full ? (t & 7) * (full >> 3) + (t >> 3) : t;
 }
 
+constexpr int ROW_CHUNK = 16;
+
 template <int SP>
 __device__ __forceinline__ bool locate(const BSet& S, int& bi, int& i, int& j, int& k) {
     const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
@@ -112,14 +114,19 @@ __device__ __forceinline__ bool locate(const BSet& S, int& bi, int& i, int& j, i
     if (a >= n0 * n1 * n2) return false;
     const int r = a / n0;
     i = o0 + (a - r * n0);
-    if (SP == SP_FK) {
-        // k-face kernels read stencils along k: rows run k fastest, so the eight rows of a stencil
-        // are neighbouring tiles (and, after the renumbering above, tiles of the same XCD)
-        k = o2 + r % n2;
-        j = o1 + r / n2;
+    // rows (j, k) in chunks of ROW_CHUNK planes along k, k fastest inside a chunk: the rows a
+    // stencil reads at j +- 1 are ROW_CHUNK rows away and those at k +- 1 (k +- 4 for the
+    // reconstruction of the k faces) the neighbouring rows, so both stay in the L2 of the XCD that
+    // runs the tile (the renumbering above keeps a launch's neighbouring tiles on one XCD)
+    const int full = n2 / ROW_CHUNK * ROW_CHUNK;
+    if (r < n1 * full) {
+        const int c = r / (n1 * ROW_CHUNK), rr = r - c * (n1 * ROW_CHUNK);
+        j = o1 + rr / ROW_CHUNK;
+        k = o2 + c * ROW_CHUNK + rr % ROW_CHUNK;
     } else {
-        j = o1 + r % n1;
-        k = o2 + r / n1;
+        const int rem = n2 - full, rr = r - n1 * full;
+        j = o1 + rr / rem;
+        k = o2 + full + rr % rem;
     }
     return true;
 }

@@ -219,52 +219,55 @@ __device__ __forceinline__ void d_amut(const Blk& b, const Sub& B, int a, int c)
 __device__ __forceinline__ double kt_of(const Blk& b, size_t s) { return b.kt[s]; }
 __device__ __forceinline__ double wt_of(const Blk& b, size_t s) { return b.wt[s]; }
 
-// cell pass: mu_t, f1, sources into Res(6:7)
+// cell pass: mu_t, f1, sources into Res(6:7), in the compute type T (FP64, or FP32 in the mixed
+// mode, where the velocities come from the FP32 primitives); stored in FP64
+template <typename T>
 __global__ void __launch_bounds__(64) k_sst_cell(BSet S) {
     OCFD_LOCATE(SP_CELL)
-    const double sigma_w2 = 0.856, beta1 = 0.075, Cw1s = 0.533, beta2 = 0.0828, Cw2s = 0.440;
-    const double a1 = 0.31, betas = 0.09, c009 = (double)0.09f;
-    const double Re = p.Re;
-    const double *uu = b.prim[1], *vv = b.prim[2], *ww = b.prim[3];
+    const T sigma_w2 = T(0.856), beta1 = T(0.075), Cw1s = T(0.533), beta2 = T(0.0828), Cw2s = T(0.440);
+    const T a1 = T(0.31), betas = T(0.09), c009 = T((double)0.09f);
+    const T Re = T(p.Re);
+    const T *uu = prim_of<T>(b, 1), *vv = prim_of<T>(b, 2), *ww = prim_of<T>(b, 3);
     size_t s = IX(b, i, j, k);
-    auto DIFF = [&](const double* a, int di, int dj, int dk) { return a[IX(b, i + di, j + dj, k + dk)] - a[IX(b, i - di, j - dj, k - dk)]; };
-    auto DK = [&](int di, int dj, int dk) { return kt_of(b, IX(b, i + di, j + dj, k + dk)) - kt_of(b, IX(b, i - di, j - dj, k - dk)); };
-    auto DW = [&](int di, int dj, int dk) { return wt_of(b, IX(b, i + di, j + dj, k + dk)) - wt_of(b, IX(b, i - di, j - dj, k - dk)); };
-    double ui = DIFF(uu, 1, 0, 0), vi = DIFF(vv, 1, 0, 0), wi = DIFF(ww, 1, 0, 0), kti = DK(1, 0, 0), wti = DW(1, 0, 0);
-    double uj = DIFF(uu, 0, 1, 0), vj = DIFF(vv, 0, 1, 0), wj = DIFF(ww, 0, 1, 0), ktj = DK(0, 1, 0), wtj = DW(0, 1, 0);
-    double uk = DIFF(uu, 0, 0, 1), vk = DIFF(vv, 0, 0, 1), wk = DIFF(ww, 0, 0, 1), ktk = DK(0, 0, 1), wtk = DW(0, 0, 1);
+    auto DIFF = [&](const T* a, int di, int dj, int dk) { return a[IX(b, i + di, j + dj, k + dk)] - a[IX(b, i - di, j - dj, k - dk)]; };
+    auto DK = [&](int di, int dj, int dk) { return T(kt_of(b, IX(b, i + di, j + dj, k + dk))) - T(kt_of(b, IX(b, i - di, j - dj, k - dk))); };
+    auto DW = [&](int di, int dj, int dk) { return T(wt_of(b, IX(b, i + di, j + dj, k + dk))) - T(wt_of(b, IX(b, i - di, j - dj, k - dk))); };
+    T ui = DIFF(uu, 1, 0, 0), vi = DIFF(vv, 1, 0, 0), wi = DIFF(ww, 1, 0, 0), kti = DK(1, 0, 0), wti = DW(1, 0, 0);
+    T uj = DIFF(uu, 0, 1, 0), vj = DIFF(vv, 0, 1, 0), wj = DIFF(ww, 0, 1, 0), ktj = DK(0, 1, 0), wtj = DW(0, 1, 0);
+    T uk = DIFF(uu, 0, 0, 1), vk = DIFF(vv, 0, 0, 1), wk = DIFF(ww, 0, 0, 1), ktk = DK(0, 0, 1), wtk = DW(0, 0, 1);
     double* const* m0 = b.met[0];
-    double ix = m0[0][s], iy = m0[1][s], iz = m0[2][s], jx = m0[3][s], jy = m0[4][s], jz = m0[5][s], kx = m0[6][s], ky = m0[7][s], kz = m0[8][s];
-    double vx = (vi * ix + vj * jx) + vk * kx, wx = (wi * ix + wj * jx) + wk * kx;
-    double ktx = (kti * ix + ktj * jx) + ktk * kx, wtx = (wti * ix + wtj * jx) + wtk * kx;
-    double uy = (ui * iy + uj * jy) + uk * ky, wy = (wi * iy + wj * jy) + wk * ky;
-    double kty = (kti * iy + ktj * jy) + ktk * ky, wty = (wti * iy + wtj * jy) + wtk * ky;
-    double uz = (ui * iz + uj * jz) + uk * kz, vz = (vi * iz + vj * jz) + vk * kz;
-    double ktz = (kti * iz + ktj * jz) + ktk * kz, wtz = (wti * iz + wtj * jz) + wtk * kz;
-    double omega = sqrt((sq(wy - vz) + sq(uz - wx)) + sq(vx - uy));
-    double d = b.prim[0][s], mu = b.mu[s] * Re, dw = b.dw[s], kt = kt_of(b, s), wt = wt_of(b, s);
-    double a2a = 2.0 * sqrt(fabs(kt)) / (((c009 * wt) * dw) * Re);
-    double a2b = (500.0 * mu) / ((((d * wt) * sq(dw)) * Re) * Re);
-    double arg2 = a2a > a2b ? a2a : a2b;
-    double f2 = tanh(sq(arg2));
-    double den = a1 * wt, alt = (f2 * fabs(omega)) / Re;
-    double mutc = ((a1 * d) * kt) / (den > alt ? den : alt);
-    double Kws = (((2.0 * ((ktx * wtx + kty * wty) + ktz * wtz)) * d) * sigma_w2) / (wt + 1.0e-20);
-    double CD_kw = Kws > 1.0e-20 ? Kws : 1.0e-20;
-    double a3a = sqrt(fabs(kt)) / (((c009 * wt) * dw) * Re);
-    double arg3 = a3a > a2b ? a3a : a2b;
-    double a1b = (((4.0 * d) * sigma_w2) * kt) / (CD_kw * sq(dw));
-    double arg1 = arg3 < a1b ? arg3 : a1b;
-    double f1 = tanh(((arg1 * arg1) * arg1) * arg1);
-    double Pk = (mutc * omega) * omega;
-    double Qk = Pk / Re - (((Re * betas) * d) * wt) * kt;
-    double Cw = f1 * Cw1s + (1.0 - f1) * Cw2s;
-    double beta = f1 * beta1 + (1.0 - f1) * beta2;
-    double Qw = ((((Cw * d) * omega) * omega) / Re - ((Re * beta) * d) * sq(wt)) + ((1.0 - f1) * Kws) / Re;
-    b.Res[5][s] = Qk * b.vol[s];
-    b.Res[6][s] = Qw * b.vol[s];
-    b.mut[s] = mutc;      // Re-scaled value, divided by Re in k_sst_finish
-    b.aux[s] = f1;
+    T ix = T(m0[0][s]), iy = T(m0[1][s]), iz = T(m0[2][s]), jx = T(m0[3][s]), jy = T(m0[4][s]), jz = T(m0[5][s]), kx = T(m0[6][s]), ky = T(m0[7][s]), kz = T(m0[8][s]);
+    T vx = (vi * ix + vj * jx) + vk * kx, wx = (wi * ix + wj * jx) + wk * kx;
+    T ktx = (kti * ix + ktj * jx) + ktk * kx, wtx = (wti * ix + wtj * jx) + wtk * kx;
+    T uy = (ui * iy + uj * jy) + uk * ky, wy = (wi * iy + wj * jy) + wk * ky;
+    T kty = (kti * iy + ktj * jy) + ktk * ky, wty = (wti * iy + wtj * jy) + wtk * ky;
+    T uz = (ui * iz + uj * jz) + uk * kz, vz = (vi * iz + vj * jz) + vk * kz;
+    T ktz = (kti * iz + ktj * jz) + ktk * kz, wtz = (wti * iz + wtj * jz) + wtk * kz;
+    T omega = sqrt((sq(wy - vz) + sq(uz - wx)) + sq(vx - uy));
+    T d = T(b.prim[0][s]), mu = T(b.mu[s] * p.Re), dw = T(b.dw[s]), kt = T(kt_of(b, s)), wt = T(wt_of(b, s));
+    T a2a = T(2) * sqrt(fabs(kt)) / (((c009 * wt) * dw) * Re);
+    T a2b = (T(500) * mu) / ((((d * wt) * sq(dw)) * Re) * Re);
+    T arg2 = a2a > a2b ? a2a : a2b;
+    T f2 = tanh(sq(arg2));
+    T den = a1 * wt, alt = (f2 * fabs(omega)) / Re;
+    T mutc = ((a1 * d) * kt) / (den > alt ? den : alt);
+    T Kws = (((T(2) * ((ktx * wtx + kty * wty) + ktz * wtz)) * d) * sigma_w2) / (wt + T(1.0e-20));
+    T CD_kw = Kws > T(1.0e-20) ? Kws : T(1.0e-20);
+    T a3a = sqrt(fabs(kt)) / (((c009 * wt) * dw) * Re);
+    T arg3 = a3a > a2b ? a3a : a2b;
+    T a1b = (((T(4) * d) * sigma_w2) * kt) / (CD_kw * sq(dw));
+    T arg1 = arg3 < a1b ? arg3 : a1b;
+    T f1 = tanh(((arg1 * arg1) * arg1) * arg1);
+    T Pk = (mutc * omega) * omega;
+    T Qk = Pk / Re - (((Re * betas) * d) * wt) * kt;
+    T Cw = f1 * Cw1s + (T(1) - f1) * Cw2s;
+    T beta = f1 * beta1 + (T(1) - f1) * beta2;
+    T Qw = ((((Cw * d) * omega) * omega) / Re - ((Re * beta) * d) * sq(wt)) + ((T(1) - f1) * Kws) / Re;
+    T vol = T(b.vol[s]);
+    b.Res[5][s] = (double)(Qk * vol);
+    b.Res[6][s] = (double)(Qw * vol);
+    b.mut[s] = (double)mutc;      // Re-scaled value, divided by Re in k_sst_finish
+    b.aux[s] = (double)f1;
 }
 
 // mu_t ghost copies over the whole -1:n+1 extent (sub_turbulence_SST.f90:170-175): face f in
@@ -280,57 +283,60 @@ __global__ void k_sst_ghost(BSet S, int f) {
 
 // k and omega flux through face (i,j,k) of direction DIR, between cells (i,j,k) - e_DIR and
 // (i,j,k): convection upwinded on the normal velocity, diffusion with the face gradient of k and
-// omega; one thread per face, each face evaluated once for both of its cells
-template <int DIR>
+// omega; one thread per face, each face evaluated once for both of its cells; compute type T,
+// stored in FP64
+template <int DIR, typename T>
 __global__ void __launch_bounds__(64) k_sst_face(BSet S) {
     OCFD_LOCATE(SP_FI + DIR)
-    const double sigma_k1 = 0.85, sigma_w1 = 0.5, sigma_k2 = 1.0, sigma_w2 = 0.856;
-    const double Re = p.Re;
-    const double *uu = b.prim[1], *vv = b.prim[2], *ww = b.prim[3];
+    const T sigma_k1 = T(0.85), sigma_w1 = T(0.5), sigma_k2 = T(1.0), sigma_w2 = T(0.856);
+    const T Re = T(p.Re);
+    const T *uu = prim_of<T>(b, 1), *vv = prim_of<T>(b, 2), *ww = prim_of<T>(b, 3);
     const int di = DIR == 0, dj = DIR == 1, dk = DIR == 2;
     const int fi = i, fj = j, fk_ = k;
     size_t f = IX(b, fi, fj, fk_), fm = IX(b, fi - di, fj - dj, fk_ - dk);
-    double f1 = b.aux[f];
-    double skk = f1 * sigma_k1 + (1.0 - f1) * sigma_k2, sww = f1 * sigma_w1 + (1.0 - f1) * sigma_w2;
-    double mua = b.mu[fm] * Re, mub = b.mu[f] * Re, mta = b.mut[fm], mtb = b.mut[f];
-    double muk = (((mua + mub) + skk * (mta + mtb)) * 0.5) / Re;
-    double muw = (((mua + mub) + sww * (mta + mtb)) * 0.5) / Re;
-    auto K = [&](int a, int bb, int cc) { return kt_of(b, IX(b, a, bb, cc)); };
-    auto W = [&](int a, int bb, int cc) { return wt_of(b, IX(b, a, bb, cc)); };
-    double kti, ktj, ktk, wti, wtj, wtk;
+    T f1 = T(b.aux[f]);
+    T skk = f1 * sigma_k1 + (T(1) - f1) * sigma_k2, sww = f1 * sigma_w1 + (T(1) - f1) * sigma_w2;
+    T mua = T(b.mu[fm] * p.Re), mub = T(b.mu[f] * p.Re), mta = T(b.mut[fm]), mtb = T(b.mut[f]);
+    T muk = (((mua + mub) + skk * (mta + mtb)) * T(0.5)) / Re;
+    T muw = (((mua + mub) + sww * (mta + mtb)) * T(0.5)) / Re;
+    auto K = [&](int a, int bb, int cc) { return T(kt_of(b, IX(b, a, bb, cc))); };
+    auto W = [&](int a, int bb, int cc) { return T(wt_of(b, IX(b, a, bb, cc))); };
+    const T q = T(0.25);
+    T kti, ktj, ktk, wti, wtj, wtk;
     if constexpr (DIR == 0) {
         kti = K(fi, fj, fk_) - K(fi - 1, fj, fk_); wti = W(fi, fj, fk_) - W(fi - 1, fj, fk_);
-        ktj = 0.25 * (((K(fi, fj + 1, fk_) - K(fi, fj - 1, fk_)) + K(fi - 1, fj + 1, fk_)) - K(fi - 1, fj - 1, fk_));
-        wtj = 0.25 * (((W(fi, fj + 1, fk_) - W(fi, fj - 1, fk_)) + W(fi - 1, fj + 1, fk_)) - W(fi - 1, fj - 1, fk_));
-        ktk = 0.25 * (((K(fi, fj, fk_ + 1) - K(fi, fj, fk_ - 1)) + K(fi - 1, fj, fk_ + 1)) - K(fi - 1, fj, fk_ - 1));
-        wtk = 0.25 * (((W(fi, fj, fk_ + 1) - W(fi, fj, fk_ - 1)) + W(fi - 1, fj, fk_ + 1)) - W(fi - 1, fj, fk_ - 1));
+        ktj = q * (((K(fi, fj + 1, fk_) - K(fi, fj - 1, fk_)) + K(fi - 1, fj + 1, fk_)) - K(fi - 1, fj - 1, fk_));
+        wtj = q * (((W(fi, fj + 1, fk_) - W(fi, fj - 1, fk_)) + W(fi - 1, fj + 1, fk_)) - W(fi - 1, fj - 1, fk_));
+        ktk = q * (((K(fi, fj, fk_ + 1) - K(fi, fj, fk_ - 1)) + K(fi - 1, fj, fk_ + 1)) - K(fi - 1, fj, fk_ - 1));
+        wtk = q * (((W(fi, fj, fk_ + 1) - W(fi, fj, fk_ - 1)) + W(fi - 1, fj, fk_ + 1)) - W(fi - 1, fj, fk_ - 1));
     } else if constexpr (DIR == 1) {
-        kti = 0.25 * (((K(fi + 1, fj, fk_) - K(fi - 1, fj, fk_)) + K(fi + 1, fj - 1, fk_)) - K(fi - 1, fj - 1, fk_));
+        kti = q * (((K(fi + 1, fj, fk_) - K(fi - 1, fj, fk_)) + K(fi + 1, fj - 1, fk_)) - K(fi - 1, fj - 1, fk_));
         ktj = K(fi, fj, fk_) - K(fi, fj - 1, fk_);
-        ktk = 0.25 * (((K(fi, fj, fk_ + 1) - K(fi, fj, fk_ - 1)) + K(fi, fj - 1, fk_ + 1)) - K(fi, fj - 1, fk_ - 1));
-        wti = 0.25 * (((W(fi + 1, fj, fk_) - W(fi - 1, fj, fk_)) + W(fi + 1, fj - 1, fk_)) - W(fi - 1, fj - 1, fk_));
+        ktk = q * (((K(fi, fj, fk_ + 1) - K(fi, fj, fk_ - 1)) + K(fi, fj - 1, fk_ + 1)) - K(fi, fj - 1, fk_ - 1));
+        wti = q * (((W(fi + 1, fj, fk_) - W(fi - 1, fj, fk_)) + W(fi + 1, fj - 1, fk_)) - W(fi - 1, fj - 1, fk_));
         wtj = W(fi, fj, fk_) - W(fi, fj - 1, fk_);
-        wtk = 0.25 * (((W(fi, fj, fk_ + 1) - W(fi, fj, fk_ - 1)) + W(fi, fj - 1, fk_ + 1)) - W(fi, fj - 1, fk_ - 1));
+        wtk = q * (((W(fi, fj, fk_ + 1) - W(fi, fj, fk_ - 1)) + W(fi, fj - 1, fk_ + 1)) - W(fi, fj - 1, fk_ - 1));
     } else {
-        kti = 0.25 * (((K(fi + 1, fj, fk_) - K(fi - 1, fj, fk_)) + K(fi + 1, fj, fk_ - 1)) - K(fi - 1, fj, fk_ - 1));
-        ktj = 0.25 * (((K(fi, fj + 1, fk_) - K(fi, fj - 1, fk_)) + K(fi, fj + 1, fk_ - 1)) - K(fi, fj - 1, fk_ - 1));
+        kti = q * (((K(fi + 1, fj, fk_) - K(fi - 1, fj, fk_)) + K(fi + 1, fj, fk_ - 1)) - K(fi - 1, fj, fk_ - 1));
+        ktj = q * (((K(fi, fj + 1, fk_) - K(fi, fj - 1, fk_)) + K(fi, fj + 1, fk_ - 1)) - K(fi, fj - 1, fk_ - 1));
         ktk = K(fi, fj, fk_) - K(fi, fj, fk_ - 1);
-        wti = 0.25 * (((W(fi + 1, fj, fk_) - W(fi - 1, fj, fk_)) + W(fi + 1, fj, fk_ - 1)) - W(fi - 1, fj, fk_ - 1));
-        wtj = 0.25 * (((W(fi, fj + 1, fk_) - W(fi, fj - 1, fk_)) + W(fi, fj + 1, fk_ - 1)) - W(fi, fj - 1, fk_ - 1));
+        wti = q * (((W(fi + 1, fj, fk_) - W(fi - 1, fj, fk_)) + W(fi + 1, fj, fk_ - 1)) - W(fi - 1, fj, fk_ - 1));
+        wtj = q * (((W(fi, fj + 1, fk_) - W(fi, fj - 1, fk_)) + W(fi, fj + 1, fk_ - 1)) - W(fi, fj - 1, fk_ - 1));
         wtk = W(fi, fj, fk_) - W(fi, fj, fk_ - 1);
     }
     double* const* mt = b.met[DIR + 1];
-    double ktx = (kti * mt[0][f] + ktj * mt[3][f]) + ktk * mt[6][f], wtx = (wti * mt[0][f] + wtj * mt[3][f]) + wtk * mt[6][f];
-    double kty = (kti * mt[1][f] + ktj * mt[4][f]) + ktk * mt[7][f], wty = (wti * mt[1][f] + wtj * mt[4][f]) + wtk * mt[7][f];
-    double ktz = (kti * mt[2][f] + ktj * mt[5][f]) + ktk * mt[8][f], wtz = (wti * mt[2][f] + wtj * mt[5][f]) + wtk * mt[8][f];
-    double s1x = b.nv[3 * DIR][f], s1y = b.nv[3 * DIR + 1][f], s1z = b.nv[3 * DIR + 2][f];
-    double vn1 = (uu[fm] * s1x + vv[fm] * s1y) + ww[fm] * s1z;
-    double vn2 = (uu[f] * s1x + vv[f] * s1y) + ww[f] * s1z;
-    double kfi = 0.5 * ((vn1 + fabs(vn1)) * kt_of(b, fm) + (vn2 - fabs(vn2)) * kt_of(b, f));
-    double wfi = 0.5 * ((vn1 + fabs(vn1)) * wt_of(b, fm) + (vn2 - fabs(vn2)) * wt_of(b, f));
-    double Sf = b.S[DIR][f];
-    b.Fsa[DIR][0][f] = (-kfi + muk * ((ktx * s1x + kty * s1y) + ktz * s1z)) * Sf;
-    b.Fsa[DIR][1][f] = (-wfi + muw * ((wtx * s1x + wty * s1y) + wtz * s1z)) * Sf;
+    auto M = [&](int c) { return T(mt[c][f]); };
+    T ktx = (kti * M(0) + ktj * M(3)) + ktk * M(6), wtx = (wti * M(0) + wtj * M(3)) + wtk * M(6);
+    T kty = (kti * M(1) + ktj * M(4)) + ktk * M(7), wty = (wti * M(1) + wtj * M(4)) + wtk * M(7);
+    T ktz = (kti * M(2) + ktj * M(5)) + ktk * M(8), wtz = (wti * M(2) + wtj * M(5)) + wtk * M(8);
+    T s1x = T(b.nv[3 * DIR][f]), s1y = T(b.nv[3 * DIR + 1][f]), s1z = T(b.nv[3 * DIR + 2][f]);
+    T vn1 = (uu[fm] * s1x + vv[fm] * s1y) + ww[fm] * s1z;
+    T vn2 = (uu[f] * s1x + vv[f] * s1y) + ww[f] * s1z;
+    T kfi = T(0.5) * ((vn1 + fabs(vn1)) * K(fi - di, fj - dj, fk_ - dk) + (vn2 - fabs(vn2)) * K(fi, fj, fk_));
+    T wfi = T(0.5) * ((vn1 + fabs(vn1)) * W(fi - di, fj - dj, fk_ - dk) + (vn2 - fabs(vn2)) * W(fi, fj, fk_));
+    T Sf = T(b.S[DIR][f]);
+    b.Fsa[DIR][0][f] = (double)((-kfi + muk * ((ktx * s1x + kty * s1y) + ktz * s1z)) * Sf);
+    b.Fsa[DIR][1][f] = (double)((-wfi + muw * ((wtx * s1x + wty * s1y) + wtz * s1z)) * Sf);
 }
 
 // the face fluxes accumulated into Res(6:7) per direction in i, j, k order
@@ -1232,14 +1238,21 @@ void set_pre(const SetPlan& P, hipStream_t st) {
         launch_space<SP_N0>(k_sa_mut, P, st);
         launch_subs(k_amut, P, P.amut, st);
     } else if (p.turb == 3) {
-        launch_space<SP_CELL>(k_sst_cell, P, st);
+        if (p.mixed) launch_space<SP_CELL>(k_sst_cell<float>, P, st);
+        else launch_space<SP_CELL>(k_sst_cell<double>, P, st);
         int m3 = P.mx + 3;
         for (int f = 0; f < 6; f++)
             hipLaunchKernelGGL(k_sst_ghost, dim3((unsigned)((m3 + 63) / 64), (unsigned)m3, (unsigned)P.nb), dim3(64), 0, st, bset(P, SP_CELL), f);
         launch_subs(k_amut, P, P.amut, st);
-        launch_space<SP_FI>(k_sst_face<0>, P, st);
-        launch_space<SP_FJ>(k_sst_face<1>, P, st);
-        launch_space<SP_FK>(k_sst_face<2>, P, st);
+        if (p.mixed) {
+            launch_space<SP_FI>(k_sst_face<0, float>, P, st);
+            launch_space<SP_FJ>(k_sst_face<1, float>, P, st);
+            launch_space<SP_FK>(k_sst_face<2, float>, P, st);
+        } else {
+            launch_space<SP_FI>(k_sst_face<0, double>, P, st);
+            launch_space<SP_FJ>(k_sst_face<1, double>, P, st);
+            launch_space<SP_FK>(k_sst_face<2, double>, P, st);
+        }
         launch_space<SP_CELL>(k_sst_res, P, st);
         launch_space<SP_N0>(k_sst_finish, P, st);
         launch_subs(k_amut, P, P.amut, st);

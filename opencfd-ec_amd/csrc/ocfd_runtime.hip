@@ -61,6 +61,10 @@ struct Engine {
     std::vector<void*> lu_allocs;
     SetPlan set{};                            // batched-launch tables (built on first use)
     std::vector<void*> set_allocs;
+    // plain mode: every kernel launched once per block, eagerly (no graphs), the LU-SGS sweeps on
+    // 8x8 tiles; parts[b] holds block b's own tables and shares this engine's stream
+    bool per_block = false;
+    std::vector<Engine> parts;
 };
 
 // one block of an engine's list, the parameters of its case and the list position of its
@@ -300,6 +304,18 @@ void read_subs(HostBlock& hb, int nsub, const int32_t* a) {
 // block-major loop.
 void enqueue_stage(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, int stage) {
     const SetPlan& P = e.set;
+    if (e.per_block) {
+        // plain mode: block after block; the halo copies of the whole list in one launch
+        for (size_t b = 0; b < e.parts.size(); b++) {
+            if (stage == OCFD_STAGE_BC_HALO) set_bc(e.parts[b].set, e.st);
+            else enqueue_stage(e.parts[b], {refs[b]}, p0, stage);
+        }
+        if (stage == OCFD_STAGE_BC_HALO) {
+            set_halo(P, e.st);
+            for (Engine& q : e.parts) set_force_vt_kw(q.set, e.st);
+        }
+        return;
+    }
     switch (stage) {
     case OCFD_STAGE_PRE:      // Set_Un is implicit (U is not modified before the update)
         set_pre(P, e.st);
@@ -339,7 +355,7 @@ void drop_graphs(Engine& e) {
 
 // One stage of the iteration: eager launches, or the stage's graph (captured on first use).
 int run_stage(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, int stage) {
-    if (!e.use_graph) { enqueue_stage(e, refs, p0, stage); return OCFD_OK; }
+    if (!e.use_graph || e.per_block) { enqueue_stage(e, refs, p0, stage); return OCFD_OK; }
     if (!e.exec[stage]) {
         HIPCHK(hipStreamBeginCapture(e.st, hipStreamCaptureModeThreadLocal));
         enqueue_stage(e, refs, p0, stage);
@@ -368,6 +384,8 @@ void drop_plans(Engine& e) {
     drop_graphs(e);
     drop_lu_plan(e);
     drop_set_plan(e);
+    for (Engine& q : e.parts) drop_plans(q);      // the parts share the stream: nothing else to free
+    e.parts.clear();
 }
 
 template <typename T>
@@ -541,6 +559,23 @@ int build_set_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) 
 int engine_prepare(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, bool lu) {
     int r = build_set_plan(e, refs, p0);
     if (r) return r;
+    if (e.per_block) {
+        if (e.parts.size() != refs.size()) {
+            for (Engine& q : e.parts) drop_plans(q);
+            e.parts.clear();
+            e.parts.resize(refs.size());
+            for (Engine& q : e.parts) { q.st = e.st; q.lu_schedule = e.lu_schedule; q.lu_tj = 8; }
+        }
+        for (size_t b = 0; b < refs.size(); b++) {
+            Engine& q = e.parts[b];
+            q.lu_fail = e.lu_fail;
+            // a one-block list: its inner links point outside it, and the halo runs on the whole list
+            BlockRef one = refs[b];
+            one.base = 0;
+            if ((r = engine_prepare(q, {one}, p0, lu))) return r;
+        }
+        return OCFD_OK;
+    }
     if (lu && e.lu_schedule != OCFD_LU_HYPERPLANE) {
         if ((r = build_lu_plan(e, refs, p0))) return r;
         // once a sweep gives up, the update kernels of that and later iterations leave U as it is
@@ -552,6 +587,9 @@ int engine_prepare(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, 
 int engine_finish(Engine& e) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e.st));
+    int r = OCFD_OK;
+    for (Engine& q : e.parts) { int rq = check_lu_error(q); if (rq && !r) r = rq; }
+    if (r) return r;
     return check_lu_error(e);
 }
 
@@ -590,9 +628,15 @@ int engine_step(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, int
     return OCFD_OK;
 }
 
+void drop_parts(Engine& e) {
+    for (Engine& q : e.parts) drop_plans(q);
+    e.parts.clear();
+}
+
 int engine_set_lu_tile(Engine& e, int tj) {
     if (!lusgs_tile_ok(tj)) return fail(OCFD_EARG, "LU-SGS tile width must be 4, 8 or 16");
-    if (tj != e.lu_tj) { drop_graphs(e); drop_lu_plan(e); e.lu_tj = tj; }
+    if (e.per_block && tj != 8) return fail(OCFD_EARG, "the per-block launch mode keeps 8x8 tiles");
+    if (tj != e.lu_tj) { drop_graphs(e); drop_lu_plan(e); drop_parts(e); e.lu_tj = tj; }
     return OCFD_OK;
 }
 
@@ -600,7 +644,7 @@ int engine_set_lu_schedule(Engine& e, int schedule) {
     if (schedule != OCFD_LU_PIPELINED && schedule != OCFD_LU_HYPERPLANE && schedule != OCFD_LU_PIPELINED_FENCED)
         return fail(OCFD_EARG, "unknown LU-SGS schedule");
     // the fenced sweep plans 8x8 tiles, the team sweep the engine's tile width
-    if (schedule != e.lu_schedule) { drop_graphs(e); drop_lu_plan(e); e.lu_schedule = schedule; }
+    if (schedule != e.lu_schedule) { drop_graphs(e); drop_lu_plan(e); drop_parts(e); e.lu_schedule = schedule; }
     return OCFD_OK;
 }
 
@@ -610,6 +654,7 @@ int engine_set_lu_schedule(Engine& e, int schedule) {
 // 3 niter plain iterations.
 int engine_autotune(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, int niter, int* best, double* ms3) {
     if (e.lu_schedule != OCFD_LU_PIPELINED) return fail(OCFD_EARG, "tile shapes apply to the pipelined LU-SGS schedule");
+    if (e.per_block) return fail(OCFD_EARG, "the per-block launch mode keeps 8x8 tiles");
     if (niter < 1) return fail(OCFD_EARG, "niter < 1");
     static const int shapes[3] = {4, 8, 16};
     double t[3];
@@ -790,8 +835,6 @@ int ocfd_create(const ocfd_params* p, ocfd_ctx** out) {
     if (p->precision != 0 && p->precision != 1) return fail(OCFD_EARG, "precision must be 0 (all FP64) or 1 (FP32 reconstruction, fluxes and SA terms)");
     if (p->IF_TurboMachinary != 0) return fail(OCFD_EUNSUPPORTED, "the turbomachinery mode (IF_TurboMachinary = 1) is outside this path");
     if (p->IF_InnerFlow != 0 && p->IF_InnerFlow != 1) return fail(OCFD_EARG, "IF_InnerFlow must be 0 or 1");
-    if (p->precision == 1 && p->If_viscous == 1 && p->Iflag_turbulence_model == 3)
-        return fail(OCFD_EUNSUPPORTED, "precision 1 covers the laminar and SA paths; SST runs in FP64 (precision 0)");
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (ndev < 1) return fail(OCFD_EHIP, "no HIP device");
@@ -1251,10 +1294,25 @@ int ocfd_set_lu_fail_waits(ocfd_ctx* c, int32_t on) {
     if (!c) return fail(OCFD_EARG, "null context");
     Engine& e = c->e;
     e.lu_fail = on ? 1 : 0;
-    if (e.lu.err) {     // the sweeps read the word at run time: captured graphs stay valid
-        HIPCHK(hipStreamSynchronize(e.st));
-        HIPCHK(hipMemcpy(e.lu.err + 2, &e.lu_fail, sizeof(int), hipMemcpyHostToDevice));
+    // the sweeps read the word at run time: captured graphs stay valid
+    HIPCHK(hipStreamSynchronize(e.st));
+    if (e.lu.err) HIPCHK(hipMemcpy(e.lu.err + 2, &e.lu_fail, sizeof(int), hipMemcpyHostToDevice));
+    for (Engine& q : e.parts) {
+        q.lu_fail = e.lu_fail;
+        if (q.lu.err) HIPCHK(hipMemcpy(q.lu.err + 2, &q.lu_fail, sizeof(int), hipMemcpyHostToDevice));
     }
+    return OCFD_OK;
+}
+
+int ocfd_set_per_block(ocfd_ctx* c, int32_t on) {
+    if (!c) return fail(OCFD_EARG, "null context");
+    Engine& e = c->e;
+    if ((on != 0) == e.per_block) return OCFD_OK;
+    if (on && e.lu_tj != 8) { int r = engine_set_lu_tile(e, 8); if (r) return r; }
+    HIPCHK(hipStreamSynchronize(e.st));
+    drop_graphs(e);
+    drop_parts(e);
+    e.per_block = on != 0;
     return OCFD_OK;
 }
 

@@ -57,7 +57,7 @@ EXPORTS = [
     "ocfd_step_lusgs", "ocfd_residual", "ocfd_update_limit", "ocfd_bc", "ocfd_halo", "ocfd_force_vt_kw",
     "ocfd_res_norms", "ocfd_res_norms_noalloc", "ocfd_forces", "ocfd_block_status", "ocfd_get_counters", "ocfd_set_graph", "ocfd_sync",
     "ocfd_step_timed", "ocfd_set_lu_schedule", "ocfd_set_lu_tile", "ocfd_get_lu_tile", "ocfd_autotune_lu",
-    "ocfd_set_lu_fail_waits",
+    "ocfd_set_lu_fail_waits", "ocfd_set_per_block",
     "ocfd_batch_create", "ocfd_batch_destroy", "ocfd_batch_add", "ocfd_batch_remove", "ocfd_batch_step",
     "ocfd_batch_set_graph", "ocfd_batch_set_lu_tile", "ocfd_batch_autotune_lu", "ocfd_batch_step_timed", "ocfd_mem_bytes",
     "ocfd_check_division",
@@ -100,6 +100,7 @@ def lib():
         L.ocfd_set_lu_schedule.argtypes = [vp, i32]
         L.ocfd_set_lu_tile.argtypes = [vp, i32]
         L.ocfd_set_lu_fail_waits.argtypes = [vp, C.c_int32]
+        L.ocfd_set_per_block.argtypes = [vp, C.c_int32]
         L.ocfd_get_lu_tile.argtypes = [vp, ip]
         L.ocfd_autotune_lu.argtypes = [vp, i32, ip, dp]
         L.ocfd_batch_set_lu_tile.argtypes = [vp, i32]
@@ -351,6 +352,10 @@ class Solver:
     def set_lu_schedule(self, schedule: int): _check(lib().ocfd_set_lu_schedule(self._h, schedule))
     def set_lu_tile(self, tj: int): _check(lib().ocfd_set_lu_tile(self._h, tj))
     def set_lu_fail_waits(self, on: bool): _check(lib().ocfd_set_lu_fail_waits(self._h, int(bool(on))))
+
+    def set_per_block(self, on: bool):
+        """Plain launch mode: every kernel once per block, eagerly, 8x8 LU-SGS tiles."""
+        _check(lib().ocfd_set_per_block(self._h, int(bool(on))))
 
     def lu_tile(self) -> int:
         v = C.c_int32()

@@ -533,15 +533,25 @@ def seeded_cuts(n_nodes: int, parts: int, rng: np.random.Generator, lo: int = 17
     raise RuntimeError("no admissible split found")
 
 
-def multiblock_airfoil(seed: int = 3, scale: float = 1.0) -> Case:
+def multiblock_airfoil(seed: int = 3, scale: float = 1.0, **params) -> Case:
     """48-block uneven grid: the 577x193x129 C-grid around a NACA 4412 section (span 1) cut 6x4x2
     at seeded planes, every block dim within [17, 161] nodes (14,155,776 cells); SST, Re 3e6,
-    Mach 0.7, MUSCL3 + AUSMPW (flux 6).  `scale` < 1 shrinks the parent for tests."""
+    Mach 0.7, MUSCL3 + AUSMPW (flux 6), CFL 0.1.  `scale` < 1 shrinks the parent for tests.
+
+    CFL 0.1: from freestream the reference's SST diverges at CFL 1 to 5 on this grid (2e-6 wall
+    spacing).  In a wall-adjacent cell the omega destruction term -beta rho omega^2 has a Jacobian
+    about 60 times the LU-SGS diagonal there (the sweep treats the source explicitly), so omega
+    overshoots below zero; the boundary condition is applied before force_vt_kw clamps it
+    (NS_Time_advance, sub_time_advance.f90:6-23; wall_bound, sub_boundary.f90:405-454), so the
+    ghost takes omega_g = 2 rho wt - U7 with the unclamped negative U7, which feeds back a larger
+    omega, and |omega| squares every two iterations until it overflows (iteration 8 to 10).  At
+    CFL 0.1 the k and omega residuals decay (the oracle over 100 iterations at 1/4 and 1/2 scale)."""
     def n(v):
         return v if scale == 1.0 else max(33, int(round((v - 1) * scale)) // 32 * 32 + 1)
     ni, nj, nk = n(577), n(193), n(129)
-    parent = naca_cgrid(ni, nj, nk, m=0.04, p=0.4, t=0.12, span=1.0, Iflag_turbulence_model=3, Re=3e6,
-                        Ma=0.7, Iflag_Scheme=5, Iflag_Flux=6)
+    args = dict(Iflag_turbulence_model=3, Re=3e6, Ma=0.7, Iflag_Scheme=5, Iflag_Flux=6, CFL=0.1)
+    args.update(params)
+    parent = naca_cgrid(ni, nj, nk, m=0.04, p=0.4, t=0.12, span=1.0, **args)
     rng = np.random.default_rng(seed)
     lo, hi = (17, 161) if scale == 1.0 else (5, 161)
     cuts = [[seeded_cuts(ni, 6, rng, lo, hi), seeded_cuts(nj, 4, rng, lo, hi), seeded_cuts(nk, 2, rng, lo, hi)]]

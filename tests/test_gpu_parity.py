@@ -6,6 +6,9 @@ library builds the geometry from node coordinates (ocfd_add_block_nodes + ocfd_s
 Tolerance: <= 1e-11 relative per cell with the floor 1e-8 * max|ref| (FP64 drop-in norm);
 transcendental-free cases through (a) must be bit-exact.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -161,6 +164,47 @@ def test_channel_97x33x9_2000_iterations(path):
     s.close()
 
 
+CONVERGED_SA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "converged_sa_cgrid_65x25x9.json")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", ["dropin", "device_geometry", "mixed"])
+def test_converged_sa_forces_match_reference(path):
+    """Converged forces of a turbulent case: the 65x25x9 NACA 0012 C-grid (wall spacing 0.002, SA,
+    Re 1e5, Mach 0.5, AoA 2 deg, MUSCL3 + Van Leer, CFL 10) run 30000 iterations from freestream,
+    by which point Res_rms(1) has dropped by about 2e-11.  CL and CD are held to the compiled
+    reference's (tests/golden/make_converged.py), the moments Ft0(7:9) to the oracle's, within 1e-8
+    relative in FP64 (drop-in and device-geometry paths) and within 1e-4 in mixed precision (FP32
+    reconstruction, fluxes and SA terms)."""
+    fx = json.load(open(CONVERGED_SA))
+    case = C.naca_cgrid(65, 25, 9, **fx["case_args"])
+    if path == "dropin":
+        o = oracle.Oracle(case)
+        o.init_flow()
+        s = A.Solver(case.params)
+        for no, (b, t) in enumerate(zip(case.blocks, A.inc_tables(case)), start=1):
+            s.add_block_geom(no, b.dims, t, oracle_geometry(o, no, b.dims, True))
+        s.init_flow(); s.bc(); s.halo()
+    else:
+        s = A.solver_from_case(case, precision=1 if path == "mixed" else 0)
+    s.set_graph(True)
+    s.step(1)
+    r0 = float(s.norms()[1][0])
+    s.step(fx["iterations"] - 1)
+    drop = float(s.norms()[1][0]) / r0
+    f = s.forces()
+    ref, orc = fx["reference"], fx["oracle"]
+    got = {"CL": f[9], "CD": f[10], "M1": f[6], "M2": f[7], "M3": f[8]}
+    want = {"CL": ref["CL"], "CD": ref["CD"], "M1": orc["Ft"][6], "M2": orc["Ft"][7], "M3": orc["Ft"][8]}
+    tol = 1e-4 if path == "mixed" else 1e-8
+    print(f"\n[{path}] drop {drop:.2e}  " + "  ".join(f"{k} {got[k]:.15f}/{want[k]:.15f}" for k in got), flush=True)
+    if path != "mixed":
+        assert drop < 1e-9
+    for k in got:
+        assert abs(got[k] - want[k]) <= tol * abs(want[k]), (k, got[k], want[k])
+    s.close()
+
+
 @pytest.mark.gpu
 def test_multiblock_sst_from_freestream_matches_oracle():
     """The 48-block SST generator (the 577x193x129 NACA 4412 C-grid cut 6x4x2) at a quarter of its
@@ -268,6 +312,33 @@ def test_lu_schedules_agree(name, schedule):
         for no in range(1, len(case.blocks) + 1):
             assert np.array_equal(a.download_U(no), b.download_U(no)), f"iteration {it}: block {no} differs"
             assert np.array_equal(a.field(no, A.F_DU), b.field(no, A.F_DU))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["channel_cut6", "cgrid_sa_cut12", "cgrid_sst_muscl3_ausm"])
+def test_per_block_launches_equal_batched(name):
+    """The plain launch mode (every kernel once per block, eagerly, 8x8 LU-SGS tiles) and the
+    batched mode (one launch per kernel over all blocks, stage graphs) give the same state bit for
+    bit, iteration after iteration; so do the fenced and hyperplane sweeps in the plain mode."""
+    print(f"\n[{name}]", flush=True)
+    cs = CASES[name]
+    case = cs.builder()
+    U0 = C.perturbed_state(case, seed=cs.seed, amp=cs.amp)
+    a = A.solver_from_case(case, states=U0)
+    b = A.solver_from_case(case, states=U0)
+    a.set_graph(True)
+    b.set_per_block(True)
+    for it in range(3):
+        a.step(1); b.step(1)
+        for no in range(1, len(case.blocks) + 1):
+            assert np.array_equal(a.download_U(no), b.download_U(no)), f"iteration {it}: block {no} differs"
+    for sched in (A.LU_PIPELINED_FENCED, A.LU_HYPERPLANE):
+        b.set_lu_schedule(sched)
+        a.step(1); b.step(1)
+        for no in range(1, len(case.blocks) + 1):
+            assert np.array_equal(a.download_U(no), b.download_U(no)), f"schedule {sched}: block {no} differs"
+    assert np.array_equal(a.norms()[1], b.norms()[1])
+    a.close(); b.close()
 
 
 @pytest.mark.gpu

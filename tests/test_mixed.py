@@ -1,5 +1,5 @@
-"""Mixed precision (precision=1): reconstruction, face fluxes and the SA terms in FP32; the
-residual assembly, spectral radii, LU-SGS, update, limiter and norms in FP64.
+"""Mixed precision (precision=1): reconstruction, face fluxes and the SA and SST terms in FP32;
+the residual assembly, spectral radii, LU-SGS, update, limiter and norms in FP64.
 
 The FP64 path is the drop-in and is held to the reference bit for bit elsewhere.  The mixed path
 is held to what FP32 arithmetic allows:
@@ -41,7 +41,8 @@ def _res(case, U0, precision):
 
 
 @pytest.mark.parametrize("name", ["channel_roe_muscl3", "channel_sw_ud3", "channel_vl_muscl2c", "channel_roe_weno5_char",
-                                  "channel_roe_ud5_cons", "cgrid_sa_weno5_roe", "cgrid_laminar_vl", "cgrid_sa_cut12"])
+                                  "channel_roe_ud5_cons", "cgrid_sa_weno5_roe", "cgrid_laminar_vl", "cgrid_sa_cut12",
+                                  "cgrid_sst_muscl3_ausm"])
 def test_mixed_residual_tracks_fp64(name):
     cs = CASES[name]
     case = cs.builder()
@@ -82,12 +83,6 @@ def test_mixed_converged_forces_match_fp64():
     for q in (9, 10):                                   # CL, CD
         assert abs(f64[q] - f_prev[q]) < 1e-5 * abs(f64[q])       # the FP64 forces have settled
         assert abs(f32[q] - f64[q]) <= FORCE_TOL * abs(f64[q]), (q, f32[q], f64[q])
-
-
-def test_mixed_refuses_sst():
-    case = CASES["cgrid_sst_muscl3_ausm"].builder()
-    with pytest.raises(A.OcfdError, match="precision"):
-        A.Solver(case.params, precision=1)
 
 
 def test_batch_needs_one_precision():

@@ -59,7 +59,7 @@ MEASURED_TRAFFIC = {("naca0012_cgrid_257x97x33", "fp64"): {"residual": 3163.2, "
                     ("swept_wing_24block", "fp64"): {"lu_forward": 546.4}}
 # stages that hold exactly one kernel launch per iteration: (kernel, the pass whose per-cell words
 # are its algorithmic bytes)
-SINGLE_KERNEL_STAGES = {"lu_forward": ("k_lusgs_team<{nv}, true, {tj}>", 1)}
+SINGLE_KERNEL_STAGES = {"lu_forward": ("k_lusgs_team<{nv}, true, {tj}>", 1), "lu_backward": ("k_lusgs_team<{nv}, false, {tj}>", 2)}
 FLUX_NAMES = {1: "Steger-Warming", 2: "HLL", 3: "HLLC", 4: "Roe", 5: "Van Leer", 6: "AUSMPW"}
 SCHEME_NAMES = {5: "MUSCL3", 7: "WENO5"}
 
@@ -194,36 +194,34 @@ def cpu_baseline(case, what: str, iters: int):
                 sample=sample + " (oracle C restatement, serial loops)")
 
 
+def _plain_solver(case):
+    """The plain GPU solver of one request: per-block eager launches, 8x8 LU-SGS tiles."""
+    s = A.solver_from_case(case)
+    s.set_per_block(True)
+    return s
+
+
 def serve_main(args) -> int:
     """The serving sweep: requests arrive (Poisson), are admitted against the HBM budget and
     advance together through shared launches until their Res_rms(1) has dropped by --drop; the
-    baseline runs the plain solver on one request at a time to the same criterion."""
+    baselines run one request at a time to the same criterion, with the solver's batched launches
+    over the request's blocks and graphs, and in the plain form (per-block eager launches).
+    --serve-phase splits the served run and the baselines over separate processes."""
     from ocfd_amd import serve as SV
     dims = SV.DIMS if args.serve_dims == "small" else SV.DIMS_LARGE
     reqs = SV.sweep_requests(args.serve, lam=args.rate, seed=5, dims=dims)
-    srv = SV.Server(mem_budget=args.serve_budget * 1e9, round_iters=args.round_iters, drop=args.drop,
-                    max_iters=args.max_iters, graph=bool(args.graph))
     log = (lambda m: print("# " + m, file=sys.stderr, flush=True)) if args.verbose else None
-    t0 = time.perf_counter()
-    res = srv.run(reqs, log=log)
-    wall = time.perf_counter() - t0
-    served = SV.summarize(res, wall)
-    print(f"# served: {json.dumps(served)}", file=sys.stderr, flush=True)
-    k = min(args.seq_sample, len(reqs))
-    sub = sorted(reqs, key=lambda r: r.arrival)[:k]
-    # the baseline starts each request as soon as the previous one is done (no idle gaps): its
-    # solves/hour is an upper bound for the sequential solver at this arrival rate
-    sub = [SV.Request(r.rid, 0.0, r.case, r.cells) for r in sub]
-    t0 = time.perf_counter()
-    seq = srv.run_sequential(sub, log=log)
-    swall = time.perf_counter() - t0
-    sequential = SV.summarize(seq, swall)
-    same = {r.rid: r for r in res}
-    dev = [max(abs(q.CL - same[q.rid].CL) / max(abs(q.CL), 1e-12), abs(q.CD - same[q.rid].CD) / max(abs(q.CD), 1e-12))
-           for q in seq if q.rid in same]
+
+    def server(make_solver=A.solver_from_case, graph=bool(args.graph)):
+        return SV.Server(mem_budget=args.serve_budget * 1e9, round_iters=args.round_iters, drop=args.drop,
+                         max_iters=args.max_iters, graph=graph, make_solver=make_solver)
+
+    def rounded(d):
+        return {k2: (round(v, 4) if isinstance(v, float) else v) for k2, v in d.items()}
+
     out = {
         "metric": "sweep solves/hour with p50/p99 latency",
-        "value": round(served["solves_per_hour"], 2),
+        "value": None,
         "unit": "solves/h",
         "n_gpus": 1,
         "higher_is_better": True,
@@ -232,13 +230,49 @@ def serve_main(args) -> int:
         "data": "synthetic requests generated in-process (NACA 4-digit C-grids, random Mach/AoA/Re)",
         "config": {"workload": f"naca_sweep_{args.serve}", "requests": args.serve, "arrival_rate_per_s": args.rate,
                    "dims": [list(d) for d in dims], "drop": args.drop, "round_iters": args.round_iters,
-                   "max_iters": args.max_iters, "mem_budget_GB": args.serve_budget},
-        "served": {k2: (round(v, 4) if isinstance(v, float) else v) for k2, v in served.items()},
-        "sequential": dict({k2: (round(v, 4) if isinstance(v, float) else v) for k2, v in sequential.items()},
-                           sample=f"the first {k} requests by arrival, back to back"),
-        "speedup_solves_per_hour": round(served["solves_per_hour"] / max(sequential["solves_per_hour"], 1e-9), 3),
-        "served_vs_sequential_max_rel_force_diff": max(dev) if dev else None,
+                   "max_iters": args.max_iters, "mem_budget_GB": args.serve_budget, "phase": args.serve_phase},
     }
+    res, served_ref = [], None
+    if args.serve_compare:
+        # the served run of an earlier process (--serve-phase served): its forces and solves/hour
+        with open(args.serve_compare) as f:
+            served_ref = json.load(f)
+        out["served"] = served_ref["served"]
+        out["value"] = served_ref["value"]
+        out["served_forces"] = served_ref["served_forces"]
+    if args.serve_phase in ("both", "served"):
+        srv = server()
+        t0 = time.perf_counter()
+        res = srv.run(reqs, log=log)
+        wall = time.perf_counter() - t0
+        served = SV.summarize(res, wall)
+        served.update(peak_active=srv.peak_active, peak_GB=round(srv.peak_used / 1e9, 2))
+        print(f"# served: {json.dumps(served)}", file=sys.stderr, flush=True)
+        out["served"] = rounded(served)
+        out["value"] = round(served["solves_per_hour"], 2)
+        out["served_forces"] = {r.rid: [r.CL, r.CD, r.iterations] for r in res}
+    if args.serve_phase in ("both", "sequential"):
+        k = min(args.seq_sample, len(reqs))
+        # the baseline starts each request as soon as the previous one is done (no idle gaps): its
+        # solves/hour is an upper bound for the sequential solver at this arrival rate
+        sub = [SV.Request(r.rid, 0.0, r.case, r.cells) for r in sorted(reqs, key=lambda r: r.arrival)[:k]]
+        kinds = {"sequential": (A.solver_from_case, bool(args.graph)), "sequential_plain": (_plain_solver, False)}
+        for name in args.seq_kinds.split(","):
+            make, graph = kinds[name]
+            t0 = time.perf_counter()
+            seq = server(make, graph).run_sequential(sub, log=log)
+            swall = time.perf_counter() - t0
+            sq = SV.summarize(seq, swall)
+            print(f"# {name}: {json.dumps(sq)}", file=sys.stderr, flush=True)
+            out[name] = dict(rounded(sq), sample=f"the first {k} requests by arrival, back to back",
+                             forces={r.rid: [r.CL, r.CD, r.iterations] for r in seq})
+            if "served_forces" in out:
+                same = {int(rid): v for rid, v in out["served_forces"].items()}
+                dev = [max(abs(q.CL - same[q.rid][0]) / max(abs(q.CL), 1e-12), abs(q.CD - same[q.rid][1]) / max(abs(q.CD), 1e-12))
+                       for q in seq if q.rid in same]
+                out[name]["iterations_equal_served"] = all(q.iterations == same[q.rid][2] for q in seq if q.rid in same)
+                out[name]["max_rel_force_diff_to_served"] = max(dev) if dev else None
+                out[name]["speedup_served"] = round(out["served"]["solves_per_hour"] / max(sq["solves_per_hour"], 1e-9), 3)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -315,7 +349,14 @@ def main() -> int:
     ap.add_argument("--round-iters", type=int, default=20, help="serving: iterations between admission/convergence checks")
     ap.add_argument("--max-iters", type=int, default=6000, help="serving: iteration cap per request")
     ap.add_argument("--serve-budget", type=float, default=200.0, help="serving: HBM budget of admitted requests, GB")
-    ap.add_argument("--seq-sample", type=int, default=32, help="serving: requests the sequential baseline runs")
+    ap.add_argument("--seq-sample", type=int, default=32, help="serving: requests the sequential baselines run")
+    ap.add_argument("--seq-kinds", default="sequential,sequential_plain",
+                    help="serving: the sequential baselines to run (sequential: batched over the request's blocks "
+                         "with graphs; sequential_plain: per-block eager launches)")
+    ap.add_argument("--serve-compare", default=None,
+                    help="serving: JSON line of an earlier --serve-phase served run to compare the baselines with")
+    ap.add_argument("--serve-phase", choices=("both", "served", "sequential"), default="both",
+                    help="serving: run the served sweep, the sequential baselines, or both")
     ap.add_argument("--serve-dims", choices=("small", "large"), default="small",
                     help="serving: request grids 97x33x9/129x49x13/193x65x17 (small) or 129x49x17/193x65x25/257x97x33 (large)")
     ap.add_argument("--verbose", action="store_true")
@@ -377,7 +418,7 @@ def main() -> int:
         passes[pname] = dict(ms=t, alg_bytes=alg, gbs=alg / (t * 1e-3) / 1e9)
     # the roofline's kernel: the largest kernel that runs alone in its stage, so the stage's
     # HIP events (on the library's stream) time exactly that kernel over the timed launches
-    singles = {k: v for k, v in SINGLE_KERNEL_STAGES.items() if not (k.startswith("lu_") and args.lu == "hyperplane")}
+    singles = {k: v for k, v in SINGLE_KERNEL_STAGES.items() if not (k.startswith("lu_") and args.lu != "pipelined")}
     stage_t = {k: stage_ms[k] / args.steps for k in singles}
     if stage_t:
         dom = max(stage_t, key=stage_t.get)

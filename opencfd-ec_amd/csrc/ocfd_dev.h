@@ -26,6 +26,7 @@ struct Blk {
     double *dt, *mu, *muf, *mut;        // mu: Sutherland; muf: mu after the turbulence-model (x*Re)/Re round trip
     double *prim[7];                     // rho u v w p T c (per iteration; limiter snapshot afterwards)
     double *Lc[3], *Lv[3];
+    double *al[3];                       // du_LU_SGS's diagonal of the cell pass (one, or three for SA), for the sweeps
     double *vol, *S[3], *nv[9], *met[4][9], *dw, *dtm, *x, *y, *z, *xc, *yc, *zc;
     double *F[3][5];                     // inviscid face flux, m = 1..5
     double *Fv[3][4];                    // viscous face flux, m = 2..5
@@ -102,9 +103,9 @@ __device__ __forceinline__ int xcd_tile(int t, int n) {
 
 constexpr int ROW_CHUNK = 16;
 
+// point `threadIdx.x` of tile t of the launch's iteration space
 template <int SP>
-__device__ __forceinline__ bool locate(const BSet& S, int& bi, int& i, int& j, int& k) {
-    const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+__device__ __forceinline__ bool locate_tile(const BSet& S, int t, int& bi, int& i, int& j, int& k) {
     bi = S.tile_blk[t];
     int n0, n1, n2, o0, o1, o2;
     space_dims(SP, S.blk[bi], n0, n1, n2, o0, o1, o2);
@@ -129,6 +130,11 @@ __device__ __forceinline__ bool locate(const BSet& S, int& bi, int& i, int& j, i
         k = o2 + full + rr % rem;
     }
     return true;
+}
+
+template <int SP>
+__device__ __forceinline__ bool locate(const BSet& S, int& bi, int& i, int& j, int& k) {
+    return locate_tile<SP>(S, xcd_tile((int)blockIdx.x, (int)gridDim.x), bi, i, j, k);
 }
 
 #define OCFD_LOCATE(SP)                                  \
@@ -191,11 +197,37 @@ struct LuPlan {
     int* prog_fwd = nullptr;     // last completed step per tile, -1 before the sweep
     int* prog_bwd = nullptr;
     int* err = nullptr;          // set when a wait exceeds its poll limit
-    // sweep-ordered copies of the sweep's inputs and of DU (layout in ocfd_lusgs.hip), P elements
+    // sweep-ordered copies of the sweep's inputs and of DU (layout below), P elements
     double* pk = nullptr;
     long long P = 0;
     int nwin = 0;                // i-windows per tile of the pack/unpack transposes
     int tj = 8;                  // tile width in j (tiles are tj x 64/tj (j,k) columns)
 };
+
+// lanes of a pipelined LU-SGS tile (one 64-wide wave per tile and sweep direction)
+constexpr int LU_NT = 64;
+
+// The sweep-ordered storage.  For a tile, step q (the cells its lanes visit at forward step q; the
+// backward sweep visits them at step nsteps-1-q) holds NF fields of 64 consecutive doubles, one
+// per lane: element (q, f, lane) at pofs + (q NF + f) 64 + lane.  A step's inputs therefore sit in
+// one contiguous run that a lane addresses from one pointer with constant offsets.  AL holds the
+// LU-SGS diagonal: alfa of components 1..5 (and of 6, 7 for SST) and, for SA only, the SA
+// component's alfa of the forward (AL+1) and of the backward sweep (AL+2), which sum the viscous
+// radii in different orders.  The buffer
+// has LU_PAD_STEPS steps of padding at both ends, so loads a few steps past either end of a tile
+// (prefetches of steps that do not exist, at most the larger tile side + 2 steps before an
+// upstream tile's first step) stay inside it.
+template <int NV> struct Pk {
+    static constexpr int U = 0, R = NV, DUF = 2 * NV, DUB = 3 * NV, LC = 4 * NV, LV = 4 * NV + 3, AL = 4 * NV + 6,
+                         S = 4 * NV + 9, N = 4 * NV + 12, NF = 4 * NV + 21;
+};
+constexpr int LU_PAD_STEPS = 24;
+constexpr int PK_W = 32;      // i-window of one workgroup of the sweep-order transposes and cell pass
+
+// Before a sweep every DU element it writes holds this NaN (a payload no arithmetic produces); a
+// consumer in a downstream tile reloads an element until it is something else.
+constexpr unsigned long long LU_SENTINEL = 0x7FF4C0DEDEADBEEFull;
+__device__ __forceinline__ bool is_sentinel(double v) { return (unsigned long long)__double_as_longlong(v) == LU_SENTINEL; }
+__device__ __forceinline__ double sentinel_value() { return __longlong_as_double((long long)LU_SENTINEL); }
 
 }  // namespace ocfd

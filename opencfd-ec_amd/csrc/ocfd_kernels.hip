@@ -595,10 +595,11 @@ __global__ void __launch_bounds__(64) k_face(BSet S) {
 // Res(1:5): inviscid i, j, k then viscous i, j, k face differences, in the reference order
 // (sub_Residual.f90:346-354,507-516,692-701,864-872,1039-1048,1219-1229)
 // followed by Res(6) of SA, then comput_Lijk + comput_dt (sub_time_acceleraction.f90:7-112)
-template <typename T>
-__global__ void __launch_bounds__(64) k_cell(BSet S) {
-    OCFD_LOCATE(SP_CELL)
-    size_t s = IX(b, i, j, k);
+// Res(m) goes to res(m, value) and the spectral radii of direction dir to lam(dir, Lc, Lv) as
+// soon as they are formed, so a caller's stores free their registers
+template <typename T, typename ResSink, typename LamSink>
+__device__ __forceinline__ void cell_assemble(const Blk& b, const Par& p, int i, int j, int k, size_t s, ResSink&& res,
+                                              LamSink&& lam, double* Lc, double* Lv, double& dt) {
     size_t sp[3] = {s + 1, s + (size_t)b.NI, s + (size_t)b.NI * b.NJ};
     bool visc = p.visc == 1 && b.flags[0] != 1;
 #pragma unroll
@@ -611,14 +612,13 @@ __global__ void __launch_bounds__(64) k_cell(BSet S) {
             r = (r + get_Fv<T>(b, 1, m - 1, sp[1])) - get_Fv<T>(b, 1, m - 1, s);
             r = (r + get_Fv<T>(b, 2, m - 1, sp[2])) - get_Fv<T>(b, 2, m - 1, s);
         }
-        b.Res[m][s] = r;
+        res(m, r);
     }
-    if (p.turb == 2) b.Res[5][s] = sa_residual<T>(b, p, i, j, k, s);
+    if (p.turb == 2) res(5, sa_residual<T>(b, p, i, j, k, s));
     const double g43 = p.gamma > (double)(4.0f / 3.0f) ? p.gamma : (double)(4.0f / 3.0f);
     double d = b.prim[0][s], u = b.prim[1][s], v = b.prim[2][s], w = b.prim[3][s], c = b.prim[6][s];
     double D0 = p.visc == 1 ? (g43 / d) * (b.muf[s] / p.PrL + b.mut[s] / p.PrT) : 0.0;
     double vol = b.vol[s], vol1 = 1.0 / vol;
-    double Lc[3], Lv[3];
 #pragma unroll
     for (int dir = 0; dir < 3; dir++) {
         double un = 0.5 * ((u * (b.nv[3 * dir][s] + b.nv[3 * dir][sp[dir]]) + v * (b.nv[3 * dir + 1][s] + b.nv[3 * dir + 1][sp[dir]]))
@@ -626,10 +626,8 @@ __global__ void __launch_bounds__(64) k_cell(BSet S) {
         double S0 = (b.S[dir][s] + b.S[dir][sp[dir]]) * 0.5;
         Lc[dir] = (fabs(un) + c) * S0;
         Lv[dir] = ((D0 * S0) * S0) * vol1;
-        b.Lc[dir][s] = Lc[dir];
-        b.Lv[dir][s] = Lv[dir];
+        lam(dir, Lc[dir], Lv[dir]);
     }
-    double dt;
     if (p.local_dt == 1) {
         double dt_fac = b.flags[0] == 0 ? 1.0 : 0.1;
         dt = ((dt_fac * p.CFL) * vol) / (((Lc[0] + Lc[1]) + Lc[2]) + 1.0 * ((Lv[0] + Lv[1]) + Lv[2]));
@@ -639,8 +637,33 @@ __global__ void __launch_bounds__(64) k_cell(BSet S) {
     } else {
         dt = p.dt_global;
     }
-    b.dt[s] = dt;
 }
+
+// the cell pass in box order: Res, Lc, Lv and dt into the block arrays
+template <typename T>
+__global__ void __launch_bounds__(64) k_cell(BSet S) {
+    OCFD_LOCATE(SP_CELL)
+    const size_t s = IX(b, i, j, k);
+    double Lc[3], Lv[3], dt;
+    cell_assemble<T>(
+        b, p, i, j, k, s, [&](int m, double r) { b.Res[m][s] = r; },
+        [&](int dir, double lc, double lv) { b.Lc[dir][s] = lc; b.Lv[dir][s] = lv; }, Lc, Lv, dt);
+    b.dt[s] = dt;
+    // du_LU_SGS's diagonal (sub_LU_SGS.f90:27-37 forward, :81-90 backward): of components 1..5 (and
+    // 6, 7 of SST), and for SA of the SA component in the forward and in the backward sweep, which
+    // sum the viscous radii in different orders
+    const double vol = b.vol[s];
+    const double sumLc = (Lc[0] + Lc[1]) + Lc[2], sumLv = (Lv[0] + Lv[1]) + Lv[2];
+    double alfa = vol / dt + p.w_LU * sumLc;
+    if (p.visc == 1) alfa = alfa + 2.0 * sumLv;
+    b.al[0][s] = alfa + 0.0 * vol;
+    if (p.NV == 6) {
+        const double a1 = p.visc == 1 ? alfa + sumLv : alfa, a2 = p.visc == 1 ? ((alfa + Lv[0]) + Lv[1]) + Lv[2] : alfa;
+        b.al[1][s] = a1 + 0.0 * vol;
+        b.al[2][s] = a2 + 0.0 * vol;
+    }
+}
+
 
 // ---------------------------------------------------------------------------------------------
 // du_LU_SGS (sub_LU_SGS.f90:4-129): cells of hyperplane i+j+k = plane, one thread per (j,k)
@@ -716,20 +739,107 @@ __global__ void k_lusgs_plane(Blk b, Par p, int plane) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// U = Un + DU (sub_time_advance.f90:92-101) and the limit_flow_U snapshot/detection
-// (sub_limitflow.f90:46-94).  The snapshot (rho,u,v,w,p) goes to prim[0..4].
-__global__ void __launch_bounds__(64) k_update_detect(BSet S) {
-    if (halted(S)) return;
-    OCFD_LOCATE(SP_CELL)
-    size_t s = IX(b, i, j, k);
+// U = Un + DU (sub_time_advance.f90:92-101), the limit_flow_U snapshot/detection
+// (sub_limitflow.f90:46-94) and, for SA, limit_flow_SA's detection (:182-231).  The snapshot
+// (rho,u,v,w,p) goes to prim[0..4]; U6 after the update goes to prim[5], U6 below SAmin is clamped
+// and U6 above LSAmax is flagged in the block's mask.  limit_flow_U changes only U(1:5) and the SA
+// pass only U6, so detecting both before either replaces anything gives the reference's order.
+__device__ __forceinline__ long sa_cell_index(const Blk& b, int i, int j, int k) {
+    return (long)(i - 1) + (long)(b.nx - 1) * ((long)(j - 1) + (long)(b.ny - 1) * (long)(k - 1));
+}
+__device__ __forceinline__ bool sa_flagged(const Blk& b, long c) { return (b.samask[c >> 6] >> (c & 63)) & 1ull; }
+
+__device__ __forceinline__ void update_cell(const Blk& b, const Par& p, int i, int j, int k, size_t s, const double* DU) {
     double U[MAXV];
-    for (int m = 0; m < p.NV; m++) { U[m] = b.U[m][s] + b.DU[m][s]; b.U[m][s] = U[m]; }
+    for (int m = 0; m < p.NV; m++) { U[m] = b.U[m][s] + DU[m]; b.U[m][s] = U[m]; }
     if (p.limit_flow != 1) return;
     double d1 = U[0], u1 = U[1] / d1, v1 = U[2] / d1, w1 = U[3] / d1;
     double p1 = (U[4] - (0.5 * d1) * ((u1 * u1 + v1 * v1) + w1 * w1)) * (p.gamma - 1.0);
     b.prim[0][s] = d1; b.prim[1][s] = u1; b.prim[2][s] = v1; b.prim[3][s] = w1; b.prim[4][s] = p1;
     bool bad = d1 < p.Ldmin || p1 < p.Lpmin || d1 > p.Ldmax || p1 > p.Lpmax || fabs(u1) > p.Lumax || fabs(v1) > p.Lumax || fabs(w1) > p.Lumax;
     if (bad) atomicAdd(&b.flags[1], 1);
+    if (p.NV == 6) {
+        const double v = U[5];
+        b.prim[5][s] = v;
+        if (v > p.LSAmax) {
+            atomicAdd(&b.flags[2], 1);
+            long c = sa_cell_index(b, i, j, k);
+            atomicOr(&b.samask[c >> 6], 1ull << (c & 63));
+        } else if (v < 1.0e-8) {
+            b.U[5][s] = 1.0e-8;
+        }
+    }
+}
+
+// the update in box order, DU from the block arrays
+__global__ void __launch_bounds__(64) k_update_detect(BSet S) {
+    if (halted(S)) return;
+    OCFD_LOCATE(SP_CELL)
+    const size_t s = IX(b, i, j, k);
+    double DU[MAXV];
+    for (int m = 0; m < p.NV; m++) DU[m] = b.DU[m][s];
+    update_cell(b, p, i, j, k, s, DU);
+}
+
+// The update of an iteration with the pipelined sweeps: the backward sweep's DU comes from the
+// sweep-ordered storage.  A workgroup takes one sweep tile's cells of an i-window (the tiles and
+// windows of that storage), UP_ROWS lanes of the tile per pass.  A pass reads its lanes' DU along
+// the steps (each step's UP_ROWS lanes are one contiguous run), puts the sentinel back into those
+// DU elements and into the forward DU of the same elements (the next sweeps find every element
+// they write holding it), and parks DU in LDS; then its threads update the pass's cells, i fastest,
+// and store DU in the block arrays (the host reads it there).  Lane l of a pass holds cell
+// i = q - d(l) + 1 at step q, d = jl + kl.  After a sweep gave up, the update leaves U, DU and the
+// storage as they are; the host puts the sentinel back into the whole storage.
+constexpr int UP_ROWS = 16, UP_T = UP_ROWS * PK_W;
+
+template <int NV, int TJ>
+__global__ void __launch_bounds__(UP_T) k_update_pk(BSet S, const LuTask* __restrict__ tiles, double* __restrict__ pk, int nwin) {
+    using F = Pk<NV>;
+    static_assert(UP_ROWS % TJ == 0, "a pass covers whole j rows of the tile");
+    constexpr int DSPAN = TJ - 1 + UP_ROWS / TJ - 1;          // spread of d over a pass's lanes
+    constexpr int SLOTS = (PK_W + DSPAN) * UP_ROWS;           // (step, lane) slots a pass can touch
+    __shared__ double buf[NV][UP_ROWS][PK_W + 1];
+    if (halted(S)) return;
+    const int tw = xcd_tile((int)blockIdx.x, (int)gridDim.x);     // (tile, i-window), windows fastest
+    const LuTask T_ = tiles[tw / nwin];
+    const Blk& b = S.blk[T_.blk];
+    const Par& p = S.par[T_.blk];
+    const int nxm = b.nx - 1;
+    const int i0 = 1 + (tw % nwin) * PK_W;
+    if (i0 > nxm) return;
+    const int W = min(PK_W, nxm - i0 + 1);
+    const int t = (int)threadIdx.x, rr = t / PK_W, c = t % PK_W;
+#pragma unroll 1
+    for (int lane0 = 0; lane0 < LU_NT; lane0 += UP_ROWS) {
+        const int kl0 = lane0 / TJ;
+        if (kl0 >= T_.tk) break;                        // the same for the whole workgroup
+        // step qb + qo: lane l (d = dmin + dl) holds cell i0 + qo - dl there
+        const int qb = i0 - 1 + kl0;
+        for (int slot = t; slot < SLOTS; slot += UP_T) {
+            const int l = slot % UP_ROWS, qo = slot / UP_ROWS;
+            const int jl = (lane0 + l) % TJ, kl = (lane0 + l) / TJ, ci = qo - (jl + kl - kl0);
+            if (jl < T_.tj && kl < T_.tk && ci >= 0 && ci < W) {
+                double* el = pk + T_.pofs + ((long long)(qb + qo) * F::NF + F::DUB) * LU_NT + lane0 + l;
+#pragma unroll
+                for (int m = 0; m < NV; m++) {
+                    buf[m][l][ci] = el[m * LU_NT];
+                    el[m * LU_NT] = sentinel_value();
+                    el[(F::DUF - F::DUB + m) * LU_NT] = sentinel_value();
+                }
+            }
+        }
+        __syncthreads();
+        const int jl = (lane0 + rr) % TJ, kl = (lane0 + rr) / TJ;
+        if (jl < T_.tj && kl < T_.tk && c < W) {
+            const int i = i0 + c, j = T_.j0 + jl, k = T_.k0 + kl;
+            const size_t s = IX(b, i, j, k);
+            double DU[MAXV];
+#pragma unroll
+            for (int m = 0; m < NV; m++) { DU[m] = buf[m][rr][c]; b.DU[m][s] = DU[m]; }
+            update_cell(b, p, i, j, k, s, DU);
+        }
+        __syncthreads();
+    }
 }
 
 __device__ __forceinline__ bool lim_bad(const Blk& b, const Par& p, size_t s) {
@@ -739,10 +849,9 @@ __device__ __forceinline__ bool lim_bad(const Blk& b, const Par& p, size_t s) {
 
 // limit_flow_U replacement step (sub_limitflow.f90:84-150): bad cells take the mean of the valid
 // 3x3x3 neighbours of the snapshot, or their own clamped values; sets the IF_OverLimit latch
-__global__ void __launch_bounds__(64) k_limit_apply(BSet S) {
-    if (halted(S)) return;
-    OCFD_LOCATE(SP_CELL)
-    if (b.flags[1] == 0) return;
+// (one launch of a few thousand workgroups looping over the cell tiles: when no block has a bad
+// cell, the usual case, it ends within microseconds)
+__device__ __forceinline__ void limit_apply_cell(const Blk& b, const Par& p, int i, int j, int k) {
     if (i == 1 && j == 1 && k == 1) b.flags[0] = 1;
     size_t s = IX(b, i, j, k);
     if (!lim_bad(b, p, s)) return;
@@ -771,6 +880,17 @@ __global__ void __launch_bounds__(64) k_limit_apply(BSet S) {
     b.U[4][s] = p0 / (p.gamma - 1.0) + (0.5 * d0) * ((u0 * u0 + v0 * v0) + w0 * w0);
 }
 
+__global__ void __launch_bounds__(64) k_limit_apply(BSet S, int ntiles) {
+    if (halted(S)) return;
+    for (int t = (int)blockIdx.x; t < ntiles; t += (int)gridDim.x) {
+        int bi, i, j, k;
+        if (!locate_tile<SP_CELL>(S, t, bi, i, j, k)) continue;
+        const Blk& b = S.blk[bi];
+        if (b.flags[1] == 0) continue;
+        limit_apply_cell(b, S.par[bi], i, j, k);
+    }
+}
+
 // limit_flow_SA (sub_limitflow.f90:182-231).  The reference clamps and averages in place in
 // lexicographic (k, j, i) order, which in this layout is increasing linear index.  Cells below
 // SAmin are clamped in parallel (their original value is kept in prim[5], free after the
@@ -783,11 +903,6 @@ __global__ void __launch_bounds__(64) k_limit_apply(BSet S) {
 // values before it.  A flagged cell none of whose 26 neighbours is flagged reads only final values,
 // so those cells are done in parallel; the flagged cells with flagged neighbours are then done one
 // by one in cell order.  The detect kernel snapshots U6 into prim[5] and marks the flagged cells.
-__device__ __forceinline__ long sa_cell_index(const Blk& b, int i, int j, int k) {
-    return (long)(i - 1) + (long)(b.nx - 1) * ((long)(j - 1) + (long)(b.ny - 1) * (long)(k - 1));
-}
-__device__ __forceinline__ bool sa_flagged(const Blk& b, long c) { return (b.samask[c >> 6] >> (c & 63)) & 1ull; }
-
 __device__ __forceinline__ void sa_replace(const Blk& b, const Par& p, int i, int j, int k) {
     const double SAmin = 1.0e-8;
     double* U6 = b.U[5];
@@ -805,25 +920,7 @@ __device__ __forceinline__ void sa_replace(const Blk& b, const Par& p, int i, in
     U6[IX(b, i, j, k)] = kn == 0 ? p.LSAmax : s0 / (double)kn;
 }
 
-__global__ void __launch_bounds__(64) k_limit_sa_detect(BSet S) {
-    if (halted(S)) return;
-    OCFD_LOCATE(SP_CELL)
-    size_t s = IX(b, i, j, k);
-    double v = b.U[5][s];
-    b.prim[5][s] = v;
-    if (v > p.LSAmax) {
-        atomicAdd(&b.flags[2], 1);
-        long c = sa_cell_index(b, i, j, k);
-        atomicOr(&b.samask[c >> 6], 1ull << (c & 63));
-    } else if (v < 1.0e-8) {
-        b.U[5][s] = 1.0e-8;
-    }
-}
-
-__global__ void __launch_bounds__(64) k_limit_sa_isolated(BSet S) {
-    if (halted(S)) return;
-    OCFD_LOCATE(SP_CELL)
-    if (b.flags[2] == 0) return;
+__device__ __forceinline__ void limit_sa_isolated_cell(const Blk& b, const Par& p, int i, int j, int k) {
     long c = sa_cell_index(b, i, j, k);
     if (!sa_flagged(b, c)) return;
     for (int ka = -1; ka <= 1; ka++) for (int ja = -1; ja <= 1; ja++) for (int ia = -1; ia <= 1; ia++) {
@@ -835,6 +932,17 @@ __global__ void __launch_bounds__(64) k_limit_sa_isolated(BSet S) {
     sa_replace(b, p, i, j, k);
     // no flagged cell reads this cell's bit (it has no flagged neighbour), so it can go now
     atomicAnd(&b.samask[c >> 6], ~(1ull << (c & 63)));
+}
+
+__global__ void __launch_bounds__(64) k_limit_sa_isolated(BSet S, int ntiles) {
+    if (halted(S)) return;
+    for (int t = (int)blockIdx.x; t < ntiles; t += (int)gridDim.x) {
+        int bi, i, j, k;
+        if (!locate_tile<SP_CELL>(S, t, bi, i, j, k)) continue;
+        const Blk& b = S.blk[bi];
+        if (b.flags[2] == 0) continue;
+        limit_sa_isolated_cell(b, S.par[bi], i, j, k);
+    }
 }
 
 // one workgroup per block: the remaining (clustered) cells in cell order; clears the mask
@@ -1302,15 +1410,44 @@ void set_viscous_dt(const SetPlan& P, hipStream_t st) {
     else viscous_dt_t<double>(P, st);
 }
 
-void set_update_limit(const SetPlan& P, hipStream_t st) {
-    launch_space<SP_CELL>(k_update_detect, P, st);
+
+// limit_flow_U's replacement and limit_flow_SA after the update
+static void limit_after_update(const SetPlan& P, hipStream_t st) {
     if (P.p0.limit_flow != 1) return;
-    launch_space<SP_CELL>(k_limit_apply, P, st);
+    const int n = P.ntiles[SP_CELL];
+    if (n <= 0) return;
+    const dim3 grid((unsigned)std::min(n, 4096));
+    hipLaunchKernelGGL(k_limit_apply, grid, dim3(64), 0, st, bset(P, SP_CELL), n);
     if (P.p0.NV == 6) {
-        launch_space<SP_CELL>(k_limit_sa_detect, P, st);
-        launch_space<SP_CELL>(k_limit_sa_isolated, P, st);
+        hipLaunchKernelGGL(k_limit_sa_isolated, grid, dim3(64), 0, st, bset(P, SP_CELL), n);
         hipLaunchKernelGGL(k_limit_sa_ordered, dim3((unsigned)P.nb), dim3(64), 0, st, bset(P, SP_CELL));
     }
+}
+
+void set_update_limit(const SetPlan& P, hipStream_t st) {
+    launch_space<SP_CELL>(k_update_detect, P, st);
+    limit_after_update(P, st);
+}
+
+template <int NV, int TJ>
+static void update_pk(const SetPlan& P, const LuPlan& L, hipStream_t st) {
+    if (L.n_fwd > 0)
+        hipLaunchKernelGGL((k_update_pk<NV, TJ>), dim3((unsigned)(L.n_fwd * L.nwin)), dim3(UP_T), 0, st, bset(P, SP_CELL), L.fwd,
+                           L.pk, L.nwin);
+}
+void set_update_limit_sweep_order(const SetPlan& P, const LuPlan& L, hipStream_t st) {
+    switch (P.p0.NV * 100 + L.tj) {
+    case 504: update_pk<5, 4>(P, L, st); break;
+    case 508: update_pk<5, 8>(P, L, st); break;
+    case 516: update_pk<5, 16>(P, L, st); break;
+    case 604: update_pk<6, 4>(P, L, st); break;
+    case 608: update_pk<6, 8>(P, L, st); break;
+    case 616: update_pk<6, 16>(P, L, st); break;
+    case 704: update_pk<7, 4>(P, L, st); break;
+    case 708: update_pk<7, 8>(P, L, st); break;
+    default: update_pk<7, 16>(P, L, st); break;
+    }
+    limit_after_update(P, st);
 }
 
 void set_bc(const SetPlan& P, hipStream_t st) { launch_subs(k_bc, P, P.bc, st); }

@@ -38,6 +38,8 @@ void set_pre(const SetPlan& P, hipStream_t st);
 void set_inviscid(const SetPlan& P, hipStream_t st);
 void set_viscous_dt(const SetPlan& P, hipStream_t st);
 void set_update_limit(const SetPlan& P, hipStream_t st);
+// the update taking DU from the pipelined sweeps' storage (and putting the sentinel back)
+void set_update_limit_sweep_order(const SetPlan& P, const LuPlan& L, hipStream_t st);
 void set_bc(const SetPlan& P, hipStream_t st);
 void set_halo(const SetPlan& P, hipStream_t st);
 void set_force_vt_kw(const SetPlan& P, hipStream_t st);
@@ -64,8 +66,11 @@ bool lusgs_tile_ok(int tj);
 int lusgs_fields(int NV);
 int lusgs_windows(int max_cells_i);
 void launch_lusgs_pack_geometry(const LuPlan& L, const Par& p, hipStream_t st);
-void launch_lusgs_reset(const LuPlan& L, int NV, hipStream_t st);
+// the sweeps' per-iteration inputs (U, Res, Lc, Lv, the diagonal) into sweep order
 void launch_lusgs_pack(const LuPlan& L, const Par& p, hipStream_t st);
+void launch_lusgs_reset(const LuPlan& L, int NV, hipStream_t st);
+// the backward sweep's DU from the sweep-ordered storage into the block arrays
+void launch_lusgs_du_to_box(const LuPlan& L, int NV, hipStream_t st);
 void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st);
 // device geometry precompute (ocfd_geometry.hip)
 void launch_geometry(const Blk& b, const Sub* subs, int nsub, hipStream_t st);

@@ -24,7 +24,8 @@
 
 namespace ocfd {
 
-constexpr int LU_TJ = 8, LU_TK = 8, LU_NT = LU_TJ * LU_TK;
+constexpr int LU_TJ = 8, LU_TK = 8;
+static_assert(LU_TJ * LU_TK == LU_NT, "the fenced sweep's tiles are 8 x 8");
 constexpr long LU_POLL_LIMIT = 1L << 26;
 
 // err[0]: the error flag a wait sets when it gives up.  err[2]: nonzero makes every wait that
@@ -224,10 +225,6 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_sweep(const Blk* __restrict__ b
 // flag.
 
 constexpr int LT_NT = 3 * LU_NT;
-constexpr unsigned long long LU_SENTINEL = 0x7FF4C0DEDEADBEEFull;
-
-__device__ __forceinline__ bool is_sentinel(double v) { return (unsigned long long)__double_as_longlong(v) == LU_SENTINEL; }
-__device__ __forceinline__ double sentinel_value() { return __longlong_as_double((long long)LU_SENTINEL); }
 __device__ __forceinline__ void st_coherent(double* p, double v) {
     __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -237,59 +234,29 @@ __device__ __forceinline__ double ld_coherent(const double* p) {
                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
-// The sweep-ordered storage.  For a tile, step q (the cells its lanes visit at forward step q; the
-// backward sweep visits them at step nsteps-1-q) holds NF fields of 64 consecutive doubles, one
-// per lane: element (q, f, lane) at pofs + (q NF + f) 64 + lane.  A step's inputs therefore sit in
-// one contiguous run that a lane addresses from one pointer with constant offsets.  AL holds the
-// LU-SGS diagonal: alfa of components 1..5 (and of 6, 7 for SST), and of the SA component in the
-// forward and in the backward sweep, which sum the viscous radii in different orders.  The buffer
-// has LU_PAD_STEPS steps of padding at both ends, so loads a few steps past either end of a tile
-// (prefetches of steps that do not exist, at most the larger tile side + 2 steps before an
-// upstream tile's first step) stay inside it.
-template <int NV> struct Pk {
-    static constexpr int U = 0, R = NV, DUF = 2 * NV, DUB = 3 * NV, LC = 4 * NV, LV = 4 * NV + 3, AL = 4 * NV + 6,
-                         S = 4 * NV + 9, N = 4 * NV + 12, NF = 4 * NV + 21;
-};
-constexpr int LU_PAD_STEPS = 24;
 
 // value of field f of the sweep-ordered storage at box index s (DUF/DUB are never packed)
 template <int NV>
-__device__ __forceinline__ double pack_value(const Blk& b, const Par& p, int f, size_t s) {
+__device__ __forceinline__ double pack_value(const Blk& b, int f, size_t s) {
     using F = Pk<NV>;
     if (f < F::R) return b.U[f][s];
     if (f < F::DUF) return b.Res[f - F::R][s];
     if (f < F::LV) return b.Lc[f - F::LC][s];
     if (f < F::AL) return b.Lv[f - F::LV][s];
-    if (f < F::S) {
-        // du_LU_SGS (sub_LU_SGS.f90:27-37 forward, :81-90 backward)
-        const double Lv0 = b.Lv[0][s], Lv1 = b.Lv[1][s], Lv2 = b.Lv[2][s], vol = b.vol[s];
-        const double sumLc = (b.Lc[0][s] + b.Lc[1][s]) + b.Lc[2][s];
-        const double sumLv = (Lv0 + Lv1) + Lv2;
-        double alfa = vol / b.dt[s] + p.w_LU * sumLc;
-        if (p.visc == 1) alfa = alfa + 2.0 * sumLv;
-        double a = alfa;
-        if (p.visc == 1 && NV == 6) {
-            if (f == F::AL + 1) a = alfa + sumLv;
-            else if (f == F::AL + 2) a = ((alfa + Lv0) + Lv1) + Lv2;
-        }
-        return a + 0.0 * vol;
-    }
+    if (f < F::S) return b.al[f - F::AL][s];
     if (f < F::N) return b.S[f - F::S][s];
     return b.nv[f - F::N][s];
 }
 
-constexpr int PK_W = 32;      // i-window of one transpose workgroup
 constexpr int PK_T = 256;
 
-// box -> sweep order for fields [f0,f1) and [f2,f3), or (TO_PACKED false) field range [f0,f1) of
-// the sweep order (the backward DU) back into the box arrays of DU, writing the sentinel back
-// into it and into the forward DU; one tile, one i-window and one field per workgroup, through
-// LDS so both sides move in contiguous runs.  Neighbouring i-windows of a tile share the step
-// rows at their boundary (each writes some lanes of them), so a tile's windows run back to back
-// on one XCD, where the L2 merges those partial lines before they go to memory.
-template <int NV, bool TO_PACKED, int TJ>
+// box -> sweep order for fields [f0,f1) and then from f2 on (one per grid z); one tile, one
+// i-window and one field per workgroup, through LDS so both sides move in contiguous runs.  Neighbouring i-windows of a tile
+// share the step rows at their boundary (each writes some lanes of them), so a tile's windows run
+// back to back on one XCD, where the L2 merges those partial lines before they go to memory.
+template <int NV, int TJ>
 __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
-                                                       double* __restrict__ pk, long long P, Par p, int nwin, int f0, int f1, int f2, int f3) {
+                                                       double* __restrict__ pk, int nwin, int f0, int f1, int f2) {
     __shared__ double buf[LU_NT][PK_W + 1];
     const int tw = xcd_tile((int)blockIdx.x, (int)gridDim.x);     // (tile, i-window), windows fastest
     const LuTask T = tiles[tw / nwin];
@@ -301,40 +268,48 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
     const int nsteps = nxm + T.tj + T.tk - 2;
     const int t = threadIdx.x;
     constexpr int NF = Pk<NV>::NF;
-    // one field per workgroup (grid z): field z of [f0,f1) then [f2,f3)
     const int z = (int)blockIdx.z;
     const int f = z < f1 - f0 ? f0 + z : f2 + (z - (f1 - f0));
-    {
-        double* pf = pk + (size_t)T.pofs + (size_t)f * LU_NT;     // element (q, f, lane) at pf[q NF 64 + lane]
-        if (TO_PACKED) {
-            for (int e = t; e < LU_NT * PK_W; e += PK_T) {
-                int r = e / PK_W, c = e % PK_W, jl = r % TJ, kl = r / TJ;
-                if (jl < T.tj && kl < T.tk && c < W) buf[r][c] = pack_value<NV>(b, p, f, IX(b, i0 + c, T.j0 + jl, T.k0 + kl));
-            }
-            __syncthreads();
-            for (int e = t; e < (W + TJ + LU_NT / TJ - 2) * LU_NT; e += PK_T) {
-                int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % TJ, kl = lane / TJ;
-                int i = q - jl - kl + 1;
-                if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * NF * LU_NT + lane] = buf[lane][i - i0];
-            }
-        } else {
-            for (int e = t; e < (W + TJ + LU_NT / TJ - 2) * LU_NT; e += PK_T) {
-                int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % TJ, kl = lane / TJ;
-                int i = q - jl - kl + 1;
-                if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) {
-                    double* el = pf + (size_t)q * NF * LU_NT + lane;
-                    buf[lane][i - i0] = *el;
-                    *el = sentinel_value();
-                    el[(Pk<NV>::DUF - Pk<NV>::DUB) * LU_NT] = sentinel_value();     // the forward DU too
-                }
-            }
-            __syncthreads();
-            double* box = b.DU[f - Pk<NV>::DUB];
-            for (int e = t; e < LU_NT * PK_W; e += PK_T) {
-                int r = e / PK_W, c = e % PK_W, jl = r % TJ, kl = r / TJ;
-                if (jl < T.tj && kl < T.tk && c < W) box[IX(b, i0 + c, T.j0 + jl, T.k0 + kl)] = buf[r][c];
-            }
-        }
+    double* pf = pk + (size_t)T.pofs + (size_t)f * LU_NT;     // element (q, f, lane) at pf[q NF 64 + lane]
+    for (int e = t; e < LU_NT * PK_W; e += PK_T) {
+        int r = e / PK_W, c = e % PK_W, jl = r % TJ, kl = r / TJ;
+        if (jl < T.tj && kl < T.tk && c < W) buf[r][c] = pack_value<NV>(b, f, IX(b, i0 + c, T.j0 + jl, T.k0 + kl));
+    }
+    __syncthreads();
+    for (int e = t; e < (W + TJ + LU_NT / TJ - 2) * LU_NT; e += PK_T) {
+        int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % TJ, kl = lane / TJ;
+        int i = q - jl - kl + 1;
+        if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * NF * LU_NT + lane] = buf[lane][i - i0];
+    }
+}
+
+// sweep order -> block arrays for the backward sweep's DU (component m per grid z), for a host that
+// reads DU between the sweeps and the update; the storage keeps it, and the update reads it there
+template <int NV, int TJ>
+__global__ void __launch_bounds__(PK_T) k_lu_du_to_box(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
+                                                       const double* __restrict__ pk, int nwin) {
+    __shared__ double buf[LU_NT][PK_W + 1];
+    const int tw = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const LuTask T = tiles[tw / nwin];
+    const Blk& b = blks[T.blk];
+    const int nxm = b.nx - 1;
+    const int i0 = 1 + (tw % nwin) * PK_W;
+    if (i0 > nxm) return;
+    const int W = min(PK_W, nxm - i0 + 1);
+    const int nsteps = nxm + T.tj + T.tk - 2;
+    const int t = threadIdx.x, m = (int)blockIdx.z;
+    constexpr int NF = Pk<NV>::NF;
+    const double* pf = pk + (size_t)T.pofs + (size_t)(Pk<NV>::DUB + m) * LU_NT;
+    for (int e = t; e < (W + TJ + LU_NT / TJ - 2) * LU_NT; e += PK_T) {
+        int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % TJ, kl = lane / TJ;
+        int i = q - jl - kl + 1;
+        if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) buf[lane][i - i0] = pf[(size_t)q * NF * LU_NT + lane];
+    }
+    __syncthreads();
+    double* box = b.DU[m];
+    for (int e = t; e < LU_NT * PK_W; e += PK_T) {
+        int r = e / PK_W, c = e % PK_W, jl = r % TJ, kl = r / TJ;
+        if (jl < T.tj && kl < T.tk && c < W) box[IX(b, i0 + c, T.j0 + jl, T.k0 + kl)] = buf[r][c];
     }
 }
 
@@ -680,17 +655,13 @@ std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, int 
 template <int NV, int TJ>
 static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st) {
     if (!fenced) {
-        using F = Pk<NV>;
-        dim3 tgrid((unsigned)(L.n_fwd * L.nwin), 1u, (unsigned)NV);
         if (forward) {
             hipLaunchKernelGGL((k_lusgs_team<NV, true, TJ>), dim3((unsigned)L.n_fwd), dim3(LT_NT), 0, st,
                                L.blks, L.fwd, L.pk, L.err, p);
         } else {
+            // the update takes DU from the sweep-ordered storage (and puts the sentinel back)
             hipLaunchKernelGGL((k_lusgs_team<NV, false, TJ>), dim3((unsigned)L.n_bwd), dim3(LT_NT), 0, st,
                                L.blks, L.bwd, L.pk, L.err, p);
-            // DU back into the block layout (and the sentinel back into DUB and DUF)
-            hipLaunchKernelGGL((k_lu_transpose<NV, false, TJ>), tgrid, dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p,
-                               L.nwin, F::DUB, F::DUB + NV, F::DUB + NV, F::DUB + NV);
         }
         return;
     }
@@ -703,20 +674,22 @@ static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, h
                            L.blks, L.bwd, L.prog_bwd, L.prog_fwd, L.n_fwd, L.err, p);
 }
 
-// this iteration's sweep inputs into sweep order: U, Res, Lc, Lv and the diagonal
+// this iteration's sweep inputs into sweep order: U, Res, Lc, Lv and the diagonal (one field, or
+// three for SA)
 template <int NV, int TJ>
 static void pack_nv(const LuPlan& L, const Par& p, hipStream_t st) {
     using F = Pk<NV>;
-    hipLaunchKernelGGL((k_lu_transpose<NV, true, TJ>), dim3((unsigned)(L.n_fwd * L.nwin), 1u, (unsigned)(F::DUF + (F::S - F::LC))),
-                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p, L.nwin, F::U, F::DUF, F::LC, F::S);
+    constexpr int NAL = NV == 6 ? 3 : 1;
+    hipLaunchKernelGGL((k_lu_transpose<NV, TJ>), dim3((unsigned)(L.n_fwd * L.nwin), 1u, (unsigned)(F::DUF + 6 + NAL)),
+                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.nwin, F::U, F::DUF, F::LC);
 }
 
 // the geometry the sweeps read (S, n), into sweep order once per plan
 template <int NV, int TJ>
 static void pack_geometry_nv(const LuPlan& L, const Par& p, hipStream_t st) {
     using F = Pk<NV>;
-    hipLaunchKernelGGL((k_lu_transpose<NV, true, TJ>), dim3((unsigned)(L.n_fwd * L.nwin), 1u, (unsigned)(F::NF - F::S)),
-                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.P, p, L.nwin, F::S, F::NF, F::NF, F::NF);
+    hipLaunchKernelGGL((k_lu_transpose<NV, TJ>), dim3((unsigned)(L.n_fwd * L.nwin), 1u, (unsigned)(F::NF - F::S)),
+                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.nwin, F::S, F::NF, F::NF);
 }
 
 // NVAR and the plan's tile width as template arguments
@@ -737,8 +710,12 @@ static void by_shape(int NV, int tj, A&&... a) {
 template <int NV, int TJ> struct SweepK { static void run(const LuPlan& L, const Par& p, bool f, bool fe, hipStream_t st) { sweep_nv<NV, TJ>(L, p, f, fe, st); } };
 template <int NV, int TJ> struct PackK { static void run(const LuPlan& L, const Par& p, hipStream_t st) { pack_nv<NV, TJ>(L, p, st); } };
 template <int NV, int TJ> struct GeoK { static void run(const LuPlan& L, const Par& p, hipStream_t st) { pack_geometry_nv<NV, TJ>(L, p, st); } };
-
-void launch_lusgs_pack(const LuPlan& L, const Par& p, hipStream_t st) { by_shape<PackK>(p.NV, L.tj, L, p, st); }
+template <int NV, int TJ> struct DuBoxK {
+    static void run(const LuPlan& L, hipStream_t st) {
+        hipLaunchKernelGGL((k_lu_du_to_box<NV, TJ>), dim3((unsigned)(L.n_fwd * L.nwin), 1u, (unsigned)NV), dim3(PK_T), 0, st,
+                           L.blks, L.fwd, L.pk, L.nwin);
+    }
+};
 
 void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st) {
     by_shape<SweepK>(p.NV, fenced ? 8 : L.tj, L, p, forward, fenced, st);
@@ -757,6 +734,10 @@ static void reset_nv(const LuPlan& L, hipStream_t st) {
 }
 
 void launch_lusgs_pack_geometry(const LuPlan& L, const Par& p, hipStream_t st) { by_shape<GeoK>(p.NV, L.tj, L, p, st); }
+void launch_lusgs_pack(const LuPlan& L, const Par& p, hipStream_t st) { by_shape<PackK>(p.NV, L.tj, L, p, st); }
+void launch_lusgs_du_to_box(const LuPlan& L, int NV, hipStream_t st) {
+    if (L.n_fwd > 0) by_shape<DuBoxK>(NV, L.tj, L, st);
+}
 
 void launch_lusgs_reset(const LuPlan& L, int NV, hipStream_t st) {
     if (NV == 5) reset_nv<5>(L, st);

@@ -65,6 +65,7 @@ struct Engine {
     // 8x8 tiles; parts[b] holds block b's own tables and shares this engine's stream
     bool per_block = false;
     std::vector<Engine> parts;
+    ocfd_batch* owner = nullptr;              // set for a batch's engine: its sweep storage is the batch's pool
 };
 
 // one block of an engine's list, the parameters of its case and the list position of its
@@ -89,6 +90,10 @@ struct ocfd_ctx {
     size_t bytes = 0;                // device memory of the block arrays
     Engine e;
     ocfd_batch* batch = nullptr;     // the batch the context steps through, if any
+    // the context's region of its batch's sweep storage (elements), kept while it stays in the
+    // batch; lu_tj is the tile width the region was laid out for (0: no region)
+    long long lu_off = 0, lu_len = 0;
+    int lu_tj = 0;
     std::vector<BlockRef> refs() {
         std::vector<BlockRef> r;
         for (HostBlock& hb : blocks) r.push_back(BlockRef{&hb, &p, 0});
@@ -99,6 +104,12 @@ struct ocfd_ctx {
 struct ocfd_batch {
     Engine e;
     std::vector<ocfd_ctx*> members;
+    // sweep-ordered storage of the pipelined LU-SGS sweeps: one region per member, laid out and
+    // filled (sentinels, packed geometry) when the member first steps in the batch and released when
+    // it leaves, so a change of membership only rebuilds the tile tables
+    double* pool = nullptr;
+    long long pool_cap = 0;                               // elements
+    std::vector<std::pair<long long, long long>> pool_free;   // free (offset, length), by offset
     std::vector<BlockRef> refs() {
         std::vector<BlockRef> r;
         for (ocfd_ctx* c : members) {
@@ -144,6 +155,7 @@ int carve_block(ocfd_ctx* c, HostBlock& hb) {
     for (double** q : one) if ((r = dalloc(hb, q, n))) return r;
     for (int q = 0; q < 7; q++) if ((r = dalloc(hb, &b.prim[q], n))) return r;
     for (int q = 0; q < 3; q++) if ((r = dalloc(hb, &b.Lc[q], n)) || (r = dalloc(hb, &b.Lv[q], n)) || (r = dalloc(hb, &b.S[q], n))) return r;
+    for (int q = 0; q < (NV == 6 ? 3 : 1); q++) if ((r = dalloc(hb, &b.al[q], n))) return r;
     for (int q = 0; q < 9; q++) if ((r = dalloc(hb, &b.nv[q], n))) return r;
     for (int s = 1; s < 4; s++) for (int q = 0; q < 9; q++) if ((r = dalloc(hb, &b.met[s][q], n))) return r;
     if (turb) {
@@ -336,7 +348,8 @@ void enqueue_stage(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, 
             for (const BlockRef& r : refs) launch_lusgs_sweep(r.hb->d, *r.p, stage == OCFD_STAGE_LU_FORWARD, e.st);
         break;
     case OCFD_STAGE_UPDATE:
-        set_update_limit(P, e.st);
+        if (e.lu_schedule == OCFD_LU_PIPELINED && e.lu.fwd) set_update_limit_sweep_order(P, e.lu, e.st);
+        else set_update_limit(P, e.st);
         break;
     case OCFD_STAGE_BC_HALO:
         set_bc(P, e.st);
@@ -402,9 +415,12 @@ int upload_table(std::vector<void*>& allocs, const std::vector<T>& v, T** out) {
 // Device tables of the pipelined sweeps: the block table, the tile lists of both sweep
 // directions, one progress counter per tile and direction (-1 = nothing done), the error flag
 // and the sweep-ordered storage with the geometry packed into it.
+int build_lu_plan_batch(ocfd_batch* b, const Par& p0);
+
 int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) {
     const int NV = p0.NV;
     if (e.lu.fwd) return OCFD_OK;
+    if (e.owner) return build_lu_plan_batch(e.owner, p0);
     std::vector<Blk> blks;
     for (const BlockRef& r : refs) blks.push_back(r.hb->d);
     long long P = 0;
@@ -442,6 +458,143 @@ int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) {
     return OCFD_OK;
 }
 
+// ---- a batch's sweep storage pool ----
+// Regions are multiples of one step of a tile (NF x 64 elements), so a member's storage keeps the
+// alignment of a context's own (the fields of an element follow from its offset).
+void pool_release(ocfd_batch* b, ocfd_ctx* c) {
+    if (c->lu_tj == 0) return;
+    auto& fr = b->pool_free;
+    fr.emplace_back(c->lu_off, c->lu_len);
+    std::sort(fr.begin(), fr.end());
+    std::vector<std::pair<long long, long long>> merged;
+    for (auto& r : fr) {
+        if (!merged.empty() && merged.back().first + merged.back().second == r.first) merged.back().second += r.second;
+        else merged.push_back(r);
+    }
+    fr.swap(merged);
+    c->lu_tj = 0; c->lu_off = c->lu_len = 0;
+}
+
+void pool_destroy(ocfd_batch* b) {
+    for (ocfd_ctx* c : b->members) { c->lu_tj = 0; c->lu_off = c->lu_len = 0; }
+    if (b->pool) (void)hipFree(b->pool);
+    b->pool = nullptr; b->pool_cap = 0; b->pool_free.clear();
+}
+
+// a free range of n elements (first fit); the pool grows (contents copied) when none is left
+int pool_take(ocfd_batch* b, long long n, long long step, long long* off) {
+    auto& fr = b->pool_free;
+    for (size_t q = 0; q < fr.size(); q++) {
+        if (fr[q].second < n) continue;
+        *off = fr[q].first;
+        fr[q].first += n; fr[q].second -= n;
+        if (fr[q].second == 0) fr.erase(fr.begin() + q);
+        return OCFD_OK;
+    }
+    const long long cap = (std::max(b->pool_cap + n, b->pool_cap + b->pool_cap / 2) + step - 1) / step * step;
+    void* q = nullptr;
+    HIPCHK(hipMalloc(&q, (size_t)cap * sizeof(double)));
+    if (b->pool) {
+        HIPCHK(hipMemcpyAsync(q, b->pool, (size_t)b->pool_cap * sizeof(double), hipMemcpyDeviceToDevice, b->e.st));
+        HIPCHK(hipStreamSynchronize(b->e.st));
+        HIPCHK(hipFree(b->pool));
+    }
+    // the old tail (free or not) plus the new space, merged with a free range ending at the old cap
+    long long start = b->pool_cap;
+    if (!fr.empty() && fr.back().first + fr.back().second == b->pool_cap) { start = fr.back().first; fr.pop_back(); }
+    b->pool = (double*)q;
+    b->pool_cap = cap;
+    *off = start;
+    if (start + n < cap) fr.emplace_back(start + n, cap - start - n);
+    return OCFD_OK;
+}
+
+// A member's region: its tiles laid out as a context's own sweep storage would be, zeroed, the
+// sentinel in its DU fields and padding, its geometry packed (through a temporary table set)
+int pool_fill_member(ocfd_batch* b, ocfd_ctx* c, int tj, const Par& p0) {
+    const int NV = p0.NV;
+    std::vector<Blk> blks;
+    for (HostBlock& hb : c->blocks) blks.push_back(hb.d);
+    long long P = 0;
+    std::vector<LuTask> f = lusgs_tasks(blks, true, lusgs_fields(NV), tj, &P);
+    const long long step = (long long)lusgs_fields(NV) * 64;
+    P = (P + step - 1) / step * step;
+    long long off = 0;
+    int r = pool_take(b, P, step, &off);
+    if (r) return r;
+    c->lu_off = off; c->lu_len = P; c->lu_tj = tj;
+    std::vector<void*> tmp;
+    Blk* pb; LuTask* pf;
+    if ((r = upload_table(tmp, blks, &pb)) || (r = upload_table(tmp, f, &pf))) {
+        for (void* q : tmp) (void)hipFree(q);
+        return r;
+    }
+    LuPlan L{};
+    L.blks = pb; L.fwd = pf; L.n_fwd = (int)f.size();
+    L.pk = b->pool + off; L.P = P; L.tj = tj;
+    int maxnx = 0;
+    for (const Blk& q : blks) maxnx = std::max(maxnx, q.nx - 1);
+    L.nwin = lusgs_windows(maxnx);
+    HIPCHK(hipMemsetAsync(L.pk, 0, (size_t)P * sizeof(double), b->e.st));
+    launch_lusgs_pack_geometry(L, p0, b->e.st);
+    launch_lusgs_reset(L, NV, b->e.st);
+    hipError_t e = hipStreamSynchronize(b->e.st);
+    for (void* q : tmp) (void)hipFree(q);
+    if (e != hipSuccess) return fail(OCFD_EHIP, std::string("filling a member's sweep storage: ") + hipGetErrorString(e));
+    return OCFD_OK;
+}
+
+// The batch's sweep tables over the members' regions: each member's tiles as it would plan them
+// alone, with storage offsets moved to its region and block and tile numbers to its place in the
+// batch's lists.  A member without a region (new, or laid out for another tile width) gets one.
+int build_lu_plan_batch(ocfd_batch* b, const Par& p0) {
+    Engine& e = b->e;
+    const int NV = p0.NV;
+    const int tj = e.lu_schedule == OCFD_LU_PIPELINED_FENCED ? 8 : e.lu_tj;
+    int r;
+    for (ocfd_ctx* c : b->members)
+        if (c->lu_tj != 0 && c->lu_tj != tj) pool_release(b, c);
+    for (ocfd_ctx* c : b->members)
+        if (c->lu_tj == 0 && (r = pool_fill_member(b, c, tj, p0))) return r;
+    std::vector<Blk> blks;
+    std::vector<LuTask> f, w;
+    int maxnx = 0;
+    for (ocfd_ctx* c : b->members) {
+        std::vector<Blk> mb;
+        for (HostBlock& hb : c->blocks) mb.push_back(hb.d);
+        const int bb = (int)blks.size(), tf = (int)f.size(), tw = (int)w.size();
+        std::vector<LuTask> mf = lusgs_tasks(mb, true, lusgs_fields(NV), tj, nullptr);
+        std::vector<LuTask> mw = lusgs_tasks(mb, false, lusgs_fields(NV), tj, nullptr);
+        auto shift = [&](std::vector<LuTask>& v, int tb, std::vector<LuTask>& out) {
+            for (LuTask t : v) {
+                t.blk += bb;
+                t.pofs += c->lu_off; t.pofs_j += c->lu_off; t.pofs_k += c->lu_off;
+                if (t.dep_j >= 0) t.dep_j += tb;
+                if (t.dep_k >= 0) t.dep_k += tb;
+                out.push_back(t);
+            }
+        };
+        shift(mf, tf, f);
+        shift(mw, tw, w);
+        for (const Blk& q : mb) { blks.push_back(q); maxnx = std::max(maxnx, q.nx - 1); }
+    }
+    std::vector<int> mf(f.size(), -1), mb(w.size(), -1), zero{0, 0, e.lu_fail, 0};
+    Blk* pb; LuTask *pf, *pw; int *qf, *qb, *pe;
+    if ((r = upload_table(e.lu_allocs, blks, &pb)) || (r = upload_table(e.lu_allocs, f, &pf)) ||
+        (r = upload_table(e.lu_allocs, w, &pw)) || (r = upload_table(e.lu_allocs, mf, &qf)) ||
+        (r = upload_table(e.lu_allocs, mb, &qb)) || (r = upload_table(e.lu_allocs, zero, &pe)))
+        return r;
+    e.lu.blks = pb; e.lu.fwd = pf; e.lu.bwd = pw;
+    e.lu.n_fwd = (int)f.size(); e.lu.n_bwd = (int)w.size();
+    e.lu.prog_fwd = qf; e.lu.prog_bwd = qb; e.lu.err = pe;
+    e.lu.pk = b->pool;
+    e.lu.P = b->pool_cap;
+    e.lu.nwin = lusgs_windows(maxnx);
+    e.lu.tj = tj;
+    e.lu_nv = NV;
+    return OCFD_OK;
+}
+
 // After a sweep gave up waiting: clear the flag and re-arm both counter sets.
 int check_lu_error(Engine& e) {
     if (!e.lu.err) return OCFD_OK;
@@ -453,7 +606,17 @@ int check_lu_error(Engine& e) {
     HIPCHK(hipMemcpy(e.lu.prog_bwd, minus.data(), e.lu.n_bwd * sizeof(int), hipMemcpyHostToDevice));
     int z = 0;
     HIPCHK(hipMemcpy(e.lu.err, &z, sizeof(int), hipMemcpyHostToDevice));
-    launch_lusgs_reset(e.lu, e.lu_nv, e.st);
+    if (e.owner) {
+        // a batch: every member's region as laid out on its own
+        for (ocfd_ctx* c : e.owner->members) {
+            if (c->lu_tj == 0) continue;
+            LuPlan L{};
+            L.pk = e.owner->pool + c->lu_off; L.P = c->lu_len;
+            launch_lusgs_reset(L, e.lu_nv, e.st);
+        }
+    } else {
+        launch_lusgs_reset(e.lu, e.lu_nv, e.st);
+    }
     HIPCHK(hipStreamSynchronize(e.st));
     return fail(OCFD_EHIP, "LU-SGS sweep: a tile waited beyond its poll limit; the iteration is invalid");
 }
@@ -753,6 +916,16 @@ int run_norms(ocfd_ctx* c, char* dev, const NormLayout& L, double* res_max, doub
 }
 void enqueue_residual(ocfd_ctx* c) {
     for (int s = OCFD_STAGE_PRE; s <= OCFD_STAGE_LU_BACKWARD; s++) enqueue_stage(c, s);
+    // du_LU_SGS leaves DU in the block arrays; the pipelined sweeps leave it in their storage,
+    // where the update reads it, so it is copied out for the host here
+    Engine& e = c->e;
+    if (e.lu_schedule == OCFD_LU_PIPELINED) {
+        if (e.per_block) {
+            for (Engine& q : e.parts) if (q.lu.fwd) launch_lusgs_du_to_box(q.lu, c->p.NV, e.st);
+        } else if (e.lu.fwd) {
+            launch_lusgs_du_to_box(e.lu, c->p.NV, e.st);
+        }
+    }
 }
 void enqueue_bc(ocfd_ctx* c) { set_bc(c->e.set, c->e.st); }
 void enqueue_halo(ocfd_ctx* c) { set_halo(c->e.set, c->e.st); }
@@ -858,6 +1031,7 @@ int ocfd_destroy(ocfd_ctx* c) {
     if (c->batch) {
         ocfd_batch* b = c->batch;
         (void)hipStreamSynchronize(b->e.st);
+        pool_release(b, c);
         for (size_t q = 0; q < b->members.size(); q++)
             if (b->members[q] == c) { b->members.erase(b->members.begin() + q); break; }
         drop_plans(b->e);
@@ -1356,6 +1530,7 @@ static bool same_switches(const Par& a, const Par& b) {
 int ocfd_batch_create(ocfd_batch** out) {
     if (!out) return fail(OCFD_EARG, "null argument");
     ocfd_batch* b = new ocfd_batch();
+    b->e.owner = b;
     hipError_t e = hipStreamCreateWithFlags(&b->e.st, hipStreamNonBlocking);
     if (e != hipSuccess) { delete b; return fail(OCFD_EHIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e)); }
     *out = b;
@@ -1365,6 +1540,7 @@ int ocfd_batch_create(ocfd_batch** out) {
 int ocfd_batch_destroy(ocfd_batch* b) {
     if (!b) return OCFD_OK;
     engine_destroy(b->e);
+    pool_destroy(b);
     for (ocfd_ctx* c : b->members) c->batch = nullptr;     // the members may be stepped alone again
     delete b;
     return OCFD_OK;
@@ -1394,6 +1570,7 @@ int ocfd_batch_remove(ocfd_batch* b, ocfd_ctx* c) {
     for (size_t q = 0; q < b->members.size(); q++) {
         if (b->members[q] != c) continue;
         HIPCHK(hipStreamSynchronize(b->e.st));
+        pool_release(b, c);
         b->members.erase(b->members.begin() + q);
         c->batch = nullptr;
         drop_plans(b->e);

@@ -82,3 +82,42 @@ def test_member_refused_alone_and_destroy_leaves_batch():
     S[1].step(1)                                       # released by the batch's destruction
     ref, _ = _alone(cases[1], 31, 4)
     assert np.array_equal(S[1].download_U(1), ref[0])
+
+
+@pytest.mark.gpu
+def test_membership_churn_reuses_sweep_storage():
+    """Members join and leave between steps as serving requests do, a new member taking the
+    sweep storage a departed one released, and the tile width changes in between (which lays the
+    storage out again); every member's state still equals its case stepped alone."""
+    cases = _cases() + [C.naca_cgrid(65, 25, 9, Ma=0.5, AoA=1.0, **CG),
+                        C.cut_blocks(C.naca_cgrid(49, 17, 5, Ma=0.65, AoA=2.0, **CG), [[[25], [], []]])]
+    seeds = [50 + q for q in range(len(cases))]
+    S = [A.solver_from_case(cs, states=C.perturbed_state(cs, seed=sd, amp=1e-4)) for cs, sd in zip(cases, seeds)]
+    done = [0] * len(S)
+    bt = A.Batch()
+    bt.set_graph(True)
+
+    def step(n):
+        bt.step(n)
+        for q, s in enumerate(S):
+            if s in bt.members:
+                done[q] += n
+
+    bt.add(S[0]); bt.add(S[1]); bt.add(S[2])
+    step(2)
+    bt.remove(S[1])
+    bt.add(S[3])                     # the first member laid out after a release
+    step(2)
+    bt.remove(S[0])
+    bt.add(S[4])
+    step(1)
+    bt.set_lu_tile(4)                # every region laid out again for 4x16 tiles
+    step(2)
+    bt.add(S[1])                     # back in, after stepping nowhere meanwhile
+    step(1)
+    for q, (cs, s) in enumerate(zip(cases, S)):
+        ref, ref_rms = _alone(cs, seeds[q], done[q])
+        for no in range(1, len(cs.blocks) + 1):
+            assert np.array_equal(s.download_U(no), ref[no - 1]), f"member {q} block {no} after {done[q]} iterations"
+    assert done == [4, 3, 8, 6, 4]
+    bt.close()

@@ -892,17 +892,16 @@ __global__ void __launch_bounds__(64) k_limit_apply(BSet S, int ntiles) {
 }
 
 // limit_flow_SA (sub_limitflow.f90:182-231).  The reference clamps and averages in place in
-// lexicographic (k, j, i) order, which in this layout is increasing linear index.  Cells below
-// SAmin are clamped in parallel (their original value is kept in prim[5], free after the
-// residual); when some cell exceeds LSAmax, one work-group replays the reference loop over the
-// flagged cells in index order: an earlier neighbour contributes its final value, a later one
-// its original value, exactly as in the sequential reference.
-// SA part of limit_flow (sub_limitflow.f90): the reference visits the cells in (i,j,k) order and
-// replaces U6 above LSAmax by the mean of the admissible neighbours, the earlier neighbours with
-// their values after this pass (clamped to SAmin or already replaced), the later ones with their
-// values before it.  A flagged cell none of whose 26 neighbours is flagged reads only final values,
-// so those cells are done in parallel; the flagged cells with flagged neighbours are then done one
-// by one in cell order.  The detect kernel snapshots U6 into prim[5] and marks the flagged cells.
+// lexicographic (k, j, i) order: U6 below SAmin is clamped, U6 above LSAmax is replaced by the
+// mean of the admissible neighbours, the earlier neighbours with their values after this pass
+// (clamped or already replaced), the later ones with their values before it.  The update kernel
+// clamps in parallel (the original value is kept in prim[5]) and marks the flagged cells.  A
+// flagged cell none of whose 26 neighbours is flagged reads only values that the pass does not
+// change, so those cells are done in parallel first; the rest follow in order of
+// w = i + 2j + 4k.  Every earlier neighbour in the reference's loop has a smaller w (dk = -1:
+// di + 2dj - 4 < 0; dk = 0, dj = -1: di - 2 < 0; dk = dj = 0: di = -1) and two cells with the same
+// w are never neighbours, so the cells of one w are done in parallel after all smaller w, and
+// each reads exactly what the sequential loop gives it.
 __device__ __forceinline__ void sa_replace(const Blk& b, const Par& p, int i, int j, int k) {
     const double SAmin = 1.0e-8;
     double* U6 = b.U[5];
@@ -945,8 +944,10 @@ __global__ void __launch_bounds__(64) k_limit_sa_isolated(BSet S, int ntiles) {
     }
 }
 
-// one workgroup per block: the remaining (clustered) cells in cell order; clears the mask
-__global__ void __launch_bounds__(64) k_limit_sa_ordered(BSet S) {
+// one workgroup per block: the flagged cells left by the isolated pass, by increasing w over the
+// box that bounds them; clears the mask
+constexpr int SA_ORD_T = 256;
+__global__ void __launch_bounds__(SA_ORD_T) k_limit_sa_ordered(BSet S) {
     if (halted(S)) return;
     const Blk& b = S.blk[blockIdx.x];
     const Par& p = S.par[blockIdx.x];
@@ -954,27 +955,40 @@ __global__ void __launch_bounds__(64) k_limit_sa_ordered(BSet S) {
     const int nxm = b.nx - 1, nym = b.ny - 1;
     const long ncell = (long)nxm * nym * (b.nz - 1);
     const long nword = (ncell + 63) / 64;
-    __shared__ unsigned long long words[64];
-    for (long base = 0; base < nword; base += 64) {
-        long wi = base + threadIdx.x;
-        unsigned long long wv = wi < nword ? b.samask[wi] : 0ull;
-        words[threadIdx.x] = wv;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int q = 0; q < 64 && base + q < nword; q++) {
-                unsigned long long m = words[q];
-                while (m) {
-                    int bit = __builtin_ctzll(m);
-                    m &= m - 1;
-                    long cc = (base + q) * 64 + bit;
-                    int i = (int)(cc % nxm) + 1, j = (int)((cc / nxm) % nym) + 1, k = (int)(cc / ((long)nxm * nym)) + 1;
-                    sa_replace(b, p, i, j, k);
-                }
-            }
+    __shared__ int lo[3], hi[3];
+    if (threadIdx.x < 3) { lo[threadIdx.x] = 1 << 30; hi[threadIdx.x] = -1; }
+    __syncthreads();
+    int l0 = 1 << 30, l1 = 1 << 30, l2 = 1 << 30, h0 = -1, h1 = -1, h2 = -1;
+    for (long wi = threadIdx.x; wi < nword; wi += SA_ORD_T) {
+        unsigned long long m = b.samask[wi];
+        while (m) {
+            const int bit = __builtin_ctzll(m);
+            m &= m - 1;
+            const long cc = wi * 64 + bit;
+            const int i = (int)(cc % nxm) + 1, j = (int)((cc / nxm) % nym) + 1, k = (int)(cc / ((long)nxm * nym)) + 1;
+            l0 = min(l0, i); l1 = min(l1, j); l2 = min(l2, k);
+            h0 = max(h0, i); h1 = max(h1, j); h2 = max(h2, k);
         }
-        __syncthreads();
-        if (wv) b.samask[wi] = 0ull;
     }
+    if (h0 >= 0) {
+        atomicMin(&lo[0], l0); atomicMin(&lo[1], l1); atomicMin(&lo[2], l2);
+        atomicMax(&hi[0], h0); atomicMax(&hi[1], h1); atomicMax(&hi[2], h2);
+    }
+    __syncthreads();
+    if (hi[0] < 0) return;                      // every flagged cell was isolated
+    const int nj = hi[1] - lo[1] + 1, njk = nj * (hi[2] - lo[2] + 1);
+    const int w0 = lo[0] + 2 * lo[1] + 4 * lo[2], w1 = hi[0] + 2 * hi[1] + 4 * hi[2];
+    for (int w = w0; w <= w1; w++) {
+        for (int a = (int)threadIdx.x; a < njk; a += SA_ORD_T) {
+            const int j = lo[1] + a % nj, k = lo[2] + a / nj, i = w - 2 * j - 4 * k;
+            if (i >= lo[0] && i <= hi[0] && sa_flagged(b, sa_cell_index(b, i, j, k))) sa_replace(b, p, i, j, k);
+        }
+        // the replaced values of this w before the next reads them
+        __threadfence_block();
+        __syncthreads();
+    }
+    for (long wi = threadIdx.x; wi < nword; wi += SA_ORD_T)
+        if (b.samask[wi]) b.samask[wi] = 0ull;
 }
 
 __global__ void k_reset_flags(BSet S) {
@@ -1420,7 +1434,7 @@ static void limit_after_update(const SetPlan& P, hipStream_t st) {
     hipLaunchKernelGGL(k_limit_apply, grid, dim3(64), 0, st, bset(P, SP_CELL), n);
     if (P.p0.NV == 6) {
         hipLaunchKernelGGL(k_limit_sa_isolated, grid, dim3(64), 0, st, bset(P, SP_CELL), n);
-        hipLaunchKernelGGL(k_limit_sa_ordered, dim3((unsigned)P.nb), dim3(64), 0, st, bset(P, SP_CELL));
+        hipLaunchKernelGGL(k_limit_sa_ordered, dim3((unsigned)P.nb), dim3(SA_ORD_T), 0, st, bset(P, SP_CELL));
     }
 }
 

@@ -749,15 +749,22 @@ __device__ __forceinline__ long sa_cell_index(const Blk& b, int i, int j, int k)
 }
 __device__ __forceinline__ bool sa_flagged(const Blk& b, long c) { return (b.samask[c >> 6] >> (c & 63)) & 1ull; }
 
+// limit_flow_U's primitive state of conservative variables U(1:5) (sub_limitflow.f90:46-60)
+__device__ __forceinline__ void lim_prims(const double* U, double gamma, double& d1, double& u1, double& v1, double& w1, double& p1) {
+    d1 = U[0]; u1 = U[1] / d1; v1 = U[2] / d1; w1 = U[3] / d1;
+    p1 = (U[4] - (0.5 * d1) * ((u1 * u1 + v1 * v1) + w1 * w1)) * (gamma - 1.0);
+}
+__device__ __forceinline__ bool lim_out(const Par& p, double d1, double u1, double v1, double w1, double p1) {
+    return d1 < p.Ldmin || p1 < p.Lpmin || d1 > p.Ldmax || p1 > p.Lpmax || fabs(u1) > p.Lumax || fabs(v1) > p.Lumax || fabs(w1) > p.Lumax;
+}
+
 __device__ __forceinline__ void update_cell(const Blk& b, const Par& p, int i, int j, int k, size_t s, const double* DU) {
     double U[MAXV];
     for (int m = 0; m < p.NV; m++) { U[m] = b.U[m][s] + DU[m]; b.U[m][s] = U[m]; }
     if (p.limit_flow != 1) return;
-    double d1 = U[0], u1 = U[1] / d1, v1 = U[2] / d1, w1 = U[3] / d1;
-    double p1 = (U[4] - (0.5 * d1) * ((u1 * u1 + v1 * v1) + w1 * w1)) * (p.gamma - 1.0);
-    b.prim[0][s] = d1; b.prim[1][s] = u1; b.prim[2][s] = v1; b.prim[3][s] = w1; b.prim[4][s] = p1;
-    bool bad = d1 < p.Ldmin || p1 < p.Lpmin || d1 > p.Ldmax || p1 > p.Lpmax || fabs(u1) > p.Lumax || fabs(v1) > p.Lumax || fabs(w1) > p.Lumax;
-    if (bad) atomicAdd(&b.flags[1], 1);
+    double d1, u1, v1, w1, p1;
+    lim_prims(U, p.gamma, d1, u1, v1, w1, p1);
+    if (lim_out(p, d1, u1, v1, w1, p1)) atomicAdd(&b.flags[1], 1);
     if (p.NV == 6) {
         const double v = U[5];
         b.prim[5][s] = v;
@@ -842,29 +849,34 @@ __global__ void __launch_bounds__(UP_T) k_update_pk(BSet S, const LuTask* __rest
     }
 }
 
-__device__ __forceinline__ bool lim_bad(const Blk& b, const Par& p, size_t s) {
-    double d1 = b.prim[0][s], u1 = b.prim[1][s], v1 = b.prim[2][s], w1 = b.prim[3][s], p1 = b.prim[4][s];
-    return d1 < p.Ldmin || p1 < p.Lpmin || d1 > p.Ldmax || p1 > p.Lpmax || fabs(u1) > p.Lumax || fabs(v1) > p.Lumax || fabs(w1) > p.Lumax;
+// the limiter's primitive state of cell s from U, and whether it is out of the limits
+__device__ __forceinline__ bool lim_cell(const Blk& b, const Par& p, size_t s, double& d1, double& u1, double& v1, double& w1, double& p1) {
+    const double U[5] = {b.U[0][s], b.U[1][s], b.U[2][s], b.U[3][s], b.U[4][s]};
+    lim_prims(U, p.gamma, d1, u1, v1, w1, p1);
+    return lim_out(p, d1, u1, v1, w1, p1);
 }
 
 // limit_flow_U replacement step (sub_limitflow.f90:84-150): bad cells take the mean of the valid
-// 3x3x3 neighbours of the snapshot, or their own clamped values; sets the IF_OverLimit latch
-// (one launch of a few thousand workgroups looping over the cell tiles: when no block has a bad
-// cell, the usual case, it ends within microseconds)
+// 3x3x3 neighbours' primitive states, or their own clamped state; sets the IF_OverLimit latch.
+// The states come from U after the update: the valid cells keep it through the step, and the
+// new U of a bad cell goes to prim[0..4] (free after the residual) until k_limit_commit, once
+// every bad cell has read its neighbours.  (One launch of a few thousand workgroups looping over
+// the cell tiles: when no block has a bad cell, the usual case, it ends within microseconds.)
 __device__ __forceinline__ void limit_apply_cell(const Blk& b, const Par& p, int i, int j, int k) {
     if (i == 1 && j == 1 && k == 1) b.flags[0] = 1;
     size_t s = IX(b, i, j, k);
-    if (!lim_bad(b, p, s)) return;
+    double ds, us, vs, ws, ps;
+    if (!lim_cell(b, p, s, ds, us, vs, ws, ps)) return;
     double d0 = 0, u0 = 0, v0 = 0, w0 = 0, p0 = 0; int kn = 0;
     for (int ka = -1; ka <= 1; ka++) for (int ja = -1; ja <= 1; ja++) for (int ia = -1; ia <= 1; ia++) {
         int i1 = i + ia, j1 = j + ja, k1 = k + ka;
         if (i1 > 0 && i1 < b.nx && j1 > 0 && j1 < b.ny && k1 > 0 && k1 < b.nz) {
-            size_t t = IX(b, i1, j1, k1);
-            if (!lim_bad(b, p, t)) { d0 = d0 + b.prim[0][t]; u0 = u0 + b.prim[1][t]; v0 = v0 + b.prim[2][t]; w0 = w0 + b.prim[3][t]; p0 = p0 + b.prim[4][t]; kn++; }
+            double dt, ut, vt, wt, pt;
+            if (!lim_cell(b, p, IX(b, i1, j1, k1), dt, ut, vt, wt, pt)) { d0 = d0 + dt; u0 = u0 + ut; v0 = v0 + vt; w0 = w0 + wt; p0 = p0 + pt; kn++; }
         }
     }
     if (kn == 0) {
-        d0 = b.prim[0][s]; u0 = b.prim[1][s]; v0 = b.prim[2][s]; w0 = b.prim[3][s]; p0 = b.prim[4][s];
+        d0 = ds; u0 = us; v0 = vs; w0 = ws; p0 = ps;
         if (d0 < p.Ldmin) d0 = p.Ldmin;
         if (d0 > p.Ldmax) d0 = p.Ldmax;
         if (p0 < p.Lpmin) p0 = p.Lpmin;
@@ -876,19 +888,39 @@ __device__ __forceinline__ void limit_apply_cell(const Blk& b, const Par& p, int
         double sn1 = 1.0 / (double)kn;
         d0 = d0 * sn1; u0 = u0 * sn1; v0 = v0 * sn1; w0 = w0 * sn1; p0 = p0 * sn1;
     }
-    b.U[0][s] = d0; b.U[1][s] = d0 * u0; b.U[2][s] = d0 * v0; b.U[3][s] = d0 * w0;
-    b.U[4][s] = p0 / (p.gamma - 1.0) + (0.5 * d0) * ((u0 * u0 + v0 * v0) + w0 * w0);
+    b.prim[0][s] = d0; b.prim[1][s] = d0 * u0; b.prim[2][s] = d0 * v0; b.prim[3][s] = d0 * w0;
+    b.prim[4][s] = p0 / (p.gamma - 1.0) + (0.5 * d0) * ((u0 * u0 + v0 * v0) + w0 * w0);
 }
 
-__global__ void __launch_bounds__(64) k_limit_apply(BSet S, int ntiles) {
-    if (halted(S)) return;
-    for (int t = (int)blockIdx.x; t < ntiles; t += (int)gridDim.x) {
-        int bi, i, j, k;
-        if (!locate_tile<SP_CELL>(S, t, bi, i, j, k)) continue;
+// the cell tiles of the blocks whose flag word `which` is nonzero, spread over the grid: every
+// workgroup walks the block list (uniform, scalar loads) and skips an unflagged block whole
+template <typename F>
+__device__ __forceinline__ void flagged_tiles(const BSet& S, int nb, int ntiles, int which, F&& f) {
+    for (int bi = 0; bi < nb; bi++) {
         const Blk& b = S.blk[bi];
-        if (b.flags[1] == 0) continue;
-        limit_apply_cell(b, S.par[bi], i, j, k);
+        if (b.flags[which] == 0) continue;
+        const int t1 = bi + 1 < nb ? S.tile0[bi + 1] : ntiles;
+        for (int t = S.tile0[bi] + (int)blockIdx.x; t < t1; t += (int)gridDim.x) {
+            int bj, i, j, k;
+            if (locate_tile<SP_CELL>(S, t, bj, i, j, k)) f(b, S.par[bi], i, j, k);
+        }
     }
+}
+
+__global__ void __launch_bounds__(64) k_limit_apply(BSet S, int nb, int ntiles) {
+    if (halted(S)) return;
+    flagged_tiles(S, nb, ntiles, 1, [](const Blk& b, const Par& p, int i, int j, int k) { limit_apply_cell(b, p, i, j, k); });
+}
+
+// the new U of the bad cells, once every bad cell has read its neighbours
+__global__ void __launch_bounds__(64) k_limit_commit(BSet S, int nb, int ntiles) {
+    if (halted(S)) return;
+    flagged_tiles(S, nb, ntiles, 1, [](const Blk& b, const Par& p, int i, int j, int k) {
+        const size_t s = IX(b, i, j, k);
+        double d1, u1, v1, w1, p1;
+        if (lim_cell(b, p, s, d1, u1, v1, w1, p1))
+            for (int m = 0; m < 5; m++) b.U[m][s] = b.prim[m][s];
+    });
 }
 
 // limit_flow_SA (sub_limitflow.f90:182-231).  The reference clamps and averages in place in
@@ -933,15 +965,9 @@ __device__ __forceinline__ void limit_sa_isolated_cell(const Blk& b, const Par& 
     atomicAnd(&b.samask[c >> 6], ~(1ull << (c & 63)));
 }
 
-__global__ void __launch_bounds__(64) k_limit_sa_isolated(BSet S, int ntiles) {
+__global__ void __launch_bounds__(64) k_limit_sa_isolated(BSet S, int nb, int ntiles) {
     if (halted(S)) return;
-    for (int t = (int)blockIdx.x; t < ntiles; t += (int)gridDim.x) {
-        int bi, i, j, k;
-        if (!locate_tile<SP_CELL>(S, t, bi, i, j, k)) continue;
-        const Blk& b = S.blk[bi];
-        if (b.flags[2] == 0) continue;
-        limit_sa_isolated_cell(b, S.par[bi], i, j, k);
-    }
+    flagged_tiles(S, nb, ntiles, 2, [](const Blk& b, const Par& p, int i, int j, int k) { limit_sa_isolated_cell(b, p, i, j, k); });
 }
 
 // one workgroup per block: the flagged cells left by the isolated pass, by increasing w over the
@@ -1431,9 +1457,10 @@ static void limit_after_update(const SetPlan& P, hipStream_t st) {
     const int n = P.ntiles[SP_CELL];
     if (n <= 0) return;
     const dim3 grid((unsigned)std::min(n, 4096));
-    hipLaunchKernelGGL(k_limit_apply, grid, dim3(64), 0, st, bset(P, SP_CELL), n);
+    hipLaunchKernelGGL(k_limit_apply, grid, dim3(64), 0, st, bset(P, SP_CELL), P.nb, n);
+    hipLaunchKernelGGL(k_limit_commit, grid, dim3(64), 0, st, bset(P, SP_CELL), P.nb, n);
     if (P.p0.NV == 6) {
-        hipLaunchKernelGGL(k_limit_sa_isolated, grid, dim3(64), 0, st, bset(P, SP_CELL), n);
+        hipLaunchKernelGGL(k_limit_sa_isolated, grid, dim3(64), 0, st, bset(P, SP_CELL), P.nb, n);
         hipLaunchKernelGGL(k_limit_sa_ordered, dim3((unsigned)P.nb), dim3(SA_ORD_T), 0, st, bset(P, SP_CELL));
     }
 }

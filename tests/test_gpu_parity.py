@@ -78,19 +78,26 @@ def test_iteration_matches_oracle(name, path):
     more = CASES[name].more
     if not more:
         return
+    # Later iterations: on the drop-in path (the host's geometry) every case, with transcendental
+    # functions or not, stays within the per-cell FP64 bar (measured: at most 6e-13 over every
+    # case after three iterations); device-computed geometry differs from the host's in the last
+    # bits of its metrics, which three iterations carry to about 3e-10 on the SA C-grids.
+    later = TOL if path == "dropin" else 1e-9
     o.step(more); s.step(more)
     for no, b in enumerate(case.blocks, start=1):
         nx, ny, nz = b.dims
         _cmp(f"U{1 + more}", s.download_U(no)[:, 2:nx + 5, 2:ny + 5, 2:nz + 5], oracle_U(o, no, nv)[:, 2:nx + 5, 2:ny + 5, 2:nz + 5],
-             exact, tol=1e-9)
+             exact, tol=later)
     # monitor: residual norms and force coefficients of the last iteration
     mx, rms = s.norms()
     st = o.monitor()
-    _cmp("Res_max", mx, st["res_max"], False, tol=1e-9)
-    _cmp("Res_rms", rms, st["res_rms"], False, tol=1e-9)
+    _cmp("Res_max", mx, st["res_max"], False, tol=later)
+    _cmp("Res_rms", rms, st["res_rms"], False, tol=later)
     f = s.forces()
     ref = np.concatenate([st["Ft"], [st["CL"], st["CD"], st["CS"]]])
-    assert np.max(np.abs(f - ref)) <= 1e-9 * max(1.0, np.max(np.abs(ref))), (f, ref)
+    ferr = np.max(np.abs(f - ref)) / max(1.0, np.max(np.abs(ref)))
+    print(f"  forces     rel={ferr:.2e}")
+    assert ferr <= later, (f, ref)
 
 
 @pytest.mark.gpu
@@ -260,7 +267,8 @@ def test_graph_replay_matches_eager(name):
 def test_c_grid_257x97x33(flux, iters, path):
     """The single-block 257x97x33 NACA 0012 C-grid (SA, WENO5, Re 6e6, Ma 0.8, AoA 1.25 deg) at full
     size from freestream: per-cell residual after one iteration within 1e-11, and the state after
-    `iters` iterations within 1e-9 of the oracle.  HLLC (flux 3) runs one iteration only (the
+    `iters` iterations within 1e-11 of the oracle on the drop-in path (1e-9 with device-computed
+    geometry).  HLLC (flux 3) runs one iteration only (the
     reference HLLC diverges on this grid).  On the device-geometry path dtime_mesh comes from the
     device acos/exp (an ulp apart from the host libm); the LU-SGS update cancels Res against the
     neighbour terms in near-freestream cells, so DU there is held to 1e-9 like the state."""
@@ -285,7 +293,7 @@ def test_c_grid_257x97x33(flux, iters, path):
     if iters > 1:
         o.step(iters - 1); s.step(iters - 1)
     _cmp(f"U{iters}", s.download_U(1)[:, 2:nx + 5, 2:ny + 5, 2:nz + 5], oracle_U(o, 1, nv)[:, 2:nx + 5, 2:ny + 5, 2:nz + 5],
-         False, tol=1e-9)
+         False, tol=TOL if path == "dropin" else 1e-9)
 
 
 @pytest.mark.gpu

@@ -200,7 +200,8 @@ struct LuPlan {
     // sweep-ordered copies of the sweep's inputs and of DU (layout below), P elements
     double* pk = nullptr;
     long long P = 0;
-    int nwin = 0;                // i-windows per tile of the pack/unpack transposes
+    const int2* win = nullptr;   // (forward tile, first i) of every i-window of PK_W cells of the tiles
+    int nwins = 0;
     int tj = 8;                  // tile width in j (tiles are tj x 64/tj (j,k) columns)
 };
 

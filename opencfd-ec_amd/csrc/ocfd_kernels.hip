@@ -800,20 +800,20 @@ __global__ void __launch_bounds__(64) k_update_detect(BSet S) {
 constexpr int UP_ROWS = 16, UP_T = UP_ROWS * PK_W;
 
 template <int NV, int TJ>
-__global__ void __launch_bounds__(UP_T) k_update_pk(BSet S, const LuTask* __restrict__ tiles, double* __restrict__ pk, int nwin) {
+__global__ void __launch_bounds__(UP_T) k_update_pk(BSet S, const LuTask* __restrict__ tiles, double* __restrict__ pk,
+                                                    const int2* __restrict__ wins) {
     using F = Pk<NV>;
     static_assert(UP_ROWS % TJ == 0, "a pass covers whole j rows of the tile");
     constexpr int DSPAN = TJ - 1 + UP_ROWS / TJ - 1;          // spread of d over a pass's lanes
     constexpr int SLOTS = (PK_W + DSPAN) * UP_ROWS;           // (step, lane) slots a pass can touch
     __shared__ double buf[NV][UP_ROWS][PK_W + 1];
     if (halted(S)) return;
-    const int tw = xcd_tile((int)blockIdx.x, (int)gridDim.x);     // (tile, i-window), windows fastest
-    const LuTask T_ = tiles[tw / nwin];
+    const int2 wn = wins[xcd_tile((int)blockIdx.x, (int)gridDim.x)];     // (tile, i-window), windows fastest
+    const LuTask T_ = tiles[wn.x];
     const Blk& b = S.blk[T_.blk];
     const Par& p = S.par[T_.blk];
     const int nxm = b.nx - 1;
-    const int i0 = 1 + (tw % nwin) * PK_W;
-    if (i0 > nxm) return;
+    const int i0 = wn.y;
     const int W = min(PK_W, nxm - i0 + 1);
     const int t = (int)threadIdx.x, rr = t / PK_W, c = t % PK_W;
 #pragma unroll 1
@@ -1472,9 +1472,9 @@ void set_update_limit(const SetPlan& P, hipStream_t st) {
 
 template <int NV, int TJ>
 static void update_pk(const SetPlan& P, const LuPlan& L, hipStream_t st) {
-    if (L.n_fwd > 0)
-        hipLaunchKernelGGL((k_update_pk<NV, TJ>), dim3((unsigned)(L.n_fwd * L.nwin)), dim3(UP_T), 0, st, bset(P, SP_CELL), L.fwd,
-                           L.pk, L.nwin);
+    if (L.nwins > 0)
+        hipLaunchKernelGGL((k_update_pk<NV, TJ>), dim3((unsigned)L.nwins), dim3(UP_T), 0, st, bset(P, SP_CELL), L.fwd,
+                           L.pk, L.win);
 }
 void set_update_limit_sweep_order(const SetPlan& P, const LuPlan& L, hipStream_t st) {
     switch (P.p0.NV * 100 + L.tj) {

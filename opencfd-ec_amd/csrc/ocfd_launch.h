@@ -64,7 +64,8 @@ int check_division(long long n, unsigned long long seed, int emax, long long* mi
 std::vector<LuTask> lusgs_tasks(const std::vector<Blk>& blks, bool forward, int nfields, int tj, long long* total_elements);
 bool lusgs_tile_ok(int tj);
 int lusgs_fields(int NV);
-int lusgs_windows(int max_cells_i);
+// (forward tile, first i) of every i-window of the tiles, tile after tile
+std::vector<int2> lusgs_window_list(const std::vector<Blk>& blks, const std::vector<LuTask>& fwd);
 void launch_lusgs_pack_geometry(const LuPlan& L, const Par& p, hipStream_t st);
 // the sweeps' per-iteration inputs (U, Res, Lc, Lv, the diagonal) into sweep order
 void launch_lusgs_pack(const LuPlan& L, const Par& p, hipStream_t st);

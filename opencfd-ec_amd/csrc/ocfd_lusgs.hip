@@ -256,14 +256,13 @@ constexpr int PK_T = 256;
 // back to back on one XCD, where the L2 merges those partial lines before they go to memory.
 template <int NV, int TJ>
 __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
-                                                       double* __restrict__ pk, int nwin, int f0, int f1, int f2) {
+                                                       double* __restrict__ pk, const int2* __restrict__ wins, int f0, int f1, int f2) {
     __shared__ double buf[LU_NT][PK_W + 1];
-    const int tw = xcd_tile((int)blockIdx.x, (int)gridDim.x);     // (tile, i-window), windows fastest
-    const LuTask T = tiles[tw / nwin];
+    const int2 wn = wins[xcd_tile((int)blockIdx.x, (int)gridDim.x)];     // (tile, i-window), windows fastest
+    const LuTask T = tiles[wn.x];
     const Blk& b = blks[T.blk];
     const int nxm = b.nx - 1;
-    const int i0 = 1 + (tw % nwin) * PK_W;
-    if (i0 > nxm) return;
+    const int i0 = wn.y;
     const int W = min(PK_W, nxm - i0 + 1);
     const int nsteps = nxm + T.tj + T.tk - 2;
     const int t = threadIdx.x;
@@ -287,14 +286,13 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
 // reads DU between the sweeps and the update; the storage keeps it, and the update reads it there
 template <int NV, int TJ>
 __global__ void __launch_bounds__(PK_T) k_lu_du_to_box(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
-                                                       const double* __restrict__ pk, int nwin) {
+                                                       const double* __restrict__ pk, const int2* __restrict__ wins) {
     __shared__ double buf[LU_NT][PK_W + 1];
-    const int tw = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    const LuTask T = tiles[tw / nwin];
+    const int2 wn = wins[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
+    const LuTask T = tiles[wn.x];
     const Blk& b = blks[T.blk];
     const int nxm = b.nx - 1;
-    const int i0 = 1 + (tw % nwin) * PK_W;
-    if (i0 > nxm) return;
+    const int i0 = wn.y;
     const int W = min(PK_W, nxm - i0 + 1);
     const int nsteps = nxm + T.tj + T.tk - 2;
     const int t = threadIdx.x, m = (int)blockIdx.z;
@@ -680,16 +678,16 @@ template <int NV, int TJ>
 static void pack_nv(const LuPlan& L, const Par& p, hipStream_t st) {
     using F = Pk<NV>;
     constexpr int NAL = NV == 6 ? 3 : 1;
-    hipLaunchKernelGGL((k_lu_transpose<NV, TJ>), dim3((unsigned)(L.n_fwd * L.nwin), 1u, (unsigned)(F::DUF + 6 + NAL)),
-                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.nwin, F::U, F::DUF, F::LC);
+    hipLaunchKernelGGL((k_lu_transpose<NV, TJ>), dim3((unsigned)L.nwins, 1u, (unsigned)(F::DUF + 6 + NAL)),
+                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.win, F::U, F::DUF, F::LC);
 }
 
 // the geometry the sweeps read (S, n), into sweep order once per plan
 template <int NV, int TJ>
 static void pack_geometry_nv(const LuPlan& L, const Par& p, hipStream_t st) {
     using F = Pk<NV>;
-    hipLaunchKernelGGL((k_lu_transpose<NV, TJ>), dim3((unsigned)(L.n_fwd * L.nwin), 1u, (unsigned)(F::NF - F::S)),
-                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.nwin, F::S, F::NF, F::NF);
+    hipLaunchKernelGGL((k_lu_transpose<NV, TJ>), dim3((unsigned)L.nwins, 1u, (unsigned)(F::NF - F::S)),
+                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.win, F::S, F::NF, F::NF);
 }
 
 // NVAR and the plan's tile width as template arguments
@@ -712,8 +710,8 @@ template <int NV, int TJ> struct PackK { static void run(const LuPlan& L, const 
 template <int NV, int TJ> struct GeoK { static void run(const LuPlan& L, const Par& p, hipStream_t st) { pack_geometry_nv<NV, TJ>(L, p, st); } };
 template <int NV, int TJ> struct DuBoxK {
     static void run(const LuPlan& L, hipStream_t st) {
-        hipLaunchKernelGGL((k_lu_du_to_box<NV, TJ>), dim3((unsigned)(L.n_fwd * L.nwin), 1u, (unsigned)NV), dim3(PK_T), 0, st,
-                           L.blks, L.fwd, L.pk, L.nwin);
+        hipLaunchKernelGGL((k_lu_du_to_box<NV, TJ>), dim3((unsigned)L.nwins, 1u, (unsigned)NV), dim3(PK_T), 0, st,
+                           L.blks, L.fwd, L.pk, L.win);
     }
 };
 
@@ -724,7 +722,14 @@ void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fe
 bool lusgs_tile_ok(int tj) { return tj == 4 || tj == 8 || tj == 16; }
 
 int lusgs_fields(int NV) { return 4 * NV + 21; }
-int lusgs_windows(int max_cells_i) { return (max_cells_i + PK_W - 1) / PK_W; }
+std::vector<int2> lusgs_window_list(const std::vector<Blk>& blks, const std::vector<LuTask>& fwd) {
+    std::vector<int2> out;
+    for (size_t t = 0; t < fwd.size(); t++) {
+        const int nxm = blks[fwd[t].blk].nx - 1;
+        for (int i0 = 1; i0 <= nxm; i0 += PK_W) out.push_back(make_int2((int)t, i0));
+    }
+    return out;
+}
 
 // the sentinel into DUF and DUB (also after a sweep gave up waiting)
 template <int NV>
@@ -733,10 +738,14 @@ static void reset_nv(const LuPlan& L, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_sentinel<NV>, dim3(1024), dim3(256), 0, st, L.pk, L.P);
 }
 
-void launch_lusgs_pack_geometry(const LuPlan& L, const Par& p, hipStream_t st) { by_shape<GeoK>(p.NV, L.tj, L, p, st); }
-void launch_lusgs_pack(const LuPlan& L, const Par& p, hipStream_t st) { by_shape<PackK>(p.NV, L.tj, L, p, st); }
+void launch_lusgs_pack_geometry(const LuPlan& L, const Par& p, hipStream_t st) {
+    if (L.nwins > 0) by_shape<GeoK>(p.NV, L.tj, L, p, st);
+}
+void launch_lusgs_pack(const LuPlan& L, const Par& p, hipStream_t st) {
+    if (L.nwins > 0) by_shape<PackK>(p.NV, L.tj, L, p, st);
+}
 void launch_lusgs_du_to_box(const LuPlan& L, int NV, hipStream_t st) {
-    if (L.n_fwd > 0) by_shape<DuBoxK>(NV, L.tj, L, st);
+    if (L.nwins > 0) by_shape<DuBoxK>(NV, L.tj, L, st);
 }
 
 void launch_lusgs_reset(const LuPlan& L, int NV, hipStream_t st) {

@@ -446,9 +446,11 @@ int build_lu_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) {
     HIPCHK(hipMemsetAsync(pk, 0, bytes, e.st));
     e.lu.pk = (double*)pk;
     e.lu.P = P;
-    int maxnx = 0;
-    for (const Blk& q : blks) maxnx = std::max(maxnx, q.nx - 1);
-    e.lu.nwin = lusgs_windows(maxnx);
+    const std::vector<int2> wins = lusgs_window_list(blks, f);
+    int2* pwin;
+    if ((r = upload_table(e.lu_allocs, wins, &pwin))) return r;
+    e.lu.win = pwin;
+    e.lu.nwins = (int)wins.size();
     e.lu.tj = tj;
     launch_lusgs_pack_geometry(e.lu, p0, e.st);
     launch_lusgs_reset(e.lu, NV, e.st);
@@ -524,17 +526,16 @@ int pool_fill_member(ocfd_batch* b, ocfd_ctx* c, int tj, const Par& p0) {
     if (r) return r;
     c->lu_off = off; c->lu_len = P; c->lu_tj = tj;
     std::vector<void*> tmp;
-    Blk* pb; LuTask* pf;
-    if ((r = upload_table(tmp, blks, &pb)) || (r = upload_table(tmp, f, &pf))) {
+    const std::vector<int2> wins = lusgs_window_list(blks, f);
+    Blk* pb; LuTask* pf; int2* pwin;
+    if ((r = upload_table(tmp, blks, &pb)) || (r = upload_table(tmp, f, &pf)) || (r = upload_table(tmp, wins, &pwin))) {
         for (void* q : tmp) (void)hipFree(q);
         return r;
     }
     LuPlan L{};
     L.blks = pb; L.fwd = pf; L.n_fwd = (int)f.size();
     L.pk = b->pool + off; L.P = P; L.tj = tj;
-    int maxnx = 0;
-    for (const Blk& q : blks) maxnx = std::max(maxnx, q.nx - 1);
-    L.nwin = lusgs_windows(maxnx);
+    L.win = pwin; L.nwins = (int)wins.size();
     HIPCHK(hipMemsetAsync(L.pk, 0, (size_t)P * sizeof(double), b->e.st));
     launch_lusgs_pack_geometry(L, p0, b->e.st);
     launch_lusgs_reset(L, NV, b->e.st);
@@ -558,7 +559,6 @@ int build_lu_plan_batch(ocfd_batch* b, const Par& p0) {
         if (c->lu_tj == 0 && (r = pool_fill_member(b, c, tj, p0))) return r;
     std::vector<Blk> blks;
     std::vector<LuTask> f, w;
-    int maxnx = 0;
     for (ocfd_ctx* c : b->members) {
         std::vector<Blk> mb;
         for (HostBlock& hb : c->blocks) mb.push_back(hb.d);
@@ -576,7 +576,7 @@ int build_lu_plan_batch(ocfd_batch* b, const Par& p0) {
         };
         shift(mf, tf, f);
         shift(mw, tw, w);
-        for (const Blk& q : mb) { blks.push_back(q); maxnx = std::max(maxnx, q.nx - 1); }
+        for (const Blk& q : mb) blks.push_back(q);
     }
     std::vector<int> mf(f.size(), -1), mb(w.size(), -1), zero{0, 0, e.lu_fail, 0};
     Blk* pb; LuTask *pf, *pw; int *qf, *qb, *pe;
@@ -587,9 +587,13 @@ int build_lu_plan_batch(ocfd_batch* b, const Par& p0) {
     e.lu.blks = pb; e.lu.fwd = pf; e.lu.bwd = pw;
     e.lu.n_fwd = (int)f.size(); e.lu.n_bwd = (int)w.size();
     e.lu.prog_fwd = qf; e.lu.prog_bwd = qb; e.lu.err = pe;
+    const std::vector<int2> wins = lusgs_window_list(blks, f);
+    int2* pwin;
+    if ((r = upload_table(e.lu_allocs, wins, &pwin))) return r;
     e.lu.pk = b->pool;
     e.lu.P = b->pool_cap;
-    e.lu.nwin = lusgs_windows(maxnx);
+    e.lu.win = pwin;
+    e.lu.nwins = (int)wins.size();
     e.lu.tj = tj;
     e.lu_nv = NV;
     return OCFD_OK;

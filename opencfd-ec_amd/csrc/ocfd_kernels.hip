@@ -44,14 +44,18 @@ __global__ void __launch_bounds__(64) k_visc(BSet S) {
     b.muf[s] = (p.turb == 2 || p.turb == 3) ? (mu * p.Re) / p.Re : mu;
 }
 
-__global__ void k_visc_ghost(BSet S) {
-    const Blk& b = S.blk[blockIdx.z / 6];
-    int a = blockIdx.x * blockDim.x + threadIdx.x + 1, c = blockIdx.y + 1, f = blockIdx.z % 6;
-    int nx = b.nx, ny = b.ny, nz = b.nz;
+// mu and muf of the first cells of each face copied into its first ghost layer; one task per block
+// and face, tiles of 64 cells along the face's first tangential direction
+__global__ void __launch_bounds__(64) k_visc_ghost(SubSet S) {
+    OCFD_SUB_LOCATE
+    (void)p; (void)B;
+    if (a >= T_.n0) return;
+    const int f = T_.arg, u = a + 1, v = c + 1;
+    const int nx = b.nx, ny = b.ny, nz = b.nz;
     size_t d, s;
-    if (f < 2) { if (a > ny - 1 || c > nz - 1) return; d = IX(b, f == 0 ? 0 : nx, a, c); s = IX(b, f == 0 ? 1 : nx - 1, a, c); }
-    else if (f < 4) { if (a > nx - 1 || c > nz - 1) return; d = IX(b, a, f == 2 ? 0 : ny, c); s = IX(b, a, f == 2 ? 1 : ny - 1, c); }
-    else { if (a > nx - 1 || c > ny - 1) return; d = IX(b, a, c, f == 4 ? 0 : nz); s = IX(b, a, c, f == 4 ? 1 : nz - 1); }
+    if (f < 2) { d = IX(b, f == 0 ? 0 : nx, u, v); s = IX(b, f == 0 ? 1 : nx - 1, u, v); }
+    else if (f < 4) { d = IX(b, u, f == 2 ? 0 : ny, v); s = IX(b, u, f == 2 ? 1 : ny - 1, v); }
+    else { d = IX(b, u, v, f == 4 ? 0 : nz); s = IX(b, u, v, f == 4 ? 1 : nz - 1); }
     b.mu[d] = b.mu[s];
     b.muf[d] = b.muf[s];
 }
@@ -1239,9 +1243,12 @@ __device__ __forceinline__ void d_corner_vertices(const Blk& b, int c, int m) {
     U[IX(b, I, J, K)] = r;
 }
 
-// edges then vertices of every block of the set (blockIdx.z: block x variable)
+// edges then vertices of every block of the set (blockIdx.y: edge, blockIdx.z: block x variable);
+// a workgroup walks its whole edge
 __global__ void k_corner_edges(BSet S, int NV) {
-    d_corner_edges(S.blk[blockIdx.z / NV], NV, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z % NV);
+    const Blk& b = S.blk[blockIdx.z / NV];
+    const int len = max(b.nx, max(b.ny, b.nz));
+    for (int t = (int)threadIdx.x; t < len; t += (int)blockDim.x) d_corner_edges(b, NV, t, blockIdx.y, blockIdx.z % NV);
 }
 __global__ void k_corner_vertices(BSet S, int NV) {
     d_corner_vertices(S.blk[blockIdx.x / NV], threadIdx.x, blockIdx.x % NV);
@@ -1381,7 +1388,7 @@ void set_pre(const SetPlan& P, hipStream_t st) {
     launch_space<SP_BOX>(k_prims, P, st);
     if (p.visc != 1) return;
     launch_space<SP_CELL>(k_visc, P, st);
-    hipLaunchKernelGGL(k_visc_ghost, dim3((unsigned)((P.mx + 63) / 64), (unsigned)P.mx, (unsigned)(6 * P.nb)), dim3(64), 0, st, bset(P, SP_CELL));
+    launch_subs(k_visc_ghost, P, P.vghost, st);
     if (p.turb == 2) {
         launch_space<SP_N0>(k_sa_mut, P, st);
         launch_subs(k_amut, P, P.amut, st);
@@ -1496,7 +1503,7 @@ void set_bc(const SetPlan& P, hipStream_t st) { launch_subs(k_bc, P, P.bc, st); 
 void set_halo(const SetPlan& P, hipStream_t st) {
     if (P.halo.ntiles > 0)
         hipLaunchKernelGGL(k_halo_set, dim3((unsigned)P.halo.ntiles), dim3(256), 0, st, HaloSet{P.blk, P.halo.t, P.halo.tile_task}, P.p0.NV);
-    hipLaunchKernelGGL(k_corner_edges, dim3((unsigned)((P.mx + 63) / 64), 12, (unsigned)(P.p0.NV * P.nb)), dim3(64), 0, st, bset(P, SP_CELL), P.p0.NV);
+    hipLaunchKernelGGL(k_corner_edges, dim3(1, 12, (unsigned)(P.p0.NV * P.nb)), dim3(64), 0, st, bset(P, SP_CELL), P.p0.NV);
     hipLaunchKernelGGL(k_corner_vertices, dim3((unsigned)(P.p0.NV * P.nb)), dim3(64), 0, st, bset(P, SP_CELL), P.p0.NV);
 }
 

@@ -29,6 +29,7 @@ struct SetPlan {
     int* tile0[SP_COUNT] = {};
     int ntiles[SP_COUNT] = {};
     SubList bc, amut, kbc, force;   // force: the wall (bc 2) subfaces, one thread per face
+    SubList vghost;                  // the six faces of every block (k_visc_ghost)
     HaloList halo;
     int mx = 0;                      // max over blocks of max(nx, ny, nz)
     const int* halt = nullptr;       // the LU-SGS error word of the engine's sweeps (or null)

@@ -714,8 +714,20 @@ int build_set_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) 
             }
         }
     }
+    std::vector<SubTask> vghost;
+    for (int bi = 0; bi < P.nb; bi++) {
+        const Blk& b = blks[bi];
+        for (int f = 0; f < 6; f++) {
+            SubTask t{};
+            t.blk = bi; t.arg = f;
+            t.n0 = f < 2 ? b.ny - 1 : b.nx - 1;
+            t.n1 = f < 4 ? b.nz - 1 : b.ny - 1;
+            vghost.push_back(t);
+        }
+    }
     if ((r = build_sub_list(e.set_allocs, bc, P.bc)) || (r = build_sub_list(e.set_allocs, amut, P.amut)) ||
-        (r = build_sub_list(e.set_allocs, kbc, P.kbc)) || (r = build_sub_list(e.set_allocs, force, P.force)))
+        (r = build_sub_list(e.set_allocs, kbc, P.kbc)) || (r = build_sub_list(e.set_allocs, force, P.force)) ||
+        (r = build_sub_list(e.set_allocs, vghost, P.vghost)))
         return r;
     P.halo.ntiles = (int)halo_tile.size();
     if ((r = upload_table(e.set_allocs, halo, &P.halo.t)) || (r = upload_table(e.set_allocs, halo_tile, &P.halo.tile_task))) return r;

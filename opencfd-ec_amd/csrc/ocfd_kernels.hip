@@ -1507,7 +1507,11 @@ void set_halo(const SetPlan& P, hipStream_t st) {
     hipLaunchKernelGGL(k_corner_vertices, dim3((unsigned)(P.p0.NV * P.nb)), dim3(64), 0, st, bset(P, SP_CELL), P.p0.NV);
 }
 
+// force_vt_kw (sub_time_advance.f90:27-61): U6 (and U7) below 0 set to 1e-10 in the cells.  With SA
+// and the limiter on, the update has already raised every cell's U6 below 1e-8 to 1e-8 and the SA
+// replacement writes positive values only, so the pass cannot change anything and is not launched.
 void set_force_vt_kw(const SetPlan& P, hipStream_t st) {
+    if (P.p0.NV == 6 && P.p0.limit_flow == 1) return;
     if (P.p0.NV >= 6) launch_space<SP_CELL>(k_force_vt_kw, P, st);
 }
 

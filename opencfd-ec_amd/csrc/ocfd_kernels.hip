@@ -795,16 +795,15 @@ __global__ void __launch_bounds__(64) k_update_detect(BSet S) {
 // The update of an iteration with the pipelined sweeps: the backward sweep's DU comes from the
 // sweep-ordered storage.  A workgroup takes one sweep tile's cells of an i-window (the tiles and
 // windows of that storage), UP_ROWS lanes of the tile per pass.  A pass reads its lanes' DU along
-// the steps (each step's UP_ROWS lanes are one contiguous run), puts the sentinel back into those
-// DU elements and into the forward DU of the same elements (the next sweeps find every element
-// they write holding it), and parks DU in LDS; then its threads update the pass's cells, i fastest,
-// and store DU in the block arrays (the host reads it there).  Lane l of a pass holds cell
-// i = q - d(l) + 1 at step q, d = jl + kl.  After a sweep gave up, the update leaves U, DU and the
+// the steps (each step's UP_ROWS lanes are one contiguous run) and parks it in LDS; then its
+// threads update the pass's cells, i fastest.  DU stays in the storage until the next iteration's
+// pack; the block arrays get it only when the host asks for it.  Lane l of a pass holds cell
+// i = q - d(l) + 1 at step q, d = jl + kl.  After a sweep gave up, the update leaves U and the
 // storage as they are; the host puts the sentinel back into the whole storage.
 constexpr int UP_ROWS = 16, UP_T = UP_ROWS * PK_W;
 
 template <int NV, int TJ>
-__global__ void __launch_bounds__(UP_T) k_update_pk(BSet S, const LuTask* __restrict__ tiles, double* __restrict__ pk,
+__global__ void __launch_bounds__(UP_T) k_update_pk(BSet S, const LuTask* __restrict__ tiles, const double* __restrict__ pk,
                                                     const int2* __restrict__ wins) {
     using F = Pk<NV>;
     static_assert(UP_ROWS % TJ == 0, "a pass covers whole j rows of the tile");
@@ -830,13 +829,9 @@ __global__ void __launch_bounds__(UP_T) k_update_pk(BSet S, const LuTask* __rest
             const int l = slot % UP_ROWS, qo = slot / UP_ROWS;
             const int jl = (lane0 + l) % TJ, kl = (lane0 + l) / TJ, ci = qo - (jl + kl - kl0);
             if (jl < T_.tj && kl < T_.tk && ci >= 0 && ci < W) {
-                double* el = pk + T_.pofs + ((long long)(qb + qo) * F::NF + F::DUB) * LU_NT + lane0 + l;
+                const double* el = pk + T_.pofs + ((long long)(qb + qo) * F::NF + F::DUB) * LU_NT + lane0 + l;
 #pragma unroll
-                for (int m = 0; m < NV; m++) {
-                    buf[m][l][ci] = el[m * LU_NT];
-                    el[m * LU_NT] = sentinel_value();
-                    el[(F::DUF - F::DUB + m) * LU_NT] = sentinel_value();
-                }
+                for (int m = 0; m < NV; m++) buf[m][l][ci] = el[m * LU_NT];
             }
         }
         __syncthreads();
@@ -846,7 +841,7 @@ __global__ void __launch_bounds__(UP_T) k_update_pk(BSet S, const LuTask* __rest
             const size_t s = IX(b, i, j, k);
             double DU[MAXV];
 #pragma unroll
-            for (int m = 0; m < NV; m++) { DU[m] = buf[m][rr][c]; b.DU[m][s] = DU[m]; }
+            for (int m = 0; m < NV; m++) DU[m] = buf[m][rr][c];
             update_cell(b, p, i, j, k, s, DU);
         }
         __syncthreads();

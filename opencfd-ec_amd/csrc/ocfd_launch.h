@@ -39,7 +39,7 @@ void set_pre(const SetPlan& P, hipStream_t st);
 void set_inviscid(const SetPlan& P, hipStream_t st);
 void set_viscous_dt(const SetPlan& P, hipStream_t st);
 void set_update_limit(const SetPlan& P, hipStream_t st);
-// the update taking DU from the pipelined sweeps' storage (and putting the sentinel back)
+// the update taking DU from the pipelined sweeps' storage
 void set_update_limit_sweep_order(const SetPlan& P, const LuPlan& L, hipStream_t st);
 void set_bc(const SetPlan& P, hipStream_t st);
 void set_halo(const SetPlan& P, hipStream_t st);
@@ -71,7 +71,8 @@ void launch_lusgs_pack_geometry(const LuPlan& L, const Par& p, hipStream_t st);
 // the sweeps' per-iteration inputs (U, Res, Lc, Lv, the diagonal) into sweep order
 void launch_lusgs_pack(const LuPlan& L, const Par& p, hipStream_t st);
 void launch_lusgs_reset(const LuPlan& L, int NV, hipStream_t st);
-// the backward sweep's DU from the sweep-ordered storage into the block arrays
+// the backward sweep's DU from the sweep-ordered storage into the block arrays (valid from the
+// end of the backward sweep until the next pack)
 void launch_lusgs_du_to_box(const LuPlan& L, int NV, hipStream_t st);
 void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fenced, hipStream_t st);
 // device geometry precompute (ocfd_geometry.hip)

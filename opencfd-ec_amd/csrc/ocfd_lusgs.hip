@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(LU_NT) k_lusgs_sweep(const Blk* __restrict__ b
 // producer overwrites an element read by a neighbour tile with a device-coherent 64-bit store and
 // the consumer reloads it, device-coherent, until the sentinel is gone.  No progress counter and
 // no wait for store completion sit on the producer's path.  The forward sweep writes DUF and the
-// backward sweep DUB; the unpack of DUB into the block layout puts the sentinel back into both.
+// backward sweep DUB; the pack of the next sweeps' inputs puts the sentinel back into both.
 // Tiles are numbered upstream first and workgroups start in index order, so every wait is on a
 // tile that is running or done; every wait is also bounded by a poll limit that raises the error
 // flag.
@@ -235,7 +235,8 @@ __device__ __forceinline__ double ld_coherent(const double* p) {
 }
 
 
-// value of field f of the sweep-ordered storage at box index s (DUF/DUB are never packed)
+// value of field f of the sweep-ordered storage at box index s (not for DUF/DUB, whose packing
+// puts the sentinel into them)
 template <int NV>
 __device__ __forceinline__ double pack_value(const Blk& b, int f, size_t s) {
     using F = Pk<NV>;
@@ -254,6 +255,12 @@ constexpr int PK_T = 256;
 // i-window and one field per workgroup, through LDS so both sides move in contiguous runs.  Neighbouring i-windows of a tile
 // share the step rows at their boundary (each writes some lanes of them), so a tile's windows run
 // back to back on one XCD, where the L2 merges those partial lines before they go to memory.
+// A DU field's workgroup reads nothing and puts the sentinel into every element of its window's
+// cells, lanes without a cell included, so its stores fill whole lines: a downstream tile waits
+// only on the lanes of a tile's downstream edges, but a line written in part costs a
+// read-modify-write of the memory beneath it (storing only those lanes made the pack of the
+// 60.8 M-cell swept wing 3.0 ms slower, against 1.4 ms for whole lines; the U workgroups storing
+// the sentinel beside each U, instead of workgroups of their own, 1.7 ms).
 template <int NV, int TJ>
 __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
                                                        double* __restrict__ pk, const int2* __restrict__ wins, int f0, int f1, int f2) {
@@ -270,6 +277,14 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
     const int z = (int)blockIdx.z;
     const int f = z < f1 - f0 ? f0 + z : f2 + (z - (f1 - f0));
     double* pf = pk + (size_t)T.pofs + (size_t)f * LU_NT;     // element (q, f, lane) at pf[q NF 64 + lane]
+    if (f >= Pk<NV>::DUF && f < Pk<NV>::DUB + NV) {
+        for (int e = t; e < (W + TJ + LU_NT / TJ - 2) * LU_NT; e += PK_T) {
+            int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % TJ, kl = lane / TJ;
+            int i = q - jl - kl + 1;
+            if (i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * NF * LU_NT + lane] = sentinel_value();
+        }
+        return;
+    }
     for (int e = t; e < LU_NT * PK_W; e += PK_T) {
         int r = e / PK_W, c = e % PK_W, jl = r % TJ, kl = r / TJ;
         if (jl < T.tj && kl < T.tk && c < W) buf[r][c] = pack_value<NV>(b, f, IX(b, i0 + c, T.j0 + jl, T.k0 + kl));
@@ -283,7 +298,7 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
 }
 
 // sweep order -> block arrays for the backward sweep's DU (component m per grid z), for a host that
-// reads DU between the sweeps and the update; the storage keeps it, and the update reads it there
+// reads DU: the storage holds it from the end of the backward sweep until the next pack
 template <int NV, int TJ>
 __global__ void __launch_bounds__(PK_T) k_lu_du_to_box(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
                                                        const double* __restrict__ pk, const int2* __restrict__ wins) {
@@ -657,7 +672,7 @@ static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, h
             hipLaunchKernelGGL((k_lusgs_team<NV, true, TJ>), dim3((unsigned)L.n_fwd), dim3(LT_NT), 0, st,
                                L.blks, L.fwd, L.pk, L.err, p);
         } else {
-            // the update takes DU from the sweep-ordered storage (and puts the sentinel back)
+            // the update takes DU from the sweep-ordered storage
             hipLaunchKernelGGL((k_lusgs_team<NV, false, TJ>), dim3((unsigned)L.n_bwd), dim3(LT_NT), 0, st,
                                L.blks, L.bwd, L.pk, L.err, p);
         }
@@ -673,13 +688,14 @@ static void sweep_nv(const LuPlan& L, const Par& p, bool forward, bool fenced, h
 }
 
 // this iteration's sweep inputs into sweep order: U, Res, Lc, Lv and the diagonal (one field, or
-// three for SA)
+// three for SA), and the sentinel into the DU elements the sweeps wait on
 template <int NV, int TJ>
 static void pack_nv(const LuPlan& L, const Par& p, hipStream_t st) {
     using F = Pk<NV>;
     constexpr int NAL = NV == 6 ? 3 : 1;
-    hipLaunchKernelGGL((k_lu_transpose<NV, TJ>), dim3((unsigned)L.nwins, 1u, (unsigned)(F::DUF + 6 + NAL)),
-                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.win, F::U, F::DUF, F::LC);
+    constexpr int FEND = F::AL + NAL;
+    hipLaunchKernelGGL((k_lu_transpose<NV, TJ>), dim3((unsigned)L.nwins, 1u, (unsigned)(FEND - F::U)),
+                       dim3(PK_T), 0, st, L.blks, L.fwd, L.pk, L.win, F::U, FEND, FEND);
 }
 
 // the geometry the sweeps read (S, n), into sweep order once per plan

@@ -66,6 +66,9 @@ struct Engine {
     bool per_block = false;
     std::vector<Engine> parts;
     ocfd_batch* owner = nullptr;              // set for a batch's engine: its sweep storage is the batch's pool
+    // the last update took DU from the pipelined sweeps' storage, which still holds it: the block
+    // arrays' DU is older (flush_du copies it out)
+    bool du_pending = false;
 };
 
 // one block of an engine's list, the parameters of its case and the list position of its
@@ -368,6 +371,7 @@ void drop_graphs(Engine& e) {
 
 // One stage of the iteration: eager launches, or the stage's graph (captured on first use).
 int run_stage(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, int stage) {
+    if (stage == OCFD_STAGE_UPDATE && e.lu_schedule == OCFD_LU_PIPELINED) e.du_pending = true;
     if (!e.use_graph || e.per_block) { enqueue_stage(e, refs, p0, stage); return OCFD_OK; }
     if (!e.exec[stage]) {
         HIPCHK(hipStreamBeginCapture(e.st, hipStreamCaptureModeThreadLocal));
@@ -379,7 +383,23 @@ int run_stage(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, int s
     return OCFD_OK;
 }
 
+// The latest DU into the block arrays, from the sweep-ordered storage where the update read it
+// (in plain mode from every block's own storage).  Called before the storage or its tables go
+// away and when the host asks for DU.
+void flush_du(Engine& e) {
+    if (!e.du_pending) return;
+    e.du_pending = false;
+    if (e.lu_schedule != OCFD_LU_PIPELINED) return;
+    if (e.per_block) {
+        for (Engine& q : e.parts) if (q.lu.fwd) launch_lusgs_du_to_box(q.lu, q.lu_nv, e.st);
+    } else if (e.lu.fwd) {
+        launch_lusgs_du_to_box(e.lu, e.lu_nv, e.st);
+    }
+    (void)hipStreamSynchronize(e.st);
+}
+
 void drop_lu_plan(Engine& e) {
+    flush_du(e);
     for (void* q : e.lu_allocs) (void)hipFree(q);
     e.lu_allocs.clear();
     e.lu = LuPlan{};
@@ -610,6 +630,8 @@ int check_lu_error(Engine& e) {
     HIPCHK(hipMemcpy(e.lu.prog_bwd, minus.data(), e.lu.n_bwd * sizeof(int), hipMemcpyHostToDevice));
     int z = 0;
     HIPCHK(hipMemcpy(e.lu.err, &z, sizeof(int), hipMemcpyHostToDevice));
+    // the storage's DU is not an iteration's any more
+    e.du_pending = false;
     if (e.owner) {
         // a batch: every member's region as laid out on its own
         for (ocfd_ctx* c : e.owner->members) {
@@ -885,7 +907,10 @@ int prepare(ocfd_ctx* c) {
 
 int finish(ocfd_ctx* c) { return engine_finish(c->e); }
 
-void enqueue_stage(ocfd_ctx* c, int stage) { enqueue_stage(c->e, c->refs(), c->p, stage); }
+void enqueue_stage(ocfd_ctx* c, int stage) {
+    if (stage == OCFD_STAGE_UPDATE && c->e.lu_schedule == OCFD_LU_PIPELINED) c->e.du_pending = true;
+    enqueue_stage(c->e, c->refs(), c->p, stage);
+}
 
 // device buffer of the residual norms: the block table, the parts, the result (3 NV doubles)
 struct NormLayout { size_t tab, part, out, bytes; };
@@ -942,6 +967,7 @@ void enqueue_residual(ocfd_ctx* c) {
             launch_lusgs_du_to_box(e.lu, c->p.NV, e.st);
         }
     }
+    e.du_pending = false;
 }
 void enqueue_bc(ocfd_ctx* c) { set_bc(c->e.set, c->e.st); }
 void enqueue_halo(ocfd_ctx* c) { set_halo(c->e.set, c->e.st); }
@@ -1053,6 +1079,7 @@ int ocfd_destroy(ocfd_ctx* c) {
         drop_plans(b->e);
         c->batch = nullptr;
     }
+    c->e.du_pending = false;      // its DU goes with it
     engine_destroy(c->e);
     for (HostBlock& hb : c->blocks) for (void* q : hb.allocs) (void)hipFree(q);
     if (c->norm_buf) (void)hipFree(c->norm_buf);
@@ -1320,6 +1347,10 @@ int ocfd_download_field(ocfd_ctx* c, int32_t no, int32_t field, double* out) {
     HostBlock* hb = find_block(c, no);
     if (!hb || !out) return fail(OCFD_EARG, "unknown block or null output");
     HIPCHK(hipStreamSynchronize(c->e.st));
+    if (field == OCFD_F_DU) {
+        flush_du(c->batch ? c->batch->e : c->e);
+        HIPCHK(hipGetLastError());
+    }
     const Blk& b = hb->d;
     int nv = (field == OCFD_F_RES || field == OCFD_F_DU) ? c->p.NV : 1;
     long n0 = b.nx + 3, n1 = b.ny + 3, n2 = b.nz + 3;
@@ -1500,6 +1531,7 @@ int ocfd_set_per_block(ocfd_ctx* c, int32_t on) {
     if ((on != 0) == e.per_block) return OCFD_OK;
     if (on && e.lu_tj != 8) { int r = engine_set_lu_tile(e, 8); if (r) return r; }
     HIPCHK(hipStreamSynchronize(e.st));
+    flush_du(e);
     drop_graphs(e);
     drop_parts(e);
     e.per_block = on != 0;

@@ -20,10 +20,13 @@ def _cases():
     return [a, b, c]
 
 
-def _alone(case, seed, n):
+def _alone(case, seed, n, du=False):
     s = A.solver_from_case(case, states=C.perturbed_state(case, seed=seed, amp=1e-4))
     s.step(n)
-    return [s.download_U(no) for no in range(1, len(case.blocks) + 1)], s.norms()[1]
+    U = [s.download_U(no) for no in range(1, len(case.blocks) + 1)]
+    if du:
+        return U, s.norms()[1], [s.field(no, A.F_DU) for no in range(1, len(case.blocks) + 1)]
+    return U, s.norms()[1]
 
 
 @pytest.mark.gpu
@@ -41,9 +44,12 @@ def test_batch_members_equal_alone(graph):
     bt.step(1)                       # b: 5, c: 3
     iters = [4, 5, 3]
     for q, (cs, s) in enumerate(zip(cases, S)):
-        ref, ref_rms = _alone(cs, 10 + q, iters[q])
+        ref, ref_rms, ref_du = _alone(cs, 10 + q, iters[q], du=True)
         for no in range(1, len(cs.blocks) + 1):
             assert np.array_equal(s.download_U(no), ref[no - 1]), f"member {q} block {no}"
+            # the last update's DU: a member that left the batch took it along, the others read
+            # it from the batch's sweep storage
+            assert np.array_equal(s.field(no, A.F_DU), ref_du[no - 1]), f"member {q} block {no} DU"
         assert np.array_equal(s.norms()[1], ref_rms)
         assert s.counters()[0] == iters[q]
     bt.close()

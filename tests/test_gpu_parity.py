@@ -340,6 +340,7 @@ def test_per_block_launches_equal_batched(name):
         a.step(1); b.step(1)
         for no in range(1, len(case.blocks) + 1):
             assert np.array_equal(a.download_U(no), b.download_U(no)), f"iteration {it}: block {no} differs"
+            assert np.array_equal(a.field(no, A.F_DU), b.field(no, A.F_DU)), f"iteration {it}: block {no} DU"
     for sched in (A.LU_PIPELINED_FENCED, A.LU_HYPERPLANE):
         b.set_lu_schedule(sched)
         a.step(1); b.step(1)
@@ -347,6 +348,41 @@ def test_per_block_launches_equal_batched(name):
             assert np.array_equal(a.download_U(no), b.download_U(no)), f"schedule {sched}: block {no} differs"
     assert np.array_equal(a.norms()[1], b.norms()[1])
     a.close(); b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["channel_cut6", "cgrid_sa_cut12"])
+def test_residual_of_a_new_state_after_residual(name):
+    """ocfd_residual twice, with a new state handed over in between and no update: the second
+    residual's Res and DU equal a fresh solver's on that state bit for bit (the sweeps of the
+    second call must not take the first call's DU for an upstream tile's), and so does a step
+    after a residual."""
+    print(f"\n[{name}]", flush=True)
+    cs = CASES[name]
+    case = cs.builder()
+    U0 = C.perturbed_state(case, seed=cs.seed, amp=cs.amp)
+    U1 = C.perturbed_state(case, seed=cs.seed + 1, amp=cs.amp)
+    s = A.solver_from_case(case, states=U0)
+    s.set_graph(True)
+    s.residual()
+    s.set_interior(U1); s.bc(); s.halo()
+    s.residual()
+    r = A.solver_from_case(case, states=U1)
+    r.set_lu_schedule(A.LU_HYPERPLANE)
+    r.residual()
+    nb = len(case.blocks)
+    for no in range(1, nb + 1):
+        assert np.array_equal(s.field(no, A.F_RES), r.field(no, A.F_RES)), f"block {no} Res"
+        assert np.array_equal(s.field(no, A.F_DU), r.field(no, A.F_DU)), f"block {no} DU"
+    s.set_interior(U0); s.bc(); s.halo()
+    s.step(1)
+    q = A.solver_from_case(case, states=U0)
+    q.set_lu_schedule(A.LU_HYPERPLANE)
+    q.step(1)
+    for no in range(1, nb + 1):
+        assert np.array_equal(s.download_U(no), q.download_U(no)), f"block {no} U after a step"
+        assert np.array_equal(s.field(no, A.F_DU), q.field(no, A.F_DU)), f"block {no} DU after a step"
+    s.close(); r.close(); q.close()
 
 
 @pytest.mark.gpu

@@ -34,6 +34,10 @@ struct Blk {
     // mixed precision: FP32 primitives (rho u v w p T) and FP32 face terms; the face arrays share
     // the memory of their FP64 counterparts
     float *primf[6], *Ff[3][5], *Fvf[3][4], *Fsaf[3][2];
+    // mixed precision: FP32 copies of the geometry the FP32 kernels read (face normals, areas,
+    // metrics, and per direction the node difference t1 of the inviscid flux's face frame), made
+    // once per block from the FP64 arrays
+    float *nvf[9], *Sf[3], *metf[4][9], *t1f[3][3];
     double *aux;                         // turbulence scratch (SST f1)
     double *kt, *wt;                     // SST: k and omega per unit mass, max(U, 0) / rho, on the box
     uint8_t *bcf[3];                     // face-row flags: bit0 lower face physical, bit1 upper face physical

@@ -216,6 +216,35 @@ void launch_geometry(const Blk& b, const Sub* subs, int nsub, hipStream_t st) {
     if (b.met[0][0] != nullptr) hipLaunchKernelGGL(k_geo_metric0, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b);
 }
 
+// FP32 copies of the geometry the mixed-precision kernels read, once per block: face normals,
+// areas and metrics converted from the FP64 arrays, and per direction the node difference t1 of
+// the inviscid flux's face frame (sub_Residual.f90:199-205, 366-372, 533-539), the FP64 difference
+// converted as the FP64 kernels form it.  One thread per point of the padded box; t1 where both
+// of its nodes lie in the box.
+__global__ void k_geo_f32(Blk b) {
+    const long n = (long)b.NI * b.NJ * b.NK;
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    for (int q = 0; q < 9; q++) b.nvf[q][s] = (float)b.nv[q][s];
+    for (int d = 0; d < 3; d++) b.Sf[d][s] = (float)b.S[d][s];
+    for (int m = 0; m < 4; m++)
+        if (b.metf[m][0] != nullptr)
+            for (int q = 0; q < 9; q++) b.metf[m][q][s] = (float)b.met[m][q][s];
+    const int i = (int)(s % b.NI) - 3, j = (int)((s / b.NI) % b.NJ) - 3, k = (int)(s / ((long)b.NI * b.NJ)) - 3;
+    const bool ip = i + 1 <= b.nx + 3, jp = j + 1 <= b.ny + 3, kp = k + 1 <= b.nz + 3;
+    const double* X[3] = {b.x, b.y, b.z};
+    for (int c = 0; c < 3; c++) {
+        b.t1f[0][c][s] = jp && kp ? (float)(X[c][IX(b, i, j + 1, k)] - X[c][IX(b, i, j, k + 1)]) : 0.0f;
+        b.t1f[1][c][s] = ip && kp ? (float)(X[c][IX(b, i + 1, j, k + 1)] - X[c][s]) : 0.0f;
+        b.t1f[2][c][s] = ip && jp ? (float)(X[c][IX(b, i + 1, j + 1, k)] - X[c][s]) : 0.0f;
+    }
+}
+
+void launch_geometry_f32(const Blk& b, hipStream_t st) {
+    const long n = (long)b.NI * b.NJ * b.NK;
+    hipLaunchKernelGGL(k_geo_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, b);
+}
+
 void launch_mesh_quality(const Blk& b, hipStream_t st) {
     hipLaunchKernelGGL(k_mesh_quality, g3(b.nx - 1, b.ny - 1, b.nz - 1, 64), dim3(64), 0, st, b);
 }

@@ -1,6 +1,8 @@
 // Hot-path kernels of one LU-SGS iteration on MI355X (gfx950).  FP64, no contraction (the
 // library is built with -ffp-contract=off so results match the reference CPU build bit for bit
 // wherever no transcendental is involved).  Each kernel cites the reference routine it replaces.
+#include <type_traits>
+
 #include "ocfd_dev.h"
 #include "ocfd_launch.h"
 #include "ocfd_phys.h"
@@ -72,6 +74,17 @@ __device__ __forceinline__ double pw6(double x) { return ((((x * x) * x) * x) * 
 template <typename T> __device__ __forceinline__ const T* prim_of(const Blk& b, int m);
 template <> __device__ __forceinline__ const double* prim_of<double>(const Blk& b, int m) { return b.prim[m]; }
 template <> __device__ __forceinline__ const float* prim_of<float>(const Blk& b, int m) { return b.primf[m]; }
+// the geometry in the compute type: for FP32 the copies converted once per block, which hold the
+// values a conversion of the FP64 arrays at every read gives
+template <typename T> __device__ __forceinline__ const T* nv_of(const Blk& b, int q);
+template <> __device__ __forceinline__ const double* nv_of<double>(const Blk& b, int q) { return b.nv[q]; }
+template <> __device__ __forceinline__ const float* nv_of<float>(const Blk& b, int q) { return b.nvf[q]; }
+template <typename T> __device__ __forceinline__ const T* S_of(const Blk& b, int d);
+template <> __device__ __forceinline__ const double* S_of<double>(const Blk& b, int d) { return b.S[d]; }
+template <> __device__ __forceinline__ const float* S_of<float>(const Blk& b, int d) { return b.Sf[d]; }
+template <typename T> __device__ __forceinline__ const T* met_of(const Blk& b, int m, int q);
+template <> __device__ __forceinline__ const double* met_of<double>(const Blk& b, int m, int q) { return b.met[m][q]; }
+template <> __device__ __forceinline__ const float* met_of<float>(const Blk& b, int m, int q) { return b.metf[m][q]; }
 __device__ __forceinline__ void put_F(const Blk& b, int d, int m, size_t s, double v) { b.F[d][m][s] = v; }
 __device__ __forceinline__ void put_F(const Blk& b, int d, int m, size_t s, float v) { b.Ff[d][m][s] = v; }
 __device__ __forceinline__ void put_Fv(const Blk& b, int d, int m, size_t s, double v) { b.Fv[d][m][s] = v; }
@@ -124,7 +137,7 @@ __device__ __forceinline__ void sa_face(const Blk& b, const Par& p, int fi, int 
     const T* d = prim_of<T>(b, 0);
     const T *uu = prim_of<T>(b, 1), *vv = prim_of<T>(b, 2), *ww = prim_of<T>(b, 3);
     size_t f = IX(b, fi, fj, fk), fm = IX(b, fi - di, fj - dj, fk - dk);
-    T s1x = b.nv[3 * DIR][f], s1y = b.nv[3 * DIR + 1][f], s1z = b.nv[3 * DIR + 2][f];
+    T s1x = nv_of<T>(b, 3 * DIR)[f], s1y = nv_of<T>(b, 3 * DIR + 1)[f], s1z = nv_of<T>(b, 3 * DIR + 2)[f];
     T gi, gj, gk;
     auto VT = [&](int a, int bb, int cc) { return T(vt_of(b, IX(b, a, bb, cc))); };
     if (DIR == 0) {
@@ -140,8 +153,7 @@ __device__ __forceinline__ void sa_face(const Blk& b, const Par& p, int fi, int 
         gj = T(0.25) * (((VT(fi, fj + 1, fk) - VT(fi, fj - 1, fk)) + VT(fi, fj + 1, fk - 1)) - VT(fi, fj - 1, fk - 1));
         gk = VT(fi, fj, fk) - VT(fi, fj, fk - 1);
     }
-    double* const* mt = b.met[DIR + 1];
-    auto M = [&](int q) { return T(mt[q][f]); };
+    auto M = [&](int q) { return met_of<T>(b, DIR + 1, q)[f]; };
     T vtx = (gi * M(0) + gj * M(3)) + gk * M(6);
     T vty = (gi * M(1) + gj * M(4)) + gk * M(7);
     T vtz = (gi * M(2) + gj * M(5)) + gk * M(8);
@@ -150,7 +162,7 @@ __device__ __forceinline__ void sa_face(const Blk& b, const Par& p, int fi, int 
     T vn2 = (uu[f] * s1x + vv[f] * s1y) + ww[f] * s1z;
     T vfi = T(0.5) * ((vn1 + fabs(vn1)) * T(vt_of(b, fm)) + (vn2 - fabs(vn2)) * T(vt_of(b, f)));
     T gn = (vtx * s1x + vty * s1y) + vtz * s1z;
-    T Sf = T(b.S[DIR][f]);
+    T Sf = S_of<T>(b, DIR)[f];
     fv = (-vfi + (v0 / Re) * gn) * Sf;
     f2 = gn * Sf;
 }
@@ -182,8 +194,8 @@ __device__ __forceinline__ double sa_residual(const Blk& b, const Par& p, int i,
     T ui = UU(i + 1, j, k) - UU(i - 1, j, k), vi = VV(i + 1, j, k) - VV(i - 1, j, k), wi = WW(i + 1, j, k) - WW(i - 1, j, k);
     T uj = UU(i, j + 1, k) - UU(i, j - 1, k), vj = VV(i, j + 1, k) - VV(i, j - 1, k), wj = WW(i, j + 1, k) - WW(i, j - 1, k);
     T uk = UU(i, j, k + 1) - UU(i, j, k - 1), vk = VV(i, j, k + 1) - VV(i, j, k - 1), wk = WW(i, j, k + 1) - WW(i, j, k - 1);
-    double* const* m0 = b.met[0];
-    T ix = T(m0[0][s]), iy = T(m0[1][s]), iz = T(m0[2][s]), jx = T(m0[3][s]), jy = T(m0[4][s]), jz = T(m0[5][s]), kx = T(m0[6][s]), ky = T(m0[7][s]), kz = T(m0[8][s]);
+    auto M0 = [&](int q) { return met_of<T>(b, 0, q)[s]; };
+    T ix = M0(0), iy = M0(1), iz = M0(2), jx = M0(3), jy = M0(4), jz = M0(5), kx = M0(6), ky = M0(7), kz = M0(8);
     T vx = (vi * ix + vj * jx) + vk * kx, wx = (wi * ix + wj * jx) + wk * kx;
     T uy = (ui * iy + uj * jy) + uk * ky, wy = (wi * iy + wj * jy) + wk * ky;
     T uz = (ui * iz + uj * jz) + uk * kz, vz = (vi * iz + vj * jz) + vk * kz;
@@ -239,8 +251,8 @@ __global__ void __launch_bounds__(64) k_sst_cell(BSet S) {
     T ui = DIFF(uu, 1, 0, 0), vi = DIFF(vv, 1, 0, 0), wi = DIFF(ww, 1, 0, 0), kti = DK(1, 0, 0), wti = DW(1, 0, 0);
     T uj = DIFF(uu, 0, 1, 0), vj = DIFF(vv, 0, 1, 0), wj = DIFF(ww, 0, 1, 0), ktj = DK(0, 1, 0), wtj = DW(0, 1, 0);
     T uk = DIFF(uu, 0, 0, 1), vk = DIFF(vv, 0, 0, 1), wk = DIFF(ww, 0, 0, 1), ktk = DK(0, 0, 1), wtk = DW(0, 0, 1);
-    double* const* m0 = b.met[0];
-    T ix = T(m0[0][s]), iy = T(m0[1][s]), iz = T(m0[2][s]), jx = T(m0[3][s]), jy = T(m0[4][s]), jz = T(m0[5][s]), kx = T(m0[6][s]), ky = T(m0[7][s]), kz = T(m0[8][s]);
+    auto M0 = [&](int q) { return met_of<T>(b, 0, q)[s]; };
+    T ix = M0(0), iy = M0(1), iz = M0(2), jx = M0(3), jy = M0(4), jz = M0(5), kx = M0(6), ky = M0(7), kz = M0(8);
     T vx = (vi * ix + vj * jx) + vk * kx, wx = (wi * ix + wj * jx) + wk * kx;
     T ktx = (kti * ix + ktj * jx) + ktk * kx, wtx = (wti * ix + wtj * jx) + wtk * kx;
     T uy = (ui * iy + uj * jy) + uk * ky, wy = (wi * iy + wj * jy) + wk * ky;
@@ -328,17 +340,16 @@ __global__ void __launch_bounds__(64) k_sst_face(BSet S) {
         wtj = q * (((W(fi, fj + 1, fk_) - W(fi, fj - 1, fk_)) + W(fi, fj + 1, fk_ - 1)) - W(fi, fj - 1, fk_ - 1));
         wtk = W(fi, fj, fk_) - W(fi, fj, fk_ - 1);
     }
-    double* const* mt = b.met[DIR + 1];
-    auto M = [&](int c) { return T(mt[c][f]); };
+    auto M = [&](int c) { return met_of<T>(b, DIR + 1, c)[f]; };
     T ktx = (kti * M(0) + ktj * M(3)) + ktk * M(6), wtx = (wti * M(0) + wtj * M(3)) + wtk * M(6);
     T kty = (kti * M(1) + ktj * M(4)) + ktk * M(7), wty = (wti * M(1) + wtj * M(4)) + wtk * M(7);
     T ktz = (kti * M(2) + ktj * M(5)) + ktk * M(8), wtz = (wti * M(2) + wtj * M(5)) + wtk * M(8);
-    T s1x = T(b.nv[3 * DIR][f]), s1y = T(b.nv[3 * DIR + 1][f]), s1z = T(b.nv[3 * DIR + 2][f]);
+    T s1x = nv_of<T>(b, 3 * DIR)[f], s1y = nv_of<T>(b, 3 * DIR + 1)[f], s1z = nv_of<T>(b, 3 * DIR + 2)[f];
     T vn1 = (uu[fm] * s1x + vv[fm] * s1y) + ww[fm] * s1z;
     T vn2 = (uu[f] * s1x + vv[f] * s1y) + ww[f] * s1z;
     T kfi = T(0.5) * ((vn1 + fabs(vn1)) * K(fi - di, fj - dj, fk_ - dk) + (vn2 - fabs(vn2)) * K(fi, fj, fk_));
     T wfi = T(0.5) * ((vn1 + fabs(vn1)) * W(fi - di, fj - dj, fk_ - dk) + (vn2 - fabs(vn2)) * W(fi, fj, fk_));
-    T Sf = T(b.S[DIR][f]);
+    T Sf = S_of<T>(b, DIR)[f];
     b.Fsa[DIR][0][f] = (double)((-kfi + muk * ((ktx * s1x + kty * s1y) + ktz * s1z)) * Sf);
     b.Fsa[DIR][1][f] = (double)((-wfi + muw * ((wtx * s1x + wty * s1y) + wtz * s1z)) * Sf);
 }
@@ -393,9 +404,10 @@ __device__ __forceinline__ void inv_face(const Blk& b, const Par& p, int i, int 
     if (fl & 1) { if (pos == 1) scheme = S_CD2; else if (pos < LAP) scheme = p.bscheme; }
     if (fl & 2) { if (pos == nd) scheme = S_CD2; else if (pos > nd - LAP + 1) scheme = p.bscheme; }
     if (b.flags[0] == 1) scheme = S_UD1;
-    T s1x = T(b.nv[3 * DIR][s]), s1y = T(b.nv[3 * DIR + 1][s]), s1z = T(b.nv[3 * DIR + 2][s]);
+    T s1x = nv_of<T>(b, 3 * DIR)[s], s1y = nv_of<T>(b, 3 * DIR + 1)[s], s1z = nv_of<T>(b, 3 * DIR + 2)[s];
     T t1x, t1y, t1z;
-    if (DIR == 0) { size_t a = IX(b, i, j + 1, k), c = IX(b, i, j, k + 1); t1x = T(b.x[a] - b.x[c]); t1y = T(b.y[a] - b.y[c]); t1z = T(b.z[a] - b.z[c]); }
+    if constexpr (std::is_same<T, float>::value) { t1x = b.t1f[DIR][0][s]; t1y = b.t1f[DIR][1][s]; t1z = b.t1f[DIR][2][s]; }
+    else if (DIR == 0) { size_t a = IX(b, i, j + 1, k), c = IX(b, i, j, k + 1); t1x = T(b.x[a] - b.x[c]); t1y = T(b.y[a] - b.y[c]); t1z = T(b.z[a] - b.z[c]); }
     else if (DIR == 1) { size_t a = IX(b, i + 1, j, k + 1); t1x = T(b.x[a] - b.x[s]); t1y = T(b.y[a] - b.y[s]); t1z = T(b.z[a] - b.z[s]); }
     else { size_t a = IX(b, i + 1, j + 1, k); t1x = T(b.x[a] - b.x[s]); t1y = T(b.y[a] - b.y[s]); t1z = T(b.z[a] - b.z[s]); }
     T t10 = T(1) / sqrt((t1x * t1x + t1y * t1y) + t1z * t1z);
@@ -469,7 +481,7 @@ __device__ __forceinline__ void inv_face(const Blk& b, const Par& p, int i, int 
     QR[0] = UR[0]; QR[1] = (UR[1] * s1x + UR[2] * s1y) + UR[3] * s1z; QR[2] = (UR[1] * s2x + UR[2] * s2y) + UR[3] * s2z;
     QR[3] = (UR[1] * s3x + UR[2] * s3y) + UR[3] * s3z; QR[4] = UR[4];
     riemann_t<FLUX, T>(QL, QR, F0, T(p.gamma), p.flux);
-    T Sf = T(b.S[DIR][s]);
+    T Sf = S_of<T>(b, DIR)[s];
     put_F(b, DIR, 0, s, -F0[0] * Sf);
     put_F(b, DIR, 1, s, -((F0[1] * s1x + F0[2] * s2x) + F0[3] * s3x) * Sf);
     put_F(b, DIR, 2, s, -((F0[1] * s1y + F0[2] * s2y) + F0[3] * s3y) * Sf);
@@ -544,8 +556,8 @@ __device__ __forceinline__ void visc_face(const Blk& b, const Par& p, int i, int
         else if (DIR == 1) { gi[q] = ga; gj[q] = dn; gk[q] = gb; }
         else { gi[q] = ga; gj[q] = gb; gk[q] = dn; }
     }
-    double* const* mt = b.met[DIR + 1];
-    T ix = T(mt[0][s]), iy = T(mt[1][s]), iz = T(mt[2][s]), jx = T(mt[3][s]), jy = T(mt[4][s]), jz = T(mt[5][s]), kx = T(mt[6][s]), ky = T(mt[7][s]), kz = T(mt[8][s]);
+    auto M = [&](int q) { return met_of<T>(b, DIR + 1, q)[s]; };
+    T ix = M(0), iy = M(1), iz = M(2), jx = M(3), jy = M(4), jz = M(5), kx = M(6), ky = M(7), kz = M(8);
     T ux = (gi[0] * ix + gj[0] * jx) + gk[0] * kx, vx = (gi[1] * ix + gj[1] * jx) + gk[1] * kx;
     T wx = (gi[2] * ix + gj[2] * jx) + gk[2] * kx, Tx = (gi[3] * ix + gj[3] * jx) + gk[3] * kx;
     T uy = (gi[0] * iy + gj[0] * jy) + gk[0] * ky, vy = (gi[1] * iy + gj[1] * jy) + gk[1] * ky;
@@ -562,8 +574,8 @@ __device__ __forceinline__ void visc_face(const Blk& b, const Par& p, int i, int
     T E1 = ((u1 * s11 + v1 * s12) + w1 * s13) + k0 * Tx;
     T E2 = ((u1 * s12 + v1 * s22) + w1 * s23) + k0 * Ty;
     T E3 = ((u1 * s13 + v1 * s23) + w1 * s33) + k0 * Tz;
-    T s1x = T(b.nv[3 * DIR][s]), s1y = T(b.nv[3 * DIR + 1][s]), s1z = T(b.nv[3 * DIR + 2][s]);
-    T Sf = T(b.S[DIR][s]);
+    T s1x = nv_of<T>(b, 3 * DIR)[s], s1y = nv_of<T>(b, 3 * DIR + 1)[s], s1z = nv_of<T>(b, 3 * DIR + 2)[s];
+    T Sf = S_of<T>(b, DIR)[s];
     T f1 = ((s11 * s1x + s12 * s1y) + s13 * s1z) * Sf;
     T f2 = ((s12 * s1x + s22 * s1y) + s23 * s1z) * Sf;
     T f3 = ((s13 * s1x + s23 * s1y) + s33 * s1z) * Sf;

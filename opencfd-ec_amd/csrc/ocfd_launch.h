@@ -78,5 +78,7 @@ void launch_lusgs_pipelined(const LuPlan& L, const Par& p, bool forward, bool fe
 // device geometry precompute (ocfd_geometry.hip)
 void launch_geometry(const Blk& b, const Sub* subs, int nsub, hipStream_t st);
 void launch_mesh_quality(const Blk& b, hipStream_t st);
+// the FP32 copies of the geometry for the mixed-precision kernels (a block of a mixed context)
+void launch_geometry_f32(const Blk& b, hipStream_t st);
 void launch_wall_distance(const Blk& b, const double* wall_xyz, long npw, hipStream_t st);
 }  // namespace ocfd

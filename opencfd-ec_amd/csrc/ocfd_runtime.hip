@@ -35,6 +35,7 @@ struct HostBlock {
     std::vector<void*> allocs;
     std::vector<double> nodes[3];    // node coordinates awaiting ocfd_setup_geometry
     size_t bytes = 0;                // device memory of the block's arrays
+    bool geof_ok = false;            // mixed precision: the FP32 geometry copies match the FP64 arrays
     // the block's arrays are carved out of one device allocation: a sizing pass measures it
     char* slab = nullptr;
     size_t slab_off = 0;
@@ -177,6 +178,18 @@ int carve_block(ocfd_ctx* c, HostBlock& hb) {
             if ((r = dalloc(hb, &a, (n + 1) / 2))) return r;
             b.primf[q] = reinterpret_cast<float*>(a);
         }
+        // FP32 geometry: normals, areas, the face metrics (and the cell metrics with a turbulence
+        // model), the inviscid flux's node differences
+        auto falloc = [&](float** q) {
+            double* a = nullptr;
+            int rr = dalloc(hb, &a, (n + 1) / 2);
+            *q = reinterpret_cast<float*>(a);
+            return rr;
+        };
+        for (int q = 0; q < 9; q++) if ((r = falloc(&b.nvf[q]))) return r;
+        for (int q = 0; q < 3; q++) if ((r = falloc(&b.Sf[q]))) return r;
+        for (int m = turb ? 0 : 1; m < 4; m++) for (int q = 0; q < 9; q++) if ((r = falloc(&b.metf[m][q]))) return r;
+        for (int dd = 0; dd < 3; dd++) for (int q = 0; q < 3; q++) if ((r = falloc(&b.t1f[dd][q]))) return r;
         for (int dd = 0; dd < 3; dd++) {
             for (int m = 0; m < 5; m++) b.Ff[dd][m] = reinterpret_cast<float*>(b.F[dd][m]);
             for (int m = 0; m < 4; m++) b.Fvf[dd][m] = reinterpret_cast<float*>(b.Fv[dd][m]);
@@ -760,6 +773,11 @@ int build_set_plan(Engine& e, const std::vector<BlockRef>& refs, const Par& p0) 
 int engine_prepare(Engine& e, const std::vector<BlockRef>& refs, const Par& p0, bool lu) {
     int r = build_set_plan(e, refs, p0);
     if (r) return r;
+    if (p0.mixed) {
+        for (const BlockRef& q : refs)
+            if (!q.hb->geof_ok) { launch_geometry_f32(q.hb->d, e.st); q.hb->geof_ok = true; }
+        HIPCHK(hipGetLastError());
+    }
     if (e.per_block) {
         if (e.parts.size() != refs.size()) {
             for (Engine& q : e.parts) drop_plans(q);
@@ -1243,6 +1261,7 @@ int ocfd_setup_geometry(ocfd_ctx* c) {
         if (hb.nodes[0].empty()) return fail(OCFD_EARG, "ocfd_setup_geometry needs every block added with ocfd_add_block_nodes");
         if ((r = check_subs(c, hb))) return r;
     }
+    for (HostBlock& hb : c->blocks) hb.geof_ok = false;
     std::vector<std::vector<double>> X[3];
     for (int a = 0; a < 3; a++) {
         X[a].resize(c->blocks.size());

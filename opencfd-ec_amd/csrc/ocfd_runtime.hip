@@ -808,7 +808,7 @@ int engine_finish(Engine& e) {
     HIPCHK(hipStreamSynchronize(e.st));
     int r = OCFD_OK;
     for (Engine& q : e.parts) { int rq = check_lu_error(q); if (rq && !r) r = rq; }
-    if (r) return r;
+    if (r) { e.du_pending = false; return r; }     // the parts' storage holds no iteration's DU
     return check_lu_error(e);
 }
 

@@ -214,7 +214,7 @@ constexpr int LU_NT = 64;
 
 // The sweep-ordered storage.  For a tile, step q (the cells its lanes visit at forward step q; the
 // backward sweep visits them at step nsteps-1-q) holds NF fields of 64 consecutive doubles, one
-// per lane: element (q, f, lane) at pofs + (q NF + f) 64 + lane.  A step's inputs therefore sit in
+// per lane: element (q, f, lane) at pofs + (q NF + f) 64 + lane_pos(lane).  A step's inputs therefore sit in
 // one contiguous run that a lane addresses from one pointer with constant offsets.  AL holds the
 // LU-SGS diagonal: alfa of components 1..5 (and of 6, 7 for SST) and, for SA only, the SA
 // component's alfa of the forward (AL+1) and of the backward sweep (AL+2), which sum the viscous
@@ -226,6 +226,33 @@ template <int NV> struct Pk {
     static constexpr int U = 0, R = NV, DUF = 2 * NV, DUB = 3 * NV, LC = 4 * NV, LV = 4 * NV + 3, AL = 4 * NV + 6,
                          S = 4 * NV + 9, N = 4 * NV + 12, NF = 4 * NV + 21;
 };
+// Position of lane (jl, kl) = lane % TJ, lane / TJ of a TJ x (64/TJ) tile inside a step's run of 64
+// elements: first the tile's j-edge columns (jl = TJ-1, then jl = 0), then the rest of its k-edge
+// rows (kl = TK-1, then kl = 0), then the interior row by row.  The lanes a downstream tile reads
+// across an edge therefore share few 128-byte lines (8x8 tiles: one line per field for the
+// j-edge and two for the k-edge, against four and one in row order).
+template <int TJ>
+__device__ __host__ __forceinline__ int lane_pos(int lane) {
+    constexpr int TK = 64 / TJ;
+    const int jl = lane % TJ, kl = lane / TJ;
+    if (jl == TJ - 1) return kl;
+    if (jl == 0) return TK + kl;
+    if (kl == TK - 1) return 2 * TK + (jl - 1);
+    if (kl == 0) return 2 * TK + (TJ - 2) + (jl - 1);
+    return 2 * TK + 2 * (TJ - 2) + (kl - 1) * (TJ - 2) + (jl - 1);
+}
+// the lane at position pos of a step's run (the inverse of lane_pos)
+template <int TJ>
+__device__ __host__ __forceinline__ int pos_lane(int pos) {
+    constexpr int TK = 64 / TJ;
+    int jl, kl;
+    if (pos < TK) { jl = TJ - 1; kl = pos; }
+    else if (pos < 2 * TK) { jl = 0; kl = pos - TK; }
+    else if (pos < 2 * TK + (TJ - 2)) { kl = TK - 1; jl = pos - 2 * TK + 1; }
+    else if (pos < 2 * TK + 2 * (TJ - 2)) { kl = 0; jl = pos - 2 * TK - (TJ - 2) + 1; }
+    else { const int r = pos - 2 * TK - 2 * (TJ - 2); kl = r / (TJ - 2) + 1; jl = r % (TJ - 2) + 1; }
+    return jl + TJ * kl;
+}
 constexpr int LU_PAD_STEPS = 24;
 constexpr int PK_W = 32;      // i-window of one workgroup of the sweep-order transposes and cell pass
 

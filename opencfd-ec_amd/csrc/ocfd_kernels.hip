@@ -806,11 +806,11 @@ __global__ void __launch_bounds__(64) k_update_detect(BSet S) {
 
 // The update of an iteration with the pipelined sweeps: the backward sweep's DU comes from the
 // sweep-ordered storage.  A workgroup takes one sweep tile's cells of an i-window (the tiles and
-// windows of that storage), UP_ROWS lanes of the tile per pass.  A pass reads its lanes' DU along
-// the steps (each step's UP_ROWS lanes are one contiguous run) and parks it in LDS; then its
-// threads update the pass's cells, i fastest.  DU stays in the storage until the next iteration's
-// pack; the block arrays get it only when the host asks for it.  Lane l of a pass holds cell
-// i = q - d(l) + 1 at step q, d = jl + kl.  After a sweep gave up, the update leaves U and the
+// windows of that storage), the lanes of UP_ROWS consecutive positions of a step's run per pass
+// (one 128-byte line of each step).  A pass reads its lanes' DU along the steps and parks it in
+// LDS; then its threads update the pass's cells, i fastest.  DU stays in the storage until the
+// next iteration's pack; the block arrays get it only when the host asks for it.  Lane (jl, kl)
+// holds cell i at step i - 1 + jl + kl.  After a sweep gave up, the update leaves U and the
 // storage as they are; the host puts the sentinel back into the whole storage.
 constexpr int UP_ROWS = 16, UP_T = UP_ROWS * PK_W;
 
@@ -818,9 +818,8 @@ template <int NV, int TJ>
 __global__ void __launch_bounds__(UP_T) k_update_pk(BSet S, const LuTask* __restrict__ tiles, const double* __restrict__ pk,
                                                     const int2* __restrict__ wins) {
     using F = Pk<NV>;
-    static_assert(UP_ROWS % TJ == 0, "a pass covers whole j rows of the tile");
-    constexpr int DSPAN = TJ - 1 + UP_ROWS / TJ - 1;          // spread of d over a pass's lanes
-    constexpr int SLOTS = (PK_W + DSPAN) * UP_ROWS;           // (step, lane) slots a pass can touch
+    constexpr int DMAX = TJ + LU_NT / TJ - 2;                 // largest jl + kl of a tile
+    constexpr int SLOTS = (PK_W + DMAX) * UP_ROWS;            // (step, position) slots a pass can touch
     __shared__ double buf[NV][UP_ROWS][PK_W + 1];
     if (halted(S)) return;
     const int2 wn = wins[xcd_tile((int)blockIdx.x, (int)gridDim.x)];     // (tile, i-window), windows fastest
@@ -832,22 +831,21 @@ __global__ void __launch_bounds__(UP_T) k_update_pk(BSet S, const LuTask* __rest
     const int W = min(PK_W, nxm - i0 + 1);
     const int t = (int)threadIdx.x, rr = t / PK_W, c = t % PK_W;
 #pragma unroll 1
-    for (int lane0 = 0; lane0 < LU_NT; lane0 += UP_ROWS) {
-        const int kl0 = lane0 / TJ;
-        if (kl0 >= T_.tk) break;                        // the same for the whole workgroup
-        // step qb + qo: lane l (d = dmin + dl) holds cell i0 + qo - dl there
-        const int qb = i0 - 1 + kl0;
+    for (int pos0 = 0; pos0 < LU_NT; pos0 += UP_ROWS) {
+        // step i0 - 1 + qo: the lane at position pos0 + l holds cell i0 + qo - (jl + kl) there
         for (int slot = t; slot < SLOTS; slot += UP_T) {
             const int l = slot % UP_ROWS, qo = slot / UP_ROWS;
-            const int jl = (lane0 + l) % TJ, kl = (lane0 + l) / TJ, ci = qo - (jl + kl - kl0);
+            const int lane = pos_lane<TJ>(pos0 + l);
+            const int jl = lane % TJ, kl = lane / TJ, ci = qo - (jl + kl);
             if (jl < T_.tj && kl < T_.tk && ci >= 0 && ci < W) {
-                const double* el = pk + T_.pofs + ((long long)(qb + qo) * F::NF + F::DUB) * LU_NT + lane0 + l;
+                const double* el = pk + T_.pofs + ((long long)(i0 - 1 + qo) * F::NF + F::DUB) * LU_NT + pos0 + l;
 #pragma unroll
                 for (int m = 0; m < NV; m++) buf[m][l][ci] = el[m * LU_NT];
             }
         }
         __syncthreads();
-        const int jl = (lane0 + rr) % TJ, kl = (lane0 + rr) / TJ;
+        const int lane = pos_lane<TJ>(pos0 + rr);
+        const int jl = lane % TJ, kl = lane / TJ;
         if (jl < T_.tj && kl < T_.tk && c < W) {
             const int i = i0 + c, j = T_.j0 + jl, k = T_.k0 + kl;
             const size_t s = IX(b, i, j, k);

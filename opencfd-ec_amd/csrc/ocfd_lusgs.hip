@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
         for (int e = t; e < (W + TJ + LU_NT / TJ - 2) * LU_NT; e += PK_T) {
             int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % TJ, kl = lane / TJ;
             int i = q - jl - kl + 1;
-            if (i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * NF * LU_NT + lane] = sentinel_value();
+            if (i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * NF * LU_NT + lane_pos<TJ>(lane)] = sentinel_value();
         }
         return;
     }
@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
     for (int e = t; e < (W + TJ + LU_NT / TJ - 2) * LU_NT; e += PK_T) {
         int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % TJ, kl = lane / TJ;
         int i = q - jl - kl + 1;
-        if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * NF * LU_NT + lane] = buf[lane][i - i0];
+        if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * NF * LU_NT + lane_pos<TJ>(lane)] = buf[lane][i - i0];
     }
 }
 
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(PK_T) k_lu_du_to_box(const Blk* __restrict__ b
     for (int e = t; e < (W + TJ + LU_NT / TJ - 2) * LU_NT; e += PK_T) {
         int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % TJ, kl = lane / TJ;
         int i = q - jl - kl + 1;
-        if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) buf[lane][i - i0] = pf[(size_t)q * NF * LU_NT + lane];
+        if (jl < T.tj && kl < T.tk && i >= i0 && i < i0 + W && q < nsteps) buf[lane][i - i0] = pf[(size_t)q * NF * LU_NT + lane_pos<TJ>(lane)];
     }
     __syncthreads();
     double* box = b.DU[m];
@@ -484,10 +484,11 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
     const int i0 = FWD ? 1 - jo - ko : nxm + jo + ko;
     const int dir = FWD ? 1 : -1;
     // this lane's element at step 0 and per step
-    const long own0 = (long)T.pofs + (FWD ? 0L : (long)(nsteps - 1) * STEP) + lane;
+    const long base0 = (long)T.pofs + (FWD ? 0L : (long)(nsteps - 1) * STEP);
+    const long own0 = base0 + lane_pos<TJ>(lane);
     const long dstep = FWD ? STEP : -STEP;
     // the element of lane src one step back
-    const long src0 = own0 - dstep + (src - lane);
+    const long src0 = base0 - dstep + lane_pos<TJ>(src);
     // the upstream element: the upstream tile stores by forward step, the lane across the edge is
     // jlu + 8 kl (j) or jl + 8 klu (k); lanes without one follow their own stream
     long rem0 = own0;
@@ -497,7 +498,7 @@ __device__ void team_sweep(TeamShared<NV>& sh, const Blk& b, const LuTask& T, do
         const int lane_r = D == 1 ? jlu + TJ * kl : jl + TJ * klu;
         // forward step of the upstream cell at this lane's step 0 (forward: i - 1 + jl_r + kl_r)
         const int s0 = FWD ? (D == 1 ? jlu : klu) : nxm + (D == 1 ? T.tk : T.tj) - 2;
-        rem0 = (long)pofs_r + (long)s0 * STEP + lane_r;
+        rem0 = (long)pofs_r + (long)s0 * STEP + lane_pos<TJ>(lane_r);
     }
     const long nb0 = remote ? rem0 : src0;
     auto own_p = [&](int q) { return pk + own0 + (long)q * dstep; };

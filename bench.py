@@ -53,9 +53,9 @@ PASS_STAGES = (("residual", ("pre", "inviscid", "viscous_dt")), ("lu_forward", (
 # HBM bytes per cell of each pass, measured with rocprofv3 FETCH_SIZE and WRITE_SIZE on the MI355X
 # for the single-block C-grid in FP64 and scaled by a plain device copy of known size (the summary
 # and its commands: profiles/iteration_kernels_naca0012_cgrid_257x97x33.md)
-MEASURED_TRAFFIC = {("naca0012_cgrid_257x97x33", "fp64"): {"lu_forward": 502.8},
+MEASURED_TRAFFIC = {("naca0012_cgrid_257x97x33", "fp64"): {"lu_forward": 457.8, "lu_backward": 491.8},
                     # profiles/iteration_kernels_swept_wing_24block.md (k_lusgs_team<6, true|false, 8>)
-                    ("swept_wing_24block", "fp64"): {"lu_forward": 546.5, "lu_backward": 556.5}}
+                    ("swept_wing_24block", "fp64"): {"lu_forward": 488.6, "lu_backward": 494.8}}
 # stages that hold exactly one kernel launch per iteration: (kernel, the pass whose per-cell words
 # are its algorithmic bytes)
 SINGLE_KERNEL_STAGES = {"lu_forward": ("k_lusgs_team<{nv}, true, {tj}>", 1), "lu_backward": ("k_lusgs_team<{nv}, false, {tj}>", 2)}

@@ -255,12 +255,12 @@ constexpr int PK_T = 256;
 // i-window and one field per workgroup, through LDS so both sides move in contiguous runs.  Neighbouring i-windows of a tile
 // share the step rows at their boundary (each writes some lanes of them), so a tile's windows run
 // back to back on one XCD, where the L2 merges those partial lines before they go to memory.
-// A DU field's workgroup reads nothing and puts the sentinel into every element of its window's
-// cells, lanes without a cell included, so its stores fill whole lines: a downstream tile waits
-// only on the lanes of a tile's downstream edges, but a line written in part costs a
-// read-modify-write of the memory beneath it (storing only those lanes made the pack of the
-// 60.8 M-cell swept wing 3.0 ms slower, against 1.4 ms for whole lines; the U workgroups storing
-// the sentinel beside each U, instead of workgroups of their own, 1.7 ms).
+// A DU field's workgroup reads nothing and puts the sentinel into the elements of its window's
+// cells that a downstream tile may wait on, in whole lines: a line written in part costs a
+// read-modify-write of the memory beneath it (storing only the edge lanes of each step, before
+// they led the run, made the pack of the 60.8 M-cell swept wing 3.0 ms slower, against 1.4 ms
+// for whole steps; the U workgroups storing the sentinel beside each U, instead of workgroups of
+// their own, 1.7 ms).
 template <int NV, int TJ>
 __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ blks, const LuTask* __restrict__ tiles,
                                                        double* __restrict__ pk, const int2* __restrict__ wins, int f0, int f1, int f2) {
@@ -278,10 +278,14 @@ __global__ void __launch_bounds__(PK_T) k_lu_transpose(const Blk* __restrict__ b
     const int f = z < f1 - f0 ? f0 + z : f2 + (z - (f1 - f0));
     double* pf = pk + (size_t)T.pofs + (size_t)f * LU_NT;     // element (q, f, lane) at pf[q NF 64 + lane]
     if (f >= Pk<NV>::DUF && f < Pk<NV>::DUB + NV) {
-        for (int e = t; e < (W + TJ + LU_NT / TJ - 2) * LU_NT; e += PK_T) {
-            int q = i0 - 1 + e / LU_NT, lane = e % LU_NT, jl = lane % TJ, kl = lane / TJ;
+        // a full tile's edge lanes lead the run (lane_pos): whole lines up to the last of them;
+        // a ragged tile's edges lie elsewhere, so all of its positions
+        constexpr int NSENT = (2 * (LU_NT / TJ) + 2 * (TJ - 2) + 15) / 16 * 16;
+        const int npos = T.tj == TJ && T.tk == LU_NT / TJ ? NSENT : LU_NT;
+        for (int e = t; e < (W + TJ + LU_NT / TJ - 2) * npos; e += PK_T) {
+            int q = i0 - 1 + e / npos, pos = e % npos, lane = pos_lane<TJ>(pos), jl = lane % TJ, kl = lane / TJ;
             int i = q - jl - kl + 1;
-            if (i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * NF * LU_NT + lane_pos<TJ>(lane)] = sentinel_value();
+            if (i >= i0 && i < i0 + W && q < nsteps) pf[(size_t)q * NF * LU_NT + pos] = sentinel_value();
         }
         return;
     }

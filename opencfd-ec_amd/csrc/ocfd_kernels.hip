@@ -12,7 +12,17 @@ namespace ocfd {
 #define GRID1(n, bs) dim3((unsigned)(((n) + (bs) - 1) / (bs))), dim3(bs)
 
 // ---------------------------------------------------------------------------------------------
-// comput_duvtpc (sub_Residual.f90:1417-1456): primitives on the whole LAP box
+// get_viscous's Sutherland viscosity of temperature T (sub_Residual.f90:1355-1383); muf keeps the
+// value left after the turbulence model's mu*Re ... mu/Re round trip (sub_turbulence_SA.f90:41,262)
+__device__ __forceinline__ void visc_cell(const Blk& b, const Par& p, size_t s, double T) {
+    double Tsb = 110.4 / p.T_inf;
+    double mu = ((1.0 / p.Re) * (1.0 + Tsb)) * sqrt((T * T) * T) / (Tsb + T);
+    b.mu[s] = mu;
+    b.muf[s] = (p.turb == 2 || p.turb == 3) ? (mu * p.Re) / p.Re : mu;
+}
+
+// comput_duvtpc (sub_Residual.f90:1417-1456): primitives on the whole LAP box; for a viscous case
+// also get_viscous on the interior cells, from T in registers
 __global__ void __launch_bounds__(64) k_prims(BSet S) {
     OCFD_LOCATE(SP_BOX)
     size_t s = IX(b, i, j, k);
@@ -33,17 +43,7 @@ __global__ void __launch_bounds__(64) k_prims(BSet S) {
         b.primf[0][s] = (float)d; b.primf[1][s] = (float)u; b.primf[2][s] = (float)v; b.primf[3][s] = (float)w;
         b.primf[4][s] = (float)((p00 * d) * T); b.primf[5][s] = (float)T;
     }
-}
-
-// get_viscous (sub_Residual.f90:1355-1383); muf keeps the value left after the turbulence
-// model's mu*Re ... mu/Re round trip (sub_turbulence_SA.f90:41,262)
-__global__ void __launch_bounds__(64) k_visc(BSet S) {
-    OCFD_LOCATE(SP_CELL)
-    size_t s = IX(b, i, j, k);
-    double Tsb = 110.4 / p.T_inf, T = b.prim[5][s];
-    double mu = ((1.0 / p.Re) * (1.0 + Tsb)) * sqrt((T * T) * T) / (Tsb + T);
-    b.mu[s] = mu;
-    b.muf[s] = (p.turb == 2 || p.turb == 3) ? (mu * p.Re) / p.Re : mu;
+    if (p.visc == 1 && i >= 1 && i <= b.nx - 1 && j >= 1 && j <= b.ny - 1 && k >= 1 && k <= b.nz - 1) visc_cell(b, p, s, T);
 }
 
 // mu and muf of the first cells of each face copied into its first ghost layer; one task per block
@@ -1390,9 +1390,8 @@ static void launch_subs(void (*kern)(SubSet), const SetPlan& P, const SubList& L
 void set_pre(const SetPlan& P, hipStream_t st) {
     const Par& p = P.p0;
     hipLaunchKernelGGL(k_reset_flags, dim3((unsigned)P.nb), dim3(64), 0, st, bset(P, SP_CELL));
-    launch_space<SP_BOX>(k_prims, P, st);
+    launch_space<SP_BOX>(k_prims, P, st);      // with get_viscous on the interior cells
     if (p.visc != 1) return;
-    launch_space<SP_CELL>(k_visc, P, st);
     launch_subs(k_visc_ghost, P, P.vghost, st);
     if (p.turb == 2) {
         launch_space<SP_N0>(k_sa_mut, P, st);
